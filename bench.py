@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Throughput of the GOV-01/GOV-02 leapfrog step on one MI355X.
+
+Default workload: COLOR level, full_physics preset, 512^3, float32, cos^2 mask
+on (FROZEN), S_a Helmholtz transform every step (gamma 0.1, D 4.9); three
+seeded Gaussian blobs (one colour dominant each), amplitude 4, width 3,
+random phases, chi = 19.  A "step" is one full leapfrog step: the batched
+rocFFT S_a transform pair + k-space filter, then the fused stencil kernel.
+
+Prints one JSON line.  `value` is site updates per second of the whole step
+(GLUPS, inputs resident in HBM).  `roofline` is for the dominant kernel, the
+fused COLOR stencil, timed with HIP events on its own launch stream.
+`cpu_baseline` times the NumPy oracle (float32, one core) on a bounded
+64^3 sample of the same workload.
+
+Other workloads (--workload): complex512 (gravity_em, periodic, mask-free
+kernel), real64 (gravity_only 64^3).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lfm-physics_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "GLUPS + %HBM roofline at 512³ COLOR fp32; batched & adjoint speedup vs plain"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per site update of the stencil kernel (float32):
+# every state value read once, every new value written once.
+#   COLOR, mask: reads 6 Psi + 6 Psi_prev + chi + chi_prev + mask + 3 S_a = 72 B,
+#                writes 6 Psi' + chi' = 28 B, plus |Psi_a'|^2 (12 B) that the
+#                kernel emits as the next S_a transform input           -> 112 B
+#   COMPLEX, no mask: reads Pr, Pi, Pr_p, Pi_p, chi, chi_p; writes Pr', Pi', chi' -> 36 B
+#   REAL, mask: reads E, E_p, chi, chi_p, mask; writes E', chi'         -> 28 B
+WORKLOADS = {
+    "color512": dict(level=2, N=512, preset="full_physics", mask=True, b_alg=112,
+                     desc="COLOR full_physics 512^3 fp32, mask on, S_a FFT every step"),
+    "complex512": dict(level=1, N=512, preset="gravity_em", mask=False, b_alg=36,
+                       desc="COMPLEX gravity_em 512^3 fp32, periodic (mask-free kernel)"),
+    "real64": dict(level=0, N=64, preset="gravity_only", mask=True, b_alg=28,
+                   desc="REAL gravity_only 64^3 fp32, mask on"),
+}
+
+
+def make_config(w, N):
+    import lfm_hip as L
+    mk = {"full_physics": L.full_physics, "gravity_em": L.gravity_em,
+          "gravity_only": L.gravity_only}[w["preset"]]
+    if not w["mask"]:
+        return mk(N, boundary_type=L.BoundaryType.PERIODIC)
+    return mk(N)
+
+
+def gpu_ics(w, N, seed=42):
+    """Seeded initial fields built on the GPU (same formulas as lfm_hip.fields)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda")
+    x = torch.arange(N, dtype=torch.float32, device=dev)
+    X, Y, Z = x.view(N, 1, 1), x.view(1, N, 1), x.view(1, 1, N)
+
+    def blob(pos, amp, sig):
+        r2 = (X - pos[0]) ** 2 + (Y - pos[1]) ** 2 + (Z - pos[2]) ** 2
+        return amp * torch.exp(-r2 / (2.0 * sig ** 2))
+
+    c = N / 2.0
+    if w["level"] == 0:
+        E = blob((c, c, c), 6.0, 3.0)
+        E += torch.from_numpy((rng.standard_normal((N, N, N)) * 1e-3).astype(np.float32)).to(dev)
+        return E, None
+    if w["level"] == 1:
+        pos = rng.uniform(0, N, (8, 3))
+        amp = rng.uniform(2, 6, 8)
+        sig = rng.uniform(2, 4, 8)
+        ph = rng.uniform(0, 2 * np.pi, 8)
+        pr = torch.zeros((N, N, N), device=dev)
+        pi = torch.zeros((N, N, N), device=dev)
+        for s in range(8):
+            env = blob(pos[s], float(amp[s]), float(sig[s]))
+            pr += env * float(np.cos(ph[s]))
+            pi += env * float(np.sin(ph[s]))
+        return pr, pi
+    pr = torch.zeros((3, N, N, N), device=dev)
+    pi = torch.zeros((3, N, N, N), device=dev)
+    ph = rng.uniform(0, 2 * np.pi, 3)
+    for a in range(3):
+        # uniform in the ball of radius 0.35 * N/2 about the centre
+        while True:
+            d = rng.uniform(-1, 1, 3)
+            if d @ d <= 1:
+                break
+        pos = c + 0.35 * c * d
+        env = blob(pos, 4.0, 3.0)
+        pr[a] += env * float(np.cos(ph[a]))
+        pi[a] += env * float(np.sin(ph[a]))
+    return pr, pi
+
+
+def cpu_baseline(w, budget_s=12.0):
+    """Oracle (NumPy float32, single core) on a bounded sample: N = 64."""
+    import numpy as np
+    from oracle import lfm_oracle as O
+    import lfm_hip as L
+    N = 64 if w["level"] != 1 else 128
+    cfg = make_config(w, N)
+    rng = np.random.default_rng(42)
+    c = N / 2
+    if w["level"] == 2:
+        pos = [tuple(c + 0.35 * c * rng.uniform(-0.5, 0.5, 3)) for _ in range(3)]
+        pr, pi = L.fields.place_solitons(N, pos, 4.0, 3.0, phases=rng.uniform(0, 6.28, 3),
+                                         colors=[0, 1, 2])
+    elif w["level"] == 1:
+        pr, pi = L.fields.gaussian_soliton(N, (c, c, c), 4.0, 3.0, 0.5)
+    else:
+        pr, pi = L.fields.gaussian_soliton(N, (c, c, c), 6.0, 3.0)[0], None
+    p = dict(dt2=cfg.dt ** 2, kappa=cfg.kappa, lambda_self=cfg.lambda_self, chi0=cfg.chi0,
+             e0_sq=cfg.e0_sq, epsilon_w=cfg.epsilon_w, kappa_c=cfg.kappa_c,
+             epsilon_cc=cfg.epsilon_cc, kappa_string=cfg.kappa_string,
+             kappa_tube=cfg.kappa_tube, sa_gamma=cfg.sa_gamma, sa_d=cfg.sa_d)
+    mask = O.boundary_mask(N, cfg.boundary_fraction) if w["mask"] else np.zeros(N ** 3, np.float32)
+    st = O.OracleState(w["level"], N, pr, pi, np.full(N ** 3, 19.0), mask, p, dtype=np.float32)
+    st.step(1)
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        st.step(1)
+        steps += 1
+    dt = time.perf_counter() - t0
+    return {"value": N ** 3 * steps / dt / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
+            "sample": f"oracle NumPy float32 {w['preset']} {N}^3, {steps} steps in {dt:.1f} s "
+                      f"({steps / dt:.2f} steps/s, one thread)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="color512", choices=sorted(WORKLOADS))
+    ap.add_argument("--size", type=int, default=None, help="override the lattice size")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--direct", action="store_true", help="use the untiled kernel")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the leapfrog path here runs on a single MI355X "
+              "(multi-GPU slab decomposition is design-only); run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import lfm_hip as L
+
+    w = WORKLOADS[args.workload]
+    N = args.size or w["N"]
+    cfg = make_config(w, N)
+    policy = "reference" if w["mask"] else "boundary_type"
+    ev = L.Evolver(cfg, backend="hip", mask_policy=policy)
+    pr, pi = gpu_ics(w, N)
+    ev.set_psi_real(pr)
+    if pi is not None:
+        ev.set_psi_imag(pi)
+    del pr, pi
+    torch.cuda.synchronize()
+
+    step = (lambda: ev._step(direct=True)) if args.direct else ev._step
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+
+    # kernel-only timing: events on the launch stream around the stencil kernel
+    ev.kernel_events = []
+    nk = max(10, min(args.steps, 50))
+    for _ in range(nk):
+        step()
+    torch.cuda.synchronize()
+    kms = sum(a.elapsed_time(b) for a, b in ev.kernel_events) / len(ev.kernel_events)
+    ev.kernel_events = None
+    ok = bool(torch.isfinite(ev.native_chi()).all().item())
+
+    n3 = N ** 3
+    achieved = w["b_alg"] * n3 / (kms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tfile) and args.size is None and not args.direct:
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC, "value": n3 / (ms * 1e-3) / 1e9, "unit": "GLUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w["desc"], "grid": N, "field_level": ["REAL", "COMPLEX", "COLOR"][w["level"]],
+                   "preset": w["preset"], "mask": w["mask"], "kernel": "direct" if args.direct else "tiled"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"step_tiled<{['REAL', 'COMPLEX', 'COLOR'][w['level']]}>",
+                     "kernel_ms": kms, "bytes_per_site": w["b_alg"]},
+        "finite": ok,
+    }
+    if not args.no_cpu_baseline and args.size is None:
+        cb = cpu_baseline(w)
+        try:
+            cb["host"] = {"nproc": os.cpu_count(), "cpu": platform.processor() or
+                          subprocess.run(["sh", "-c", "grep -m1 'model name' /proc/cpuinfo"],
+                                         capture_output=True, text=True).stdout.strip()}
+        except Exception:
+            pass
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,126 @@
+/*
+ * lfm_hip.h -- C ABI of the MI355X (gfx950) GOV-01/GOV-02 leapfrog library.
+ *
+ * Plain pointers and sizes only.  Every field pointer is a caller-owned
+ * device pointer to float32 data laid out flat in C order,
+ * idx = i*N*N + j*N + k (k contiguous); colour fields are colour-major,
+ * Psi[a*N^3 + idx].  `stream` is a hipStream_t (NULL = default stream).
+ * Every call only enqueues work on `stream`; nothing synchronises the host.
+ * Every int-returning function returns 0 on success and a negative code on
+ * error; lfm_last_error() then returns a message (per calling thread).
+ *
+ * The step entry points replace the backend step methods that
+ * lfm.core.Evolver calls once per time step:
+ *   lfm_step_real_f32    <- NumpyBackend.step_real / CupyBackend.step_real
+ *                           (lfm/core/backends/numpy_backend.py:80-130,
+ *                            lfm/core/backends/cupy_backend.py:127-169)
+ *   lfm_step_complex_f32 <- step_complex (numpy_backend.py:132-205,
+ *                            cupy_backend.py:171-222)
+ *   lfm_step_color_f32   <- step_color   (numpy_backend.py:207-382,
+ *                            cupy_backend.py:224-317; the S_a transform
+ *                            that step_color runs first is lfm_scv_*)
+ */
+#ifndef LFM_HIP_H
+#define LFM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LFM_ABI_VERSION 1
+
+/* Error codes. */
+#define LFM_OK 0
+#define LFM_E_ARG (-1)     /* bad argument (null pointer, N < 8, aliasing) */
+#define LFM_E_HIP (-2)     /* HIP runtime error */
+#define LFM_E_FFT (-3)     /* rocFFT error */
+#define LFM_E_SIZE (-4)    /* size beyond a documented maximum */
+
+/* lfm_params_t.flags */
+#define LFM_F_MASK 0x01       /* mask != NULL: cos^2 absorbing blend (m in [0,1]) */
+#define LFM_F_DAMP_PREV 0x02  /* Psi prev buffers hold undamped values: the kernel
+                                 multiplies them by (1-m) when it reads them
+                                 (three-buffer rotation; see DESIGN) */
+#define LFM_F_FREEZE_CHI 0x04 /* GOV-01 only: chi is read, never written */
+#define LFM_F_DIRECT 0x08     /* force the untiled one-site-per-thread kernel */
+
+/* Scalar parameters of one step (the Evolver's per-step arguments,
+ * lfm/core/evolver.py:201-279).  Unused terms are 0. */
+typedef struct lfm_params {
+    int32_t N;
+    int32_t flags;
+    float dt2, kappa, lambda_self, chi0, e0_sq, epsilon_w;
+    float kappa_c, epsilon_cc, kappa_string, kappa_tube;
+} lfm_params_t;
+
+int lfm_abi_version(void);
+const char* lfm_last_error(void);
+
+/* REAL level (E, chi).  E_prev_out / chi_prev_out may be NULL (three-buffer
+ * rotation: the caller keeps the current buffers as the next prev). When
+ * given, E_prev_out = (1-m)*E and chi_prev_out = chi, as the reference
+ * writes them. */
+int lfm_step_real_f32(const float* E, const float* E_prev, const float* chi,
+                      const float* chi_prev, const float* mask, float* E_out,
+                      float* E_prev_out, float* chi_out, float* chi_prev_out,
+                      const lfm_params_t* p, void* stream);
+
+/* COMPLEX level (Pr, Pi, chi). */
+int lfm_step_complex_f32(const float* pr, const float* pr_prev, const float* pi,
+                         const float* pi_prev, const float* chi, const float* chi_prev,
+                         const float* mask, float* pr_out, float* pr_prev_out,
+                         float* pi_out, float* pi_prev_out, float* chi_out,
+                         float* chi_prev_out, const lfm_params_t* p, void* stream);
+
+/* COLOR level, 3 colours, Psi fields of 3*N^3 floats.
+ * sa:          NULL, or 3*N^3 S_a values (raw Helmholtz output; the kernel
+ *              clips at 0 on read) -- needed when kappa_tube > 0.
+ * sa_clip_out: NULL, or receives max(sa, 0) (the reference's sa_fields_out).
+ * psisq_out:   NULL, or receives |Psi_a'|^2 of the NEW fields (3*N^3), the
+ *              input of the next step's S_a transform. */
+int lfm_step_color_f32(const float* pr, const float* pr_prev, const float* pi,
+                       const float* pi_prev, const float* chi, const float* chi_prev,
+                       const float* mask, float* pr_out, float* pr_prev_out, float* pi_out,
+                       float* pi_prev_out, float* chi_out, float* chi_prev_out,
+                       const float* sa, float* sa_clip_out, float* psisq_out,
+                       const lfm_params_t* p, void* stream);
+
+/* S_a Helmholtz transform (numpy_backend.py:338-358, cupy_backend.py:260-279):
+ * sa_out[a] = irfftn(gamma/(gamma + D k^2) * rfftn(e[a])), k_d = 2 pi fftfreq(N),
+ * for a < ncolors, written unclipped.  A plan owns rocFFT plans and the
+ * spectrum/work buffers for one (N, ncolors); it is not thread-safe. */
+int lfm_scv_plan_create(int32_t N, int32_t ncolors, void** plan);
+int lfm_scv_plan_destroy(void* plan);
+/* e = |Psi_a|^2 given directly (ncolors*N^3). */
+int lfm_scv_apply_psisq(void* plan, const float* e, float gamma, float D, float* sa_out,
+                        void* stream);
+/* e computed from pr, pi first (into the plan's own buffer). */
+int lfm_scv_apply(void* plan, const float* pr, const float* pi, float gamma, float D,
+                  float* sa_out, void* stream);
+/* Elementwise max(x, 0) into y (n floats). */
+int lfm_clip_nonneg_f32(const float* x, float* y, int64_t n, void* stream);
+
+/* FP64 reductions (lfm/analysis/energy.py:26-135, metrics.py:29-111,
+ * structure.py:26-171).  `ncomp` Psi components of N^3 each (1 REAL,
+ * 1 COMPLEX with pi, 3 COLOR).  pi / pi_prev may be NULL (REAL).
+ * If mask != NULL and damp_prev != 0 the prev fields are read as (1-m)*prev.
+ * `partials` is device scratch of lfm_reduce_partials_len(N) doubles.
+ * out (device, 4 doubles): sum T, sum G, sum V, sum |Psi|^2. */
+int64_t lfm_reduce_partials_len(int32_t N);
+int lfm_energy_f64(const float* pr, const float* pr_prev, const float* pi,
+                   const float* pi_prev, const float* chi, const float* mask,
+                   int32_t damp_prev, int32_t ncomp, int32_t N, double dt, double c,
+                   double* partials, double* out, void* stream);
+/* chi statistics over the interior ball dist((i,j,k), N/2) <= (1-bf) N/2.
+ * out (device, 7 doubles): min, max, mean, std, well_fraction, void_fraction,
+ * interior count. */
+int lfm_chi_stats_f64(const float* chi, int32_t N, double boundary_fraction,
+                      double well_threshold, double void_threshold, double* partials,
+                      double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LFM_HIP_H */

@@ -1,0 +1,609 @@
+// GOV-01/GOV-02 leapfrog step kernels for gfx950 (MI355X).
+//
+// Two kernels compute the same step:
+//  * step_tiled  -- the production kernel.  A workgroup owns a 64 (k) x TJ (j)
+//                   column tile and marches along i (the slowest axis) over a
+//                   chunk of planes.  Each plane of every stencilled field is
+//                   read from HBM once (plus a one-cell j/k halo), staged in
+//                   LDS, and split into an in-plane partial A(q) and an
+//                   off-plane partial B(q) of the 19-point Laplacian:
+//                       A(q) = 1/3 (4 in-plane faces) + 1/6 (4 in-plane edges) - 4 f
+//                       B(q) = 1/3 f + 1/6 (4 in-plane faces)
+//                       lap(i) = A(i) + B(i-1) + B(i+1)
+//                   A register queue carries A and B from plane to plane, so
+//                   the i-direction never re-reads HBM.
+//  * step_direct -- one thread per site, 19 scattered loads per field; kept
+//                   as an independent second implementation for testing.
+//
+// Physics per site (reference: lfm/core/backends/kernel_source.py:86-233 for
+// COLOR, :377-417 REAL, :483-538 COMPLEX; numpy_backend.py:80-382):
+//   Psi' = 2 Psi - Psi_p + dt2 (lap Psi - chi^2 Psi) [- dt2 eps_cc chi^2 (Psi_a - Psi_bar)]
+//   chi' = 2 chi - chi_p + dt2 (lap chi - (kappa/chi0) chi (|Psi|^2 + eps_w J - E0^2)
+//                               - CV + chi_self - kappa_s CCV - kappa_t SCV)
+//   chi' = max(chi', -chi0); chi' = m chi0 + (1-m) chi'; Psi' *= (1-m).
+#include "lfm_internal.h"
+
+namespace lfm {
+namespace {
+
+constexpr float WF = 1.0f / 3.0f;  // face weight   (lfm/constants.py:135)
+constexpr float WE = 1.0f / 6.0f;  // edge weight   (lfm/constants.py:138)
+
+enum Level { REAL = 0, COMPLEX = 1, COLOR = 2 };
+
+// Number of real Psi component fields (Pr_0..Pr_{C-1}, Pi_0..Pi_{C-1}) and of
+// complex colours per level.
+template <int L> struct Lv;
+template <> struct Lv<REAL> { static constexpr int NPF = 1, NC = 0; };
+template <> struct Lv<COMPLEX> { static constexpr int NPF = 2, NC = 1; };
+template <> struct Lv<COLOR> { static constexpr int NPF = 6, NC = 3; };
+
+struct KParams {
+    int N, flags;
+    float dt2, kappa, lam, chi0, e0sq, epsw, kc, ecc, ks, kt;
+};
+
+struct Ptrs {
+    const float* pr;     // Psi real parts (or E), NC*N^3 (N^3 for REAL)
+    const float* pi;     // Psi imaginary parts (NULL for REAL)
+    const float* prp;    // prev
+    const float* pip;
+    const float* chi;
+    const float* chip;
+    const float* mask;   // NULL when !LFM_F_MASK
+    const float* sa;     // NULL unless COLOR with kappa_tube > 0
+    float* pr_out;
+    float* pi_out;
+    float* prp_out;      // NULL in rotation mode
+    float* pip_out;
+    float* chi_out;
+    float* chip_out;     // NULL in rotation mode
+    float* sa_clip;      // NULL or max(sa, 0)
+    float* psisq;        // NULL or |Psi'_a|^2 of the new fields
+    size_t n3;           // N^3
+};
+
+// Component field f of a level: f < NC -> real part of colour f, else the
+// imaginary part of colour f-NC (REAL: f = 0 is E).
+template <int L>
+__device__ __forceinline__ const float* field(const float* pr, const float* pi, int f,
+                                              size_t n3) {
+    constexpr int C = Lv<L>::NC;
+    if constexpr (L == REAL) return pr;
+    return f < C ? pr + f * n3 : pi + (f - C) * n3;
+}
+template <int L>
+__device__ __forceinline__ float* field_out(float* pr, float* pi, int f, size_t n3) {
+    constexpr int C = Lv<L>::NC;
+    if constexpr (L == REAL) return pr;
+    return f < C ? pr + f * n3 : pi + (f - C) * n3;
+}
+
+// Per-site update, shared by both kernels.
+//  c[f]     Psi component values at the site      lap[f]  their Laplacians
+//  jsum     sum over colours and directions of j_{a,d}   (COMPLEX/COLOR)
+//  ccv      sum_d (sum_a j_{a,d}^2 - 1/3 (sum_a j_{a,d})^2)  (COLOR)
+template <int L>
+__device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, size_t idx,
+                                            const float (&c)[Lv<L>::NPF],
+                                            const float (&lap)[Lv<L>::NPF], float chi_c,
+                                            float lap_chi, float jsum, float ccv) {
+    constexpr int NPF = Lv<L>::NPF;
+    constexpr int C = Lv<L>::NC;
+    const size_t n3 = P.n3;
+    const bool has_mask = (kp.flags & LFM_F_MASK) != 0;
+    const float m = has_mask ? P.mask[idx] : 0.0f;
+    const float a = 1.0f - m;
+    const float chi_sq = chi_c * chi_c;
+
+    float avg[2] = {0.0f, 0.0f};
+    if constexpr (L == COLOR) {
+        if (kp.ecc > 0.0f) {
+            avg[0] = (c[0] + c[1] + c[2]) * (1.0f / 3.0f);
+            avg[1] = (c[3] + c[4] + c[5]) * (1.0f / 3.0f);
+        }
+    }
+    const bool damp = has_mask && (kp.flags & LFM_F_DAMP_PREV);
+    float nw[NPF];
+#pragma unroll
+    for (int f = 0; f < NPF; ++f) {
+        float pv = field<L>(P.prp, P.pip, f, n3)[idx];
+        if (damp) pv = pv * a;
+        float v = 2.0f * c[f] - pv + kp.dt2 * (lap[f] - chi_sq * c[f]);
+        if constexpr (L == COLOR) {
+            if (kp.ecc > 0.0f) v -= kp.dt2 * kp.ecc * chi_sq * (c[f] - avg[f / C]);
+        }
+        nw[f] = v * a;
+        field_out<L>(P.pr_out, P.pi_out, f, n3)[idx] = nw[f];
+        if (P.prp_out) field_out<L>(P.prp_out, P.pip_out, f, n3)[idx] = c[f] * a;
+    }
+    if constexpr (L == COLOR) {
+        if (P.psisq) {
+#pragma unroll
+            for (int q = 0; q < C; ++q) P.psisq[q * n3 + idx] = nw[q] * nw[q] + nw[q + C] * nw[q + C];
+        }
+    }
+    if (kp.flags & LFM_F_FREEZE_CHI) return;
+
+    float psi_sq;
+    float src;
+    float cv = 0.0f, scv = 0.0f;
+    if constexpr (L == REAL) {
+        psi_sq = c[0] * c[0];
+        src = (kp.kappa / kp.chi0) * chi_c * (psi_sq - kp.e0sq);
+    } else {
+        float e[C > 0 ? C : 1];
+        psi_sq = 0.0f;
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            e[q] = c[q] * c[q] + c[q + C] * c[q + C];
+            psi_sq += e[q];
+        }
+        const float j_total = 0.5f * jsum;
+        src = (kp.kappa / kp.chi0) * chi_c * (psi_sq + kp.epsw * j_total - kp.e0sq);
+        if constexpr (L == COLOR) {
+            if (kp.kc > 0.0f) {
+                const float total_sq = psi_sq * psi_sq;
+                if (total_sq > 1e-30f) {
+                    const float sum_sq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+                    const float f_c = sum_sq / total_sq - (1.0f / 3.0f);
+                    cv = (kp.kc / kp.chi0) * chi_c * f_c * psi_sq;
+                }
+            }
+            if (P.sa) {
+                float s_sum = 0.0f, s_sq = 0.0f;
+#pragma unroll
+                for (int q = 0; q < C; ++q) {
+                    float s = P.sa[q * n3 + idx];
+                    s = s > 0.0f ? s : 0.0f;
+                    if (P.sa_clip) P.sa_clip[q * n3 + idx] = s;
+                    s_sum += s;
+                    s_sq += s * s;
+                }
+                scv = s_sq - (1.0f / 3.0f) * s_sum * s_sum;
+            }
+        }
+    }
+    const float chi_self = -4.0f * kp.lam * chi_c * (chi_sq - kp.chi0 * kp.chi0);
+    float accel = lap_chi - src - cv + chi_self;
+    if constexpr (L == COLOR) accel = accel - kp.ks * ccv - kp.kt * scv;
+    float cn = 2.0f * chi_c - P.chip[idx] + kp.dt2 * accel;
+    if (cn < -kp.chi0) cn = -kp.chi0;   // BH excision clamp (NaN passes through)
+    cn = m * kp.chi0 + a * cn;
+    P.chi_out[idx] = cn;
+    if (P.chip_out) P.chip_out[idx] = chi_c;
+}
+
+// ---------------------------------------------------------------------------
+// Direct kernel: one thread per site.
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(256) void step_direct(Ptrs P, KParams kp) {
+    constexpr int NPF = Lv<L>::NPF;
+    constexpr int C = Lv<L>::NC;
+    const int N = kp.N;
+    const size_t n3 = P.n3;
+    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= n3) return;
+    const int i = int(idx / (size_t(N) * N));
+    const int j = int((idx / N) % N);
+    const int k = int(idx % N);
+    const size_t NN = size_t(N) * N;
+    const size_t rc = size_t(i) * NN, rp = size_t((i + 1) % N) * NN, rm = size_t((i + N - 1) % N) * NN;
+    const size_t cc = size_t(j) * N, cp = size_t((j + 1) % N) * N, cm = size_t((j + N - 1) % N) * N;
+    const size_t dc = k, dp = (k + 1) % N, dm = (k + N - 1) % N;
+    // neighbour offsets: 6 faces then 12 edges
+    const size_t nb[18] = {rp + cc + dc, rm + cc + dc, rc + cp + dc, rc + cm + dc, rc + cc + dp,
+                           rc + cc + dm, rp + cp + dc, rp + cm + dc, rm + cp + dc, rm + cm + dc,
+                           rp + cc + dp, rp + cc + dm, rm + cc + dp, rm + cc + dm, rc + cp + dp,
+                           rc + cp + dm, rc + cm + dp, rc + cm + dm};
+    auto lap19 = [&](const float* F, float ctr) {
+        float fs = F[nb[0]] + F[nb[1]] + F[nb[2]] + F[nb[3]] + F[nb[4]] + F[nb[5]];
+        float es = 0.0f;
+#pragma unroll
+        for (int t = 6; t < 18; ++t) es += F[nb[t]];
+        return WF * fs + WE * es - 4.0f * ctr;
+    };
+    float c[NPF], lap[NPF];
+#pragma unroll
+    for (int f = 0; f < NPF; ++f) {
+        const float* F = field<L>(P.pr, P.pi, f, n3);
+        c[f] = F[idx];
+        lap[f] = lap19(F, c[f]);
+    }
+    const float chi_c = P.chi[idx];
+    const float lap_chi = lap19(P.chi, chi_c);
+    float jsum = 0.0f, ccv = 0.0f;
+    if constexpr (L != REAL) {
+        float jd[3][C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const float* R = field<L>(P.pr, P.pi, q, n3);
+            const float* I = field<L>(P.pr, P.pi, q + C, n3);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const size_t up = nb[2 * d], dn = nb[2 * d + 1];
+                jd[d][q] = c[q] * (I[up] - I[dn]) - c[q + C] * (R[up] - R[dn]);
+                jsum += jd[d][q];
+            }
+        }
+        if constexpr (L == COLOR) {
+            if (kp.ks > 0.0f) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const float s = jd[d][0] + jd[d][1] + jd[d][2];
+                    ccv += (jd[d][0] * jd[d][0] + jd[d][1] * jd[d][1] + jd[d][2] * jd[d][2]) -
+                           (1.0f / 3.0f) * s * s;
+                }
+            }
+        }
+    }
+    site_update<L>(kp, P, idx, c, lap, chi_c, lap_chi, jsum, ccv);
+}
+
+// ---------------------------------------------------------------------------
+// Tiled 2.5-D kernel.
+//   block = 64 x BJ threads (one wavefront per j-row group), each thread owns
+//   RJ consecutive j rows at one k column: tile TK=64 (k) x TJ=BJ*RJ (j).
+//   The grid is (k tiles) x (j tiles) x (i chunks of CH planes); the block id
+//   is remapped so that consecutive tiles of one i chunk land on the same XCD
+//   (blocks are dispatched round-robin over the 8 XCDs), which keeps the j/k
+//   halo rows shared between neighbouring tiles inside one L2.
+// ---------------------------------------------------------------------------
+constexpr int TK = 64;
+
+template <int L, int BJ, int RJ>
+__global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nkt, int njt,
+                                                       int CH) {
+    constexpr int NPF = Lv<L>::NPF;
+    constexpr int C = Lv<L>::NC;
+    constexpr int NF = NPF + 1;              // + chi
+    constexpr int TJ = BJ * RJ;
+    constexpr int LW = TK + 2;
+    constexpr int LH = TJ + 2;
+    static_assert(BJ >= 4, "halo loading uses four wavefronts");
+    __shared__ float lds[NF][LH][LW];
+
+    const int N = kp.N;
+    const size_t n3 = P.n3;
+    const size_t NN = size_t(N) * N;
+
+    // XCD-aware block remap (bijective when gridDim.x % 8 == 0).
+    const int nb = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+    const int kt = lb % nkt;
+    const int rest = lb / nkt;
+    const int jt = rest % njt;
+    const int ch = rest / njt;
+    const int k0 = kt * TK, j0 = jt * TJ;
+    const int i0 = ch * CH;
+    const int i1 = min(i0 + CH, N);
+    const int nk = min(TK, N - k0), nj = min(TJ, N - j0);
+    const int tx = threadIdx.x & 63;
+    const int ty = threadIdx.x >> 6;
+
+    const float* src[NF];
+#pragma unroll
+    for (int f = 0; f < NPF; ++f) src[f] = field<L>(P.pr, P.pi, f, n3);
+    src[NPF] = P.chi;
+
+    // Global offsets (within a plane) of this thread's loads.
+    //   interior: rows ty*RJ + rr, column tx
+    //   halo (wave 0: row -1, wave 1: row nj, wave 2: column -1, wave 3: column nk)
+    const bool kin = tx < nk;
+    int hoff = -1, hrow = 0, hcol = 0;  // LDS destination of the halo value
+    if (ty == 0 && kin) {
+        hoff = ((j0 - 1 + N) % N) * N + k0 + tx; hrow = 0; hcol = tx + 1;
+    } else if (ty == 1 && kin) {
+        hoff = ((j0 + nj) % N) * N + k0 + tx; hrow = nj + 1; hcol = tx + 1;
+    } else if (ty == 2 && tx < nj + 2) {
+        hoff = ((j0 - 1 + tx + N) % N) * N + (k0 - 1 + N) % N; hrow = tx; hcol = 0;
+    } else if (ty == 3 && tx < nj + 2) {
+        hoff = ((j0 - 1 + tx + N) % N) * N + (k0 + nk) % N; hrow = tx; hcol = nk + 1;
+    }
+    bool rin[RJ];
+#pragma unroll
+    for (int rr = 0; rr < RJ; ++rr) rin[rr] = kin && (ty * RJ + rr) < nj;
+
+    float pre[NF][RJ + 1];
+    auto prefetch = [&](int q) {
+        const size_t base = size_t((q + N) % N) * NN;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+#pragma unroll
+            for (int rr = 0; rr < RJ; ++rr)
+                pre[f][rr] = rin[rr] ? src[f][base + size_t(j0 + ty * RJ + rr) * N + k0 + tx] : 0.0f;
+            pre[f][RJ] = hoff >= 0 ? src[f][base + hoff] : 0.0f;
+        }
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+#pragma unroll
+            for (int rr = 0; rr < RJ; ++rr)
+                if (rin[rr]) lds[f][ty * RJ + rr + 1][tx + 1] = pre[f][rr];
+            if (hoff >= 0) lds[f][hrow][hcol] = pre[f][RJ];
+        }
+    };
+
+    // Register queue (per owned row).
+    float acc[RJ][NF];     // A(q-1) + B(q-2): Laplacian of the pending site minus B(q)
+    float bprev[RJ][NF];   // B(q-1)
+    float cm2[RJ][NPF];    // Psi at plane q-2 (for the i-direction current)
+    float cm1[RJ][NPF];    // Psi at plane q-1 (the pending site)
+    float chim1[RJ];       // chi at the pending site
+    float jyz[RJ];         // sum over colours of j_y + j_z at the pending site
+    float ccvyz[RJ];       // y,z part of CCV at the pending site
+#pragma unroll
+    for (int rr = 0; rr < RJ; ++rr) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) { acc[rr][f] = 0.0f; bprev[rr][f] = 0.0f; }
+#pragma unroll
+        for (int f = 0; f < NPF; ++f) { cm2[rr][f] = 0.0f; cm1[rr][f] = 0.0f; }
+        chim1[rr] = 0.0f; jyz[rr] = 0.0f; ccvyz[rr] = 0.0f;
+    }
+
+    const int cnt = (i1 - i0) + 2;  // planes i0-1 .. i1
+    prefetch(i0 - 1);
+    for (int s = 0; s < cnt; ++s) {
+        const int q = i0 - 1 + s;
+        __syncthreads();
+        stage();
+        __syncthreads();
+        if (s + 1 < cnt) prefetch(q + 1);
+#pragma unroll
+        for (int rr = 0; rr < RJ; ++rr) {
+            const int r = ty * RJ + rr + 1;  // LDS row of this site
+            const int x = tx + 1;
+            float A[NF], B[NF], cq[NF], dy[NPF], dz[NPF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                const float ctr = lds[f][r][x];
+                const float nn = lds[f][r - 1][x], ss = lds[f][r + 1][x];
+                const float ww = lds[f][r][x - 1], ee = lds[f][r][x + 1];
+                const float faces = (nn + ss) + (ww + ee);
+                const float diag = (lds[f][r - 1][x - 1] + lds[f][r - 1][x + 1]) +
+                                   (lds[f][r + 1][x - 1] + lds[f][r + 1][x + 1]);
+                A[f] = WF * faces + WE * diag - 4.0f * ctr;
+                B[f] = WF * ctr + WE * faces;
+                cq[f] = ctr;
+                if (f < NPF) { dy[f] = ss - nn; dz[f] = ee - ww; }
+            }
+            // In-plane currents (y, z) of this plane's sites.
+            float jyz_new = 0.0f, ccv_new = 0.0f;
+            if constexpr (L != REAL) {
+                float sy = 0.0f, sz = 0.0f, qy = 0.0f, qz = 0.0f;
+#pragma unroll
+                for (int a2 = 0; a2 < C; ++a2) {
+                    const float jy = cq[a2] * dy[a2 + C] - cq[a2 + C] * dy[a2];
+                    const float jz = cq[a2] * dz[a2 + C] - cq[a2 + C] * dz[a2];
+                    sy += jy; sz += jz; qy += jy * jy; qz += jz * jz;
+                }
+                jyz_new = sy + sz;
+                if constexpr (L == COLOR)
+                    ccv_new = (qy - (1.0f / 3.0f) * sy * sy) + (qz - (1.0f / 3.0f) * sz * sz);
+            }
+            if (s >= 2 && rin[rr]) {
+                // Finalise the pending site at plane q-1.
+                float lap[NPF], cpend[NPF];
+#pragma unroll
+                for (int f = 0; f < NPF; ++f) { lap[f] = acc[rr][f] + B[f]; cpend[f] = cm1[rr][f]; }
+                const float lap_chi = acc[rr][NPF] + B[NPF];
+                float jsum = 0.0f, ccv = 0.0f;
+                if constexpr (L != REAL) {
+                    float sx = 0.0f, qx = 0.0f;
+#pragma unroll
+                    for (int a2 = 0; a2 < C; ++a2) {
+                        const float jx = cm1[rr][a2] * (cq[a2 + C] - cm2[rr][a2 + C]) -
+                                         cm1[rr][a2 + C] * (cq[a2] - cm2[rr][a2]);
+                        sx += jx; qx += jx * jx;
+                    }
+                    jsum = sx + jyz[rr];
+                    if constexpr (L == COLOR) {
+                        if (kp.ks > 0.0f) ccv = ccvyz[rr] + (qx - (1.0f / 3.0f) * sx * sx);
+                    }
+                }
+                const size_t idx = size_t((q - 1 + N) % N) * NN +
+                                   size_t(j0 + ty * RJ + rr) * N + k0 + tx;
+                site_update<L>(kp, P, idx, cpend, lap, chim1[rr], lap_chi, jsum, ccv);
+            }
+            // Advance the queue.
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                acc[rr][f] = A[f] + bprev[rr][f];
+                bprev[rr][f] = B[f];
+            }
+#pragma unroll
+            for (int f = 0; f < NPF; ++f) { cm2[rr][f] = cm1[rr][f]; cm1[rr][f] = cq[f]; }
+            chim1[rr] = cq[NPF];
+            jyz[rr] = jyz_new;
+            ccvyz[rr] = ccv_new;
+        }
+    }
+}
+
+int choose_chunk(int N, int tiles) {
+    // Aim for >= 2048 workgroups (8 per CU on 256 CUs); keep >= 4 planes per
+    // chunk so the two halo planes cost at most half a chunk.
+    int nch = (2048 + tiles - 1) / tiles;
+    int maxch = N / 4 > 0 ? N / 4 : 1;
+    if (nch > maxch) nch = maxch;
+    if (nch < 1) nch = 1;
+    return (N + nch - 1) / nch;
+}
+
+template <int L, int BJ, int RJ>
+int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
+    constexpr int TJ = BJ * RJ;
+    const int N = kp.N;
+    const int nkt = (N + TK - 1) / TK;
+    const int njt = (N + TJ - 1) / TJ;
+    const int CH = choose_chunk(N, nkt * njt);
+    const int nch = (N + CH - 1) / CH;
+    const long long nblocks = (long long)nkt * njt * nch;
+    hipLaunchKernelGGL((step_tiled<L, BJ, RJ>), dim3((unsigned)nblocks), dim3(64 * BJ), 0, st, P,
+                       kp, nkt, njt, CH);
+    LFM_LAUNCH_CHECK("step_tiled");
+    return LFM_OK;
+}
+
+template <int L>
+int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
+    if (kp.flags & LFM_F_DIRECT) {
+        const unsigned nb = unsigned((P.n3 + 255) / 256);
+        hipLaunchKernelGGL((step_direct<L>), dim3(nb), dim3(256), 0, st, P, kp);
+        LFM_LAUNCH_CHECK("step_direct");
+        return LFM_OK;
+    }
+    if constexpr (L == COLOR) return launch_tiled<L, 4, 2>(P, kp, st);
+    else return launch_tiled<L, 4, 4>(P, kp, st);
+}
+
+KParams to_k(const lfm_params_t* p) {
+    KParams k;
+    k.N = p->N; k.flags = p->flags; k.dt2 = p->dt2; k.kappa = p->kappa; k.lam = p->lambda_self;
+    k.chi0 = p->chi0; k.e0sq = p->e0_sq; k.epsw = p->epsilon_w; k.kc = p->kappa_c;
+    k.ecc = p->epsilon_cc; k.ks = p->kappa_string; k.kt = p->kappa_tube;
+    return k;
+}
+
+// Argument validation common to all levels.  `ins`/`outs` list (pointer,
+// float count) pairs; NULL outputs are skipped.
+int validate(const lfm_params_t* p, const float* const* ins, const size_t* nin, int ni,
+             float* const* outs, const size_t* nout, int no, const char* who) {
+    if (!p) { set_error("%s: params is NULL", who); return LFM_E_ARG; }
+    if (p->N < 8 || p->N > 1290) {
+        set_error("%s: N=%d out of range [8, 1290]", who, p->N);
+        return LFM_E_ARG;
+    }
+    for (int a = 0; a < no; ++a) {
+        if (!outs[a]) continue;
+        for (int b = 0; b < ni; ++b)
+            if (overlaps(outs[a], nout[a] * 4, ins[b], nin[b] * 4)) {
+                set_error("%s: output %d overlaps input %d (in-place update is not allowed)",
+                          who, a, b);
+                return LFM_E_ARG;
+            }
+        for (int b = a + 1; b < no; ++b)
+            if (overlaps(outs[a], nout[a] * 4, outs[b], nout[b] * 4)) {
+                set_error("%s: outputs %d and %d overlap", who, a, b);
+                return LFM_E_ARG;
+            }
+    }
+    return LFM_OK;
+}
+
+}  // namespace
+}  // namespace lfm
+
+using namespace lfm;
+
+extern "C" int lfm_step_real_f32(const float* E, const float* E_prev, const float* chi,
+                                 const float* chi_prev, const float* mask, float* E_out,
+                                 float* E_prev_out, float* chi_out, float* chi_prev_out,
+                                 const lfm_params_t* p, void* stream) {
+    if (!E || !E_prev || !chi || !chi_prev || !E_out || !chi_out || !p) {
+        set_error("lfm_step_real_f32: required pointer is NULL");
+        return LFM_E_ARG;
+    }
+    if ((p->flags & LFM_F_MASK) && !mask) {
+        set_error("lfm_step_real_f32: LFM_F_MASK set but mask is NULL");
+        return LFM_E_ARG;
+    }
+    const size_t n3 = size_t(p->N) * p->N * p->N;
+    const float* ins[] = {E, E_prev, chi, chi_prev, mask};
+    const size_t nin[] = {n3, n3, n3, n3, n3};
+    float* outs[] = {E_out, E_prev_out, chi_out, chi_prev_out};
+    const size_t nout[] = {n3, n3, n3, n3};
+    int rc = validate(p, ins, nin, 5, outs, nout, 4, "lfm_step_real_f32");
+    if (rc) return rc;
+    Ptrs P{};
+    P.pr = E; P.prp = E_prev; P.chi = chi; P.chip = chi_prev;
+    P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
+    P.pr_out = E_out; P.prp_out = E_prev_out; P.chi_out = chi_out; P.chip_out = chi_prev_out;
+    P.n3 = n3;
+    return launch<REAL>(P, to_k(p), (hipStream_t)stream);
+}
+
+extern "C" int lfm_step_complex_f32(const float* pr, const float* pr_prev, const float* pi,
+                                    const float* pi_prev, const float* chi,
+                                    const float* chi_prev, const float* mask, float* pr_out,
+                                    float* pr_prev_out, float* pi_out, float* pi_prev_out,
+                                    float* chi_out, float* chi_prev_out, const lfm_params_t* p,
+                                    void* stream) {
+    if (!pr || !pr_prev || !pi || !pi_prev || !chi || !chi_prev || !pr_out || !pi_out ||
+        !chi_out || !p) {
+        set_error("lfm_step_complex_f32: required pointer is NULL");
+        return LFM_E_ARG;
+    }
+    if ((p->flags & LFM_F_MASK) && !mask) {
+        set_error("lfm_step_complex_f32: LFM_F_MASK set but mask is NULL");
+        return LFM_E_ARG;
+    }
+    if ((pr_prev_out == nullptr) != (pi_prev_out == nullptr)) {
+        set_error("lfm_step_complex_f32: pr_prev_out and pi_prev_out must both be given or both NULL");
+        return LFM_E_ARG;
+    }
+    const size_t n3 = size_t(p->N) * p->N * p->N;
+    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask};
+    const size_t nin[] = {n3, n3, n3, n3, n3, n3, n3};
+    float* outs[] = {pr_out, pr_prev_out, pi_out, pi_prev_out, chi_out, chi_prev_out};
+    const size_t nout[] = {n3, n3, n3, n3, n3, n3};
+    int rc = validate(p, ins, nin, 7, outs, nout, 6, "lfm_step_complex_f32");
+    if (rc) return rc;
+    Ptrs P{};
+    P.pr = pr; P.pi = pi; P.prp = pr_prev; P.pip = pi_prev; P.chi = chi; P.chip = chi_prev;
+    P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
+    P.pr_out = pr_out; P.pi_out = pi_out; P.prp_out = pr_prev_out; P.pip_out = pi_prev_out;
+    P.chi_out = chi_out; P.chip_out = chi_prev_out;
+    P.n3 = n3;
+    return launch<COMPLEX>(P, to_k(p), (hipStream_t)stream);
+}
+
+extern "C" int lfm_step_color_f32(const float* pr, const float* pr_prev, const float* pi,
+                                  const float* pi_prev, const float* chi, const float* chi_prev,
+                                  const float* mask, float* pr_out, float* pr_prev_out,
+                                  float* pi_out, float* pi_prev_out, float* chi_out,
+                                  float* chi_prev_out, const float* sa, float* sa_clip_out,
+                                  float* psisq_out, const lfm_params_t* p, void* stream) {
+    if (!pr || !pr_prev || !pi || !pi_prev || !chi || !chi_prev || !pr_out || !pi_out ||
+        !chi_out || !p) {
+        set_error("lfm_step_color_f32: required pointer is NULL");
+        return LFM_E_ARG;
+    }
+    if ((p->flags & LFM_F_MASK) && !mask) {
+        set_error("lfm_step_color_f32: LFM_F_MASK set but mask is NULL");
+        return LFM_E_ARG;
+    }
+    if ((pr_prev_out == nullptr) != (pi_prev_out == nullptr)) {
+        set_error("lfm_step_color_f32: pr_prev_out and pi_prev_out must both be given or both NULL");
+        return LFM_E_ARG;
+    }
+    if (p->kappa_tube > 0.0f && !sa) {
+        set_error("lfm_step_color_f32: kappa_tube > 0 needs the S_a fields (sa is NULL)");
+        return LFM_E_ARG;
+    }
+    if (sa_clip_out && !sa) {
+        set_error("lfm_step_color_f32: sa_clip_out given without sa");
+        return LFM_E_ARG;
+    }
+    const size_t n3 = size_t(p->N) * p->N * p->N;
+    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask, sa};
+    const size_t nin[] = {3 * n3, 3 * n3, 3 * n3, 3 * n3, n3, n3, n3, 3 * n3};
+    float* outs[] = {pr_out, pr_prev_out, pi_out, pi_prev_out, chi_out, chi_prev_out,
+                     sa_clip_out, psisq_out};
+    const size_t nout[] = {3 * n3, 3 * n3, 3 * n3, 3 * n3, n3, n3, 3 * n3, 3 * n3};
+    int rc = validate(p, ins, nin, 8, outs, nout, 8, "lfm_step_color_f32");
+    if (rc) return rc;
+    Ptrs P{};
+    P.pr = pr; P.pi = pi; P.prp = pr_prev; P.pip = pi_prev; P.chi = chi; P.chip = chi_prev;
+    P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
+    P.sa = p->kappa_tube > 0.0f ? sa : nullptr;
+    P.pr_out = pr_out; P.pi_out = pi_out; P.prp_out = pr_prev_out; P.pip_out = pi_prev_out;
+    P.chi_out = chi_out; P.chip_out = chi_prev_out;
+    P.sa_clip = P.sa ? sa_clip_out : nullptr;
+    P.psisq = psisq_out;
+    P.n3 = n3;
+    return launch<COLOR>(P, to_k(p), (hipStream_t)stream);
+}

@@ -1,0 +1,210 @@
+"""``HipBackend``: the MI355X backend object for the reference's backend slot.
+
+It has the method set ``lfm.core.Evolver`` calls on a backend
+(lfm/core/backends/numpy_backend.py:21-388, cupy_backend.py:48-363):
+``name``, ``allocate``, ``create_boundary_mask``, ``step_real``,
+``step_complex``, ``step_color``, ``to_numpy``, ``from_numpy``,
+``synchronize``.  Arrays are flat float32 torch tensors on the GPU; every
+step enqueues hand-written HIP kernels through liblfm_hip's C ABI on torch's
+current stream and returns without synchronising, like the CuPy backend.
+
+The step methods honour the reference's four-output contract exactly:
+``psi_out = (1-m) Psi'``, ``psi_prev_out = (1-m) Psi``,
+``chi_out = m chi0 + (1-m) max(chi', -chi0)``, ``chi_prev_out = chi``.
+There is no CPU path: without a GPU or without the built library the
+constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _abi
+from .fields import boundary_mask_host
+
+
+def _ptr(t) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def _require(t, n: int, what: str) -> None:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what}: expected a torch.Tensor on the GPU, got {type(t).__name__}")
+    if t.device.type != "cuda" or t.dtype != torch.float32 or not t.is_contiguous():
+        raise ValueError(f"{what}: need a contiguous float32 GPU tensor "
+                         f"(got {t.dtype} on {t.device}, contiguous={t.is_contiguous()})")
+    if t.numel() < n:
+        raise ValueError(f"{what}: has {t.numel()} elements, needs {n}")
+
+
+def params(N, dt2, kappa, lambda_self, chi0, e0_sq, epsilon_w=0.0, kappa_c=0.0,
+           epsilon_cc=0.0, kappa_string=0.0, kappa_tube=0.0, flags=0) -> _abi.LfmParams:
+    """lfm_params_t with every scalar rounded to float32, as the CuPy backend
+    passes them (cupy_backend.py:160-165,213-219,298-309)."""
+    return _abi.LfmParams(int(N), int(flags), float(dt2), float(kappa), float(lambda_self),
+                          float(chi0), float(e0_sq), float(epsilon_w), float(kappa_c),
+                          float(epsilon_cc), float(kappa_string), float(kappa_tube))
+
+
+def stream_handle() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+class ScvPlan:
+    """Owner of one lfm_scv plan (batched rocFFT R2C/C2R for ncolors fields)."""
+
+    def __init__(self, N: int, ncolors: int = 3):
+        self._lib = _abi.load()
+        h = ctypes.c_void_p()
+        _abi.check(self._lib.lfm_scv_plan_create(N, ncolors, ctypes.byref(h)),
+                   "lfm_scv_plan_create")
+        self.handle = h
+        self.N, self.nc = N, ncolors
+
+    def apply(self, pr, pi, gamma, D, sa_out):
+        _abi.check(self._lib.lfm_scv_apply(self.handle, pr.data_ptr(), pi.data_ptr(),
+                                           float(gamma), float(D), sa_out.data_ptr(),
+                                           stream_handle()), "lfm_scv_apply")
+
+    def apply_psisq(self, e, gamma, D, sa_out):
+        _abi.check(self._lib.lfm_scv_apply_psisq(self.handle, e.data_ptr(), float(gamma),
+                                                 float(D), sa_out.data_ptr(), stream_handle()),
+                   "lfm_scv_apply_psisq")
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                self._lib.lfm_scv_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+class HipBackend:
+    """GPU compute backend on hand-written gfx950 kernels (see module doc)."""
+
+    def __init__(self, device: int | str | None = None) -> None:
+        if not torch.cuda.is_available():
+            raise RuntimeError("HipBackend needs a ROCm GPU; torch.cuda.is_available() is False")
+        self._lib = _abi.load()
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None
+                                   else torch.device(device).index or 0)
+        self._scv: dict[int, ScvPlan] = {}
+        self._sa_raw: dict[int, torch.Tensor] = {}
+
+    @property
+    def name(self) -> str:
+        return "hip"
+
+    # -- memory --------------------------------------------------------------
+    def allocate(self, N: int, n_psi_arrays: int, chi0: float) -> dict[str, torch.Tensor]:
+        n = N ** 3
+        z = lambda: torch.zeros(n_psi_arrays * n, dtype=torch.float32, device=self.device)
+        c = lambda: torch.full((n,), float(chi0), dtype=torch.float32, device=self.device)
+        return {"psi_A": z(), "psi_prev_A": z(), "chi_A": c(), "chi_prev_A": c(),
+                "psi_B": z(), "psi_prev_B": z(), "chi_B": c(), "chi_prev_B": c()}
+
+    def create_boundary_mask(self, N: int, boundary_fraction: float) -> torch.Tensor:
+        return self.from_numpy(boundary_mask_host(N, boundary_fraction))
+
+    def to_numpy(self, arr) -> np.ndarray:
+        return arr.detach().to("cpu").numpy()
+
+    def from_numpy(self, arr) -> torch.Tensor:
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.float32))
+        return torch.from_numpy(a).to(self.device)
+
+    def synchronize(self) -> None:
+        torch.cuda.synchronize(self.device)
+
+    # -- steps -----------------------------------------------------------------
+    def step_real(self, psi_in, psi_prev_in, chi_in, chi_prev_in, boundary_mask, psi_out,
+                  psi_prev_out, chi_out, chi_prev_out, N, dt2, kappa, lambda_self, chi0,
+                  e0_sq) -> None:
+        n = N ** 3
+        for t, w in ((psi_in, "psi_in"), (psi_prev_in, "psi_prev_in"), (chi_in, "chi_in"),
+                     (chi_prev_in, "chi_prev_in"), (psi_out, "psi_out"),
+                     (psi_prev_out, "psi_prev_out"), (chi_out, "chi_out"),
+                     (chi_prev_out, "chi_prev_out")):
+            _require(t, n, w)
+        flags = 0
+        if boundary_mask is not None:
+            _require(boundary_mask, n, "boundary_mask")
+            flags |= _abi.LFM_F_MASK
+        p = params(N, dt2, kappa, lambda_self, chi0, e0_sq, flags=flags)
+        _abi.check(self._lib.lfm_step_real_f32(
+            _ptr(psi_in), _ptr(psi_prev_in), _ptr(chi_in), _ptr(chi_prev_in), _ptr(boundary_mask),
+            _ptr(psi_out), _ptr(psi_prev_out), _ptr(chi_out), _ptr(chi_prev_out),
+            ctypes.byref(p), stream_handle()), "lfm_step_real_f32")
+
+    def step_complex(self, psi_r_in, psi_r_prev_in, psi_i_in, psi_i_prev_in, chi_in,
+                     chi_prev_in, boundary_mask, psi_r_out, psi_r_prev_out, psi_i_out,
+                     psi_i_prev_out, chi_out, chi_prev_out, N, dt2, kappa, lambda_self, chi0,
+                     e0_sq, epsilon_w) -> None:
+        n = N ** 3
+        for t, w in ((psi_r_in, "psi_r_in"), (psi_r_prev_in, "psi_r_prev_in"),
+                     (psi_i_in, "psi_i_in"), (psi_i_prev_in, "psi_i_prev_in"),
+                     (chi_in, "chi_in"), (chi_prev_in, "chi_prev_in"),
+                     (psi_r_out, "psi_r_out"), (psi_r_prev_out, "psi_r_prev_out"),
+                     (psi_i_out, "psi_i_out"), (psi_i_prev_out, "psi_i_prev_out"),
+                     (chi_out, "chi_out"), (chi_prev_out, "chi_prev_out")):
+            _require(t, n, w)
+        flags = 0
+        if boundary_mask is not None:
+            _require(boundary_mask, n, "boundary_mask")
+            flags |= _abi.LFM_F_MASK
+        p = params(N, dt2, kappa, lambda_self, chi0, e0_sq, epsilon_w, flags=flags)
+        _abi.check(self._lib.lfm_step_complex_f32(
+            _ptr(psi_r_in), _ptr(psi_r_prev_in), _ptr(psi_i_in), _ptr(psi_i_prev_in),
+            _ptr(chi_in), _ptr(chi_prev_in), _ptr(boundary_mask), _ptr(psi_r_out),
+            _ptr(psi_r_prev_out), _ptr(psi_i_out), _ptr(psi_i_prev_out), _ptr(chi_out),
+            _ptr(chi_prev_out), ctypes.byref(p), stream_handle()), "lfm_step_complex_f32")
+
+    def scv_plan(self, N: int) -> ScvPlan:
+        if N not in self._scv:
+            self._scv[N] = ScvPlan(N, 3)
+        return self._scv[N]
+
+    def step_color(self, psi_r_in, psi_r_prev_in, psi_i_in, psi_i_prev_in, chi_in, chi_prev_in,
+                   boundary_mask, psi_r_out, psi_r_prev_out, psi_i_out, psi_i_prev_out,
+                   chi_out, chi_prev_out, N, dt2, kappa, lambda_self, chi0, e0_sq, epsilon_w,
+                   kappa_c=0.0, epsilon_cc=0.0, kappa_string=0.0, kappa_tube=0.0,
+                   sa_fields_in=None, sa_fields_out=None, sa_gamma=0.1, sa_d=4.9,
+                   dt=0.02) -> None:
+        """3-colour step.  When kappa_tube > 0 and ``sa_fields_out`` is given,
+        S_a is recomputed from the input Psi (numpy_backend.py:341,
+        cupy_backend.py:261) and its clipped value lands in ``sa_fields_out``;
+        ``sa_fields_in`` is never read, as in the reference."""
+        n = N ** 3
+        for t, w in ((psi_r_in, "psi_r_in"), (psi_r_prev_in, "psi_r_prev_in"),
+                     (psi_i_in, "psi_i_in"), (psi_i_prev_in, "psi_i_prev_in"),
+                     (psi_r_out, "psi_r_out"), (psi_r_prev_out, "psi_r_prev_out"),
+                     (psi_i_out, "psi_i_out"), (psi_i_prev_out, "psi_i_prev_out")):
+            _require(t, 3 * n, w)
+        for t, w in ((chi_in, "chi_in"), (chi_prev_in, "chi_prev_in"), (chi_out, "chi_out"),
+                     (chi_prev_out, "chi_prev_out")):
+            _require(t, n, w)
+        flags = 0
+        if boundary_mask is not None:
+            _require(boundary_mask, n, "boundary_mask")
+            flags |= _abi.LFM_F_MASK
+        sa = None
+        if kappa_tube > 0 and sa_fields_out is not None:
+            _require(sa_fields_out, 3 * n, "sa_fields_out")
+            sa = self._sa_raw.get(N)
+            if sa is None:
+                sa = self._sa_raw[N] = torch.empty(3 * n, dtype=torch.float32, device=self.device)
+            self.scv_plan(N).apply(psi_r_in, psi_i_in, sa_gamma, sa_d, sa)
+        else:
+            kappa_tube = 0.0   # no S_a fields: the SCV term is zero, as in the reference
+        p = params(N, dt2, kappa, lambda_self, chi0, e0_sq, epsilon_w, kappa_c, epsilon_cc,
+                   kappa_string, kappa_tube, flags=flags)
+        _abi.check(self._lib.lfm_step_color_f32(
+            _ptr(psi_r_in), _ptr(psi_r_prev_in), _ptr(psi_i_in), _ptr(psi_i_prev_in),
+            _ptr(chi_in), _ptr(chi_prev_in), _ptr(boundary_mask), _ptr(psi_r_out),
+            _ptr(psi_r_prev_out), _ptr(psi_i_out), _ptr(psi_i_prev_out), _ptr(chi_out),
+            _ptr(chi_prev_out), _ptr(sa), _ptr(sa_fields_out) if sa is not None else None,
+            None, ctypes.byref(p), stream_handle()), "lfm_step_color_f32")

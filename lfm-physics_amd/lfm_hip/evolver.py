@@ -1,0 +1,380 @@
+"""``Evolver``: device-resident leapfrog state and the step loop.
+
+Same public surface and observable semantics as lfm/core/evolver.py:42-536
+(getters, setters, ``evolve(steps, callback, freeze_chi)``, ``step``), but
+the buffers are arranged for the GPU:
+
+* **Three-buffer rotation.**  The reference keeps A/B sets and writes four
+  outputs per step (new Psi, damped prev, new chi, chi prev).  Here each field
+  has a pool of three slots; a step writes only the new Psi and chi into the
+  free slot and the old current slot becomes prev.  The reference's
+  ``psi_prev_out = (1-m) Psi`` is reproduced exactly by multiplying prev by
+  (1-m) when the kernel reads it (the mask does not change in time, and the
+  float32 product is the same single rounding).  The first step after a
+  ``set_psi_*`` call reads prev undamped, as the reference does.  This saves
+  1 (REAL), 2 (COMPLEX) or 6 (COLOR) float writes plus one chi write per site.
+* **chi references.**  The reference's four chi buffers (current-set chi and
+  prev, other-set chi and prev) are four references into the three-slot pool,
+  so ``freeze_chi`` alternation and ``set_chi_current`` / ``set_chi_prev``
+  behave exactly as in the reference (evolver.py:160-176,467-505).
+* **S_a.**  S_a carries no state (it is recomputed from the current Psi every
+  step, numpy_backend.py:341), so one raw buffer is kept; the step kernel
+  clips it at 0 where it is used and the getter clips on read.  The step
+  kernel also emits |Psi_a'|^2 of the new fields, which is the next step's
+  transform input, so the S_a pipeline never re-reads Psi.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _abi
+from .backend import HipBackend, ScvPlan, params, stream_handle
+from .config import BoundaryType, FieldLevel, SimulationConfig
+
+
+def get_backend(preference="auto"):
+    """Backend registry for this package: 'auto', 'gpu' and 'hip' give a
+    :class:`HipBackend`; 'cpu' is refused (no CPU path ships here)."""
+    if isinstance(preference, HipBackend):
+        return preference
+    pref = str(preference).lower()
+    if pref in ("auto", "gpu", "hip"):
+        return HipBackend()
+    if pref == "cpu":
+        raise ValueError("lfm_hip has no CPU backend; use the reference lfm package for CPU runs")
+    raise ValueError(f"Unknown backend preference '{preference}'. Use 'auto', 'gpu' or 'hip'.")
+
+
+class Evolver:
+    """Double-buffer-equivalent leapfrog evolver on the GPU.
+
+    mask_policy: "reference" (default) applies the cos^2 mask whatever the
+    boundary type, as lfm/core/evolver.py:126 does; "boundary_type" drops it
+    for BoundaryType.PERIODIC (truly periodic, mask-free kernel).
+    """
+
+    def __init__(self, config: SimulationConfig, backend="auto", mask_policy="reference"):
+        self.config = config
+        self.backend = get_backend(backend)
+        self._lib = _abi.load()
+        self.N = config.grid_size
+        self.total = self.N ** 3
+        self.step = 0
+        lvl = config.field_level
+        if lvl == FieldLevel.COLOR and config.n_colors != 3:
+            raise NotImplementedError(
+                f"n_colors={config.n_colors}: the COLOR step is defined for 3 colours "
+                "(lfm/core/backends/numpy_backend.py:240 hard-wires 3 as well)")
+        self.level = int(lvl)
+        self._n_psi = config.n_colors if lvl == FieldLevel.COLOR else 1
+        self._has_imag = lvl != FieldLevel.REAL
+        self._dt2 = config.dt ** 2
+        dev = self.backend.device
+        self.device = dev
+        n = self._n_psi * self.total
+        f32 = torch.float32
+        self._pr = [torch.zeros(n, dtype=f32, device=dev) for _ in range(3)]
+        self._pi = [torch.zeros(n, dtype=f32, device=dev) for _ in range(3)] if self._has_imag else None
+        self._chi = [torch.full((self.total,), float(config.chi0), dtype=f32, device=dev)
+                     for _ in range(3)]
+        self._cur, self._prev = 0, 1          # Psi slots
+        self._damped = False                   # Psi prev needs (1-m) on read
+        self._cc, self._cp, self._oc, self._op = 0, 1, 0, 1   # chi references
+        if mask_policy not in ("reference", "boundary_type"):
+            raise ValueError(f"mask_policy must be 'reference' or 'boundary_type', got {mask_policy!r}")
+        self.mask_policy = mask_policy
+        self.has_mask = not (mask_policy == "boundary_type"
+                             and config.boundary_type == BoundaryType.PERIODIC)
+        self.boundary_mask = self.backend.create_boundary_mask(self.N, config.boundary_fraction)
+        self._absorb = 1.0 - self.boundary_mask
+        self.sa_enabled = bool(config.sa_enabled and lvl == FieldLevel.COLOR)
+        if self.sa_enabled:
+            self._sa = torch.zeros(3 * self.total, dtype=f32, device=dev)
+            self._psisq = torch.empty(3 * self.total, dtype=f32, device=dev)
+            self._psisq_valid = False
+            self._scv = ScvPlan(self.N, 3)
+        else:
+            self._sa = None
+        self._params = {}
+        self._partials = None
+        self._red_out = None
+        # Optional list: when set, each step appends a (start, end) event pair
+        # recorded on the launch stream around the stencil kernel alone.
+        self.kernel_events: list | None = None
+
+    # ------------------------------------------------------------------ steps
+    def _p(self, flags):
+        p = self._params.get(flags)
+        if p is None:
+            c = self.config
+            p = params(self.N, self._dt2, c.kappa, c.lambda_self, c.chi0, c.e0_sq, c.epsilon_w,
+                       c.kappa_c, c.epsilon_cc, c.kappa_string,
+                       c.kappa_tube if self.sa_enabled else 0.0, flags)
+            self._params[flags] = p
+        return p
+
+    def _step(self, freeze_chi: bool = False, direct: bool = False) -> None:
+        lib = self._lib
+        cur, prev = self._cur, self._prev
+        nxt = 3 - cur - prev
+        flags = 0
+        if self.has_mask:
+            flags |= _abi.LFM_F_MASK
+            if self._damped:
+                flags |= _abi.LFM_F_DAMP_PREV
+        if freeze_chi:
+            flags |= _abi.LFM_F_FREEZE_CHI
+        if direct:
+            flags |= _abi.LFM_F_DIRECT
+        p = ctypes.byref(self._p(flags))
+        st = stream_handle()
+        cc, cp = self._cc, self._cp
+        cn = next(s for s in range(3) if s != cc and s != cp)
+        chi_in, chip_in = self._chi[cc].data_ptr(), self._chi[cp].data_ptr()
+        chi_out = self._chi[cn].data_ptr()
+        mask = self.boundary_mask.data_ptr() if self.has_mask else None
+        pr, pi = self._pr, self._pi
+        if self.level < 2:
+            ev0 = self._mark()
+        if self.level == 0:
+            rc = lib.lfm_step_real_f32(pr[cur].data_ptr(), pr[prev].data_ptr(), chi_in, chip_in,
+                                       mask, pr[nxt].data_ptr(), None, chi_out, None, p, st)
+        elif self.level == 1:
+            rc = lib.lfm_step_complex_f32(pr[cur].data_ptr(), pr[prev].data_ptr(),
+                                          pi[cur].data_ptr(), pi[prev].data_ptr(), chi_in,
+                                          chip_in, mask, pr[nxt].data_ptr(), None,
+                                          pi[nxt].data_ptr(), None, chi_out, None, p, st)
+        else:
+            sa = psisq = None
+            if self.sa_enabled:
+                if self._psisq_valid:
+                    self._scv.apply_psisq(self._psisq, self.config.sa_gamma, self.config.sa_d,
+                                          self._sa)
+                else:
+                    self._scv.apply(pr[cur], pi[cur], self.config.sa_gamma, self.config.sa_d,
+                                    self._sa)
+                sa, psisq = self._sa.data_ptr(), self._psisq.data_ptr()
+            ev0 = self._mark()
+            rc = lib.lfm_step_color_f32(pr[cur].data_ptr(), pr[prev].data_ptr(),
+                                        pi[cur].data_ptr(), pi[prev].data_ptr(), chi_in, chip_in,
+                                        mask, pr[nxt].data_ptr(), None, pi[nxt].data_ptr(), None,
+                                        chi_out, None, sa, None, psisq, p, st)
+            if self.sa_enabled:
+                self._psisq_valid = True
+        self._mark(ev0)
+        _abi.check(rc, "step")
+        self._prev, self._cur = cur, nxt
+        self._damped = True
+        if freeze_chi:
+            # chi untouched; the reference's A/B sets swap roles
+            self._cc, self._cp, self._oc, self._op = self._oc, self._op, cc, cp
+        else:
+            self._cc, self._cp, self._oc, self._op = cn, cc, cc, cp
+
+    def _mark(self, start=None):
+        """Record a timing event on the current stream (kernel_events only)."""
+        if self.kernel_events is None:
+            return None
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        if start is not None:
+            self.kernel_events.append((start, e))
+        return e
+
+    def evolve(self, steps: int, callback=None, freeze_chi: bool = False) -> None:
+        """Run `steps` leapfrog steps; callback(evolver, step) every
+        config.report_interval steps (lfm/core/evolver.py:137-180)."""
+        report = self.config.report_interval
+        for _ in range(int(steps)):
+            self._step(freeze_chi=freeze_chi)
+            self.step += 1
+            if callback is not None and report > 0 and self.step % report == 0:
+                callback(self, self.step)
+
+    # -------------------------------------------------------- device views
+    def native_psi_real(self) -> torch.Tensor:
+        return self._pr[self._cur]
+
+    def native_psi_imag(self) -> torch.Tensor | None:
+        return self._pi[self._cur] if self._has_imag else None
+
+    def native_chi(self) -> torch.Tensor:
+        return self._chi[self._cc]
+
+    def native_chi_prev(self) -> torch.Tensor:
+        return self._chi[self._cp]
+
+    def _prev_view(self, pool) -> torch.Tensor:
+        t = pool[self._prev]
+        if self._damped and self.has_mask:
+            return t * self._absorb.repeat(self._n_psi) if self._n_psi > 1 else t * self._absorb
+        return t
+
+    # ------------------------------------------------------------- getters
+    def _shape(self, flat: np.ndarray) -> np.ndarray:
+        N = self.N
+        if self.level == 2:
+            return flat.reshape(self._n_psi, N, N, N)
+        return flat.reshape(N, N, N)
+
+    def get_chi(self) -> np.ndarray:
+        return self._chi[self._cc].cpu().numpy().reshape(self.N, self.N, self.N)
+
+    def get_chi_prev(self) -> np.ndarray:
+        return self._chi[self._cp].cpu().numpy().reshape(self.N, self.N, self.N)
+
+    def get_psi_real(self) -> np.ndarray:
+        return self._shape(self._pr[self._cur].cpu().numpy())
+
+    def get_psi_imag(self) -> np.ndarray | None:
+        return self._shape(self._pi[self._cur].cpu().numpy()) if self._has_imag else None
+
+    def get_psi_real_prev(self) -> np.ndarray:
+        return self._shape(self._prev_view(self._pr).cpu().numpy())
+
+    def get_psi_imag_prev(self) -> np.ndarray | None:
+        return self._shape(self._prev_view(self._pi).cpu().numpy()) if self._has_imag else None
+
+    def get_energy_density(self) -> np.ndarray:
+        pr = self.get_psi_real()
+        e2 = np.sum(pr ** 2, axis=0) if pr.ndim == 4 else pr ** 2
+        pi = self.get_psi_imag()
+        if pi is not None:
+            e2 += np.sum(pi ** 2, axis=0) if pi.ndim == 4 else pi ** 2
+        return e2
+
+    def get_sa_fields(self) -> np.ndarray | None:
+        if self._sa is None:
+            return None
+        return torch.clamp_min(self._sa, 0.0).cpu().numpy().reshape(3, self.N, self.N, self.N)
+
+    # ------------------------------------------------------------- setters
+    def _dev(self, arr) -> torch.Tensor:
+        if isinstance(arr, torch.Tensor):
+            return arr.detach().to(device=self.device, dtype=torch.float32).reshape(-1).contiguous()
+        flat = np.ascontiguousarray(np.asarray(arr, dtype=np.float32).ravel())
+        return torch.from_numpy(flat).to(self.device)
+
+    def _undamp(self) -> None:
+        """Store the damped prev values explicitly (before a partial write)."""
+        if self._damped and self.has_mask:
+            a = self._absorb.repeat(self._n_psi) if self._n_psi > 1 else self._absorb
+            self._pr[self._prev].mul_(a)
+            if self._has_imag:
+                self._pi[self._prev].mul_(a)
+        self._damped = False
+
+    def _touch_psi(self) -> None:
+        if self.sa_enabled:
+            self._psisq_valid = False
+
+    def set_psi_real(self, arr) -> None:
+        d = self._dev(arr)
+        self._undamp()
+        self._pr[self._cur].copy_(d)
+        self._pr[self._prev].copy_(d)
+        self._touch_psi()
+
+    def set_psi_imag(self, arr) -> None:
+        if not self._has_imag:
+            raise ValueError("Cannot set imaginary part for REAL field level")
+        d = self._dev(arr)
+        self._undamp()
+        self._pi[self._cur].copy_(d)
+        self._pi[self._prev].copy_(d)
+        self._touch_psi()
+
+    def set_psi_real_prev(self, arr) -> None:
+        d = self._dev(arr)
+        self._undamp()
+        self._pr[self._prev].copy_(d)
+
+    def set_psi_imag_prev(self, arr) -> None:
+        if not self._has_imag:
+            raise ValueError("Cannot set imaginary part for REAL field level")
+        d = self._dev(arr)
+        self._undamp()
+        self._pi[self._prev].copy_(d)
+
+    def set_psi_real_current(self, arr) -> None:
+        self._pr[self._cur].copy_(self._dev(arr))
+        self._touch_psi()
+
+    def set_psi_imag_current(self, arr) -> None:
+        if not self._has_imag:
+            raise ValueError("Cannot set imaginary part for REAL field level")
+        self._pi[self._cur].copy_(self._dev(arr))
+        self._touch_psi()
+
+    def set_chi(self, arr) -> None:
+        """All four reference chi buffers := arr."""
+        self._chi[0].copy_(self._dev(arr))
+        self._cc = self._cp = self._oc = self._op = 0
+
+    def set_chi_current(self, arr) -> None:
+        """Both sets' current chi := arr; prev buffers unchanged."""
+        s = next(q for q in range(3) if q != self._cp and q != self._op)
+        self._chi[s].copy_(self._dev(arr))
+        self._cc = self._oc = s
+
+    def set_chi_prev(self, arr) -> None:
+        """Both sets' prev chi := arr; current buffers unchanged."""
+        s = next(q for q in range(3) if q != self._cc and q != self._oc)
+        self._chi[s].copy_(self._dev(arr))
+        self._cp = self._op = s
+
+    def set_sa_fields(self, arr) -> None:
+        if self._sa is None:
+            raise ValueError("SA fields not allocated -- set kappa_tube > 0 in SimulationConfig"
+                             " before creating the Evolver.")
+        self._sa.copy_(self._dev(arr))
+
+    # --------------------------------------------------------- reductions
+    def _red_bufs(self):
+        if self._partials is None:
+            n = int(self._lib.lfm_reduce_partials_len(self.N))
+            self._partials = torch.empty(n, dtype=torch.float64, device=self.device)
+            self._red_out = torch.empty(8, dtype=torch.float64, device=self.device)
+        return self._partials, self._red_out
+
+    def energy_sums(self, prev_r=None, prev_i=None, c=None) -> dict[str, float]:
+        """FP64 sums of kinetic, gradient and potential energy and of |Psi|^2
+        (lfm/analysis/energy.py:26-135) on the GPU.  By default the kinetic
+        term uses the true leapfrog prev buffers (as the reference stores
+        them: damped); pass `prev_r`/`prev_i` device tensors to use others."""
+        part, out = self._red_bufs()
+        damp = 0
+        if prev_r is None:
+            prev_r = self._pr[self._prev]
+            prev_i = self._pi[self._prev] if self._has_imag else None
+            damp = int(self._damped and self.has_mask)
+        pi = self._pi[self._cur] if self._has_imag else None
+        c = self.config.c if c is None else c
+        rc = self._lib.lfm_energy_f64(
+            self._pr[self._cur].data_ptr(), prev_r.data_ptr(), pi.data_ptr() if pi is not None else None,
+            prev_i.data_ptr() if prev_i is not None else None, self._chi[self._cc].data_ptr(),
+            self.boundary_mask.data_ptr(), damp, self._n_psi, self.N, float(self.config.dt),
+            float(c), part.data_ptr(), out.data_ptr(), stream_handle())
+        _abi.check(rc, "lfm_energy_f64")
+        T, G, V, S = out[:4].tolist()
+        return {"energy_kinetic": T, "energy_gradient": G, "energy_potential": V,
+                "energy_total": T + G + V, "psi_sq_total": S}
+
+    def total_energy(self) -> float:
+        return self.energy_sums()["energy_total"]
+
+    def chi_stats(self, well_threshold=17.0, void_threshold=18.0) -> dict[str, float]:
+        """chi min/max/mean/std and well/void fractions over the interior ball
+        (lfm/analysis/structure.py:26-108,148-171), FP64 on the GPU."""
+        part, out = self._red_bufs()
+        rc = self._lib.lfm_chi_stats_f64(self._chi[self._cc].data_ptr(), self.N,
+                                         float(self.config.boundary_fraction),
+                                         float(well_threshold), float(void_threshold),
+                                         part.data_ptr(), out.data_ptr(), stream_handle())
+        _abi.check(rc, "lfm_chi_stats_f64")
+        v = out[:7].tolist()
+        return {"chi_min": v[0], "chi_max": v[1], "chi_mean": v[2], "chi_std": v[3],
+                "well_fraction": v[4], "void_fraction": v[5], "interior_count": v[6]}

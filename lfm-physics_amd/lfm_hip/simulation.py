@@ -1,0 +1,240 @@
+"""``Simulation``: the facade over :class:`Evolver` (lfm/simulation.py:47-1565).
+
+Kept: field properties, ``run`` (metrics every ``report_interval``, the
+``evolve_chi=False`` freeze), ``metrics``, ``total_energy``, S_a access and
+npz checkpoints with the reference's keys.  Metrics reductions run in FP64 on
+the GPU; only the cluster count (scipy.ndimage.label) pulls |Psi|^2 to the
+host, as the reference does.  Initial-condition helpers and experiment
+drivers of the reference are outside this package.
+"""
+from __future__ import annotations
+
+import json
+import warnings
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from .config import BoundaryType, FieldLevel, SimulationConfig
+from .evolver import Evolver
+
+
+class Simulation:
+    def __init__(self, config: SimulationConfig | None = None, backend="auto",
+                 mask_policy: str = "reference") -> None:
+        self.config = config if config is not None else SimulationConfig()
+        self._evolver = Evolver(self.config, backend=backend, mask_policy=mask_policy)
+        self._history: list[dict[str, float]] = []
+        # device snapshot of Psi at run() start / last callback (the reference's
+        # _psi_r_prev / _psi_i_prev, lfm/simulation.py:1168-1182)
+        self._snap_r: torch.Tensor | None = None
+        self._snap_i: torch.Tensor | None = None
+        self._interior = None
+
+    # -- fields ---------------------------------------------------------------
+    @property
+    def evolver(self) -> Evolver:
+        return self._evolver
+
+    @property
+    def step(self) -> int:
+        return self._evolver.step
+
+    @property
+    def history(self) -> list[dict[str, float]]:
+        return self._history
+
+    @property
+    def chi(self) -> np.ndarray:
+        return self._evolver.get_chi()
+
+    @chi.setter
+    def chi(self, value) -> None:
+        self._evolver.set_chi(value)
+
+    @property
+    def psi_real(self) -> np.ndarray:
+        return self._evolver.get_psi_real()
+
+    @psi_real.setter
+    def psi_real(self, value) -> None:
+        self._evolver.set_psi_real(value)
+
+    @property
+    def psi_imag(self):
+        return self._evolver.get_psi_imag()
+
+    @psi_imag.setter
+    def psi_imag(self, value) -> None:
+        self._evolver.set_psi_imag(value)
+
+    @property
+    def psi_real_prev(self):
+        return None if self._snap_r is None else self._snap_r.cpu().numpy().reshape(
+            self._evolver.get_psi_real().shape)
+
+    @property
+    def sa_fields(self):
+        return self._evolver.get_sa_fields()
+
+    @sa_fields.setter
+    def sa_fields(self, value) -> None:
+        self._evolver.set_sa_fields(value)
+
+    def get_chi(self):
+        return self._evolver.get_chi()
+
+    def get_psi_real(self):
+        return self._evolver.get_psi_real()
+
+    def get_psi_imag(self):
+        return self._evolver.get_psi_imag()
+
+    def get_energy_density(self):
+        return self._evolver.get_energy_density()
+
+    def set_psi_real(self, v):
+        self._evolver.set_psi_real(v)
+
+    def set_psi_imag(self, v):
+        self._evolver.set_psi_imag(v)
+
+    def set_psi_real_prev(self, v):
+        self._evolver.set_psi_real_prev(v)
+
+    def set_psi_imag_prev(self, v):
+        self._evolver.set_psi_imag_prev(v)
+
+    def set_chi(self, v):
+        self._evolver.set_chi(v)
+
+    # -- evolution ------------------------------------------------------------
+    def _snapshot(self) -> None:
+        ev = self._evolver
+        self._snap_r = ev.native_psi_real().clone()
+        pi = ev.native_psi_imag()
+        self._snap_i = pi.clone() if pi is not None else None
+
+    def run(self, steps: int, callback=None, record_metrics: bool = True,
+            evolve_chi: bool = True) -> None:
+        """Advance `steps` steps; a metrics snapshot goes to :attr:`history`
+        every report_interval steps (lfm/simulation.py:1123-1188)."""
+        self._snapshot()
+
+        def _cb(ev, step):
+            if record_metrics:
+                m = self.metrics()
+                m["step"] = float(step)
+                self._history.append(m)
+            if callback is not None:
+                callback(self, step)
+            self._snapshot()
+
+        self._evolver.evolve(steps, callback=_cb, freeze_chi=not evolve_chi)
+
+    # -- analysis -------------------------------------------------------------
+    def get_interior_mask(self) -> np.ndarray:
+        if self._interior is None:
+            N = self.config.grid_size
+            c = N / 2.0
+            x = np.arange(N, dtype=np.float64)
+            X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+            d = np.sqrt((X - c) ** 2 + (Y - c) ** 2 + (Z - c) ** 2)
+            self._interior = d <= c * (1.0 - self.config.boundary_fraction)
+        return self._interior
+
+    def _energy(self):
+        ev = self._evolver
+        if self._snap_r is None:
+            return ev.energy_sums(prev_r=ev.native_psi_real(), prev_i=ev.native_psi_imag())
+        return ev.energy_sums(prev_r=self._snap_r, prev_i=self._snap_i)
+
+    def metrics(self, clusters: bool = True) -> dict[str, float]:
+        """The reference's compute_metrics keys (lfm/analysis/metrics.py:98-111)."""
+        e = self._energy()
+        s = self._evolver.chi_stats()
+        out = {"energy_kinetic": e["energy_kinetic"], "energy_gradient": e["energy_gradient"],
+               "energy_potential": e["energy_potential"], "energy_total": e["energy_total"],
+               "chi_min": s["chi_min"], "chi_max": s["chi_max"], "chi_mean": s["chi_mean"],
+               "chi_std": s["chi_std"], "well_fraction": s["well_fraction"],
+               "void_fraction": s["void_fraction"], "n_clusters": float("nan"),
+               "psi_sq_total": e["psi_sq_total"]}
+        if clusters:
+            out["n_clusters"] = float(self._count_clusters())
+        return out
+
+    def _count_clusters(self, percentile: float = 90.0) -> int:
+        # lfm/analysis/structure.py:111-145 (host: scipy.ndimage.label)
+        from scipy import ndimage
+        psq = self._evolver.get_energy_density().astype(np.float64)
+        interior = self.get_interior_mask()
+        vals = psq[interior]
+        if vals.size == 0:
+            return 0
+        binary = (psq > np.percentile(vals, percentile)) & interior
+        return int(ndimage.label(binary)[1])
+
+    def total_energy(self) -> float:
+        return self._energy()["energy_total"]
+
+    # -- checkpoints ----------------------------------------------------------
+    def save_checkpoint(self, path) -> None:
+        """npz with the reference's keys (lfm/simulation.py:1474-1514) plus the
+        true leapfrog prev buffers (psi_real_prev_true, chi_prev_true)."""
+        path = Path(path)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        ev = self._evolver
+        data = {"step": np.int64(self.step), "chi": ev.get_chi(), "psi_real": ev.get_psi_real(),
+                "psi_real_prev_true": ev.get_psi_real_prev(), "chi_prev_true": ev.get_chi_prev()}
+        if ev.get_psi_imag() is not None:
+            data["psi_imag"] = ev.get_psi_imag()
+            data["psi_imag_prev_true"] = ev.get_psi_imag_prev()
+        if self._snap_r is not None:
+            data["psi_real_prev"] = self.psi_real_prev
+        if self._snap_i is not None:
+            data["psi_imag_prev"] = self._snap_i.cpu().numpy().reshape(ev.get_psi_imag().shape)
+        sa = ev.get_sa_fields()
+        if sa is not None:
+            data["sa_fields"] = sa
+        cfg = {k: (v.value if hasattr(v, "value") else v) for k, v in vars(self.config).items()
+               if not k.startswith("_")}
+        data["config_json"] = np.array(json.dumps(cfg))
+        data["history_json"] = np.array(json.dumps(self._history))
+        np.savez_compressed(str(path), **data)
+
+    @classmethod
+    def load_checkpoint(cls, path, backend="auto", exact: bool = False) -> "Simulation":
+        """Restore a checkpoint (lfm/simulation.py:1516-1565).  With
+        exact=True and the true prev buffers present the run resumes with
+        its time derivatives; otherwise current = prev, as the reference."""
+        data = np.load(str(path), allow_pickle=False)
+        cfg = json.loads(str(data["config_json"]))
+        cfg["field_level"] = FieldLevel(cfg["field_level"])
+        cfg["boundary_type"] = BoundaryType(cfg["boundary_type"])
+        for k in ("dx", "sigma"):
+            cfg.pop(k, None)
+        for k in ("physical_scale", "chi_mode"):
+            if k in cfg and cfg[k] is not None:
+                from . import config as _c
+                cfg[k] = (_c.PhysicsScale if k == "physical_scale" else _c.ChiMode)(cfg[k])
+        sim = cls(SimulationConfig(**cfg), backend=backend)
+        ev = sim._evolver
+        ev.set_psi_real(data["psi_real"])
+        if "psi_imag" in data:
+            ev.set_psi_imag(data["psi_imag"])
+        ev.set_chi(data["chi"])
+        if exact and "psi_real_prev_true" in data:
+            ev.set_psi_real_prev(data["psi_real_prev_true"])
+            if "psi_imag_prev_true" in data:
+                ev.set_psi_imag_prev(data["psi_imag_prev_true"])
+            ev.set_chi_prev(data["chi_prev_true"])
+        ev.step = int(data["step"])
+        if "sa_fields" in data and ev.sa_enabled:
+            ev.set_sa_fields(data["sa_fields"])
+        if "psi_real_prev" in data:
+            sim._snap_r = ev._dev(data["psi_real_prev"])
+        if "psi_imag_prev" in data:
+            sim._snap_i = ev._dev(data["psi_imag_prev"])
+        sim._history = json.loads(str(data["history_json"]))
+        return sim

@@ -1,0 +1,349 @@
+"""GPU parity: the HIP step path against the CPU oracle (FP64 and FP32).
+
+Tolerances (written per test):
+* FP32 GPU vs FP64 oracle: relative L2 <= 1e-5 after 100 steps, every field.
+* HipBackend (reference four-output contract) vs FP32 oracle over 50 steps:
+  atol 2e-4 / rtol 1e-4, the reference's own GPU/CPU parity bar
+  (tests/test_gpu_parity.py:39-41 of the reference).
+* Golden vectors of the reference's step_color: relative L2 <= 2e-6.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import lfm_oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOL64 = 1e-5
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_params(cfg):
+    return dict(dt2=cfg.dt ** 2, kappa=cfg.kappa, lambda_self=cfg.lambda_self, chi0=cfg.chi0,
+                e0_sq=cfg.e0_sq, epsilon_w=cfg.epsilon_w, kappa_c=cfg.kappa_c,
+                epsilon_cc=cfg.epsilon_cc, kappa_string=cfg.kappa_string,
+                kappa_tube=cfg.kappa_tube, sa_gamma=cfg.sa_gamma, sa_d=cfg.sa_d)
+
+
+def make_ics(level, N, seed=42):
+    """Seeded soliton(s) + noise, float32, as the workloads build them."""
+    import lfm_hip as L
+    rng = np.random.default_rng(seed)
+    c = N / 2
+    if level == 0:
+        E = L.fields.gaussian_soliton(N, (c, c, c), 6.0, 3.0)[0]
+        E = E + (rng.standard_normal((N,) * 3) * 1e-3).astype(np.float32)
+        return E.astype(np.float32), None
+    if level == 1:
+        pr = np.zeros((N,) * 3, np.float32)
+        pi = np.zeros((N,) * 3, np.float32)
+        for _ in range(3):
+            pos = rng.uniform(0.3 * N, 0.7 * N, 3)
+            r, i = L.fields.gaussian_soliton(N, tuple(pos), rng.uniform(2, 6), rng.uniform(2, 4),
+                                             rng.uniform(0, 2 * np.pi))
+            pr += r
+            pi += i
+        return pr, pi
+    pos = [tuple(c + rng.uniform(-0.35, 0.35, 3) * c * 0.5) for _ in range(3)]
+    return L.fields.place_solitons(N, pos, 4.0, 3.0, phases=rng.uniform(0, 2 * np.pi, 3),
+                                   colors=[0, 1, 2])
+
+
+def run_pair(cfg, steps, mask_policy="reference", direct=False, seed=42):
+    import lfm_hip as L
+    import torch
+    lvl = int(cfg.field_level)
+    pr, pi = make_ics(lvl, cfg.grid_size, seed)
+    ev = L.Evolver(cfg, backend="hip", mask_policy=mask_policy)
+    ev.set_psi_real(pr)
+    if pi is not None:
+        ev.set_psi_imag(pi)
+    for _ in range(steps):
+        ev._step(direct=direct)
+        ev.step += 1
+    torch.cuda.synchronize()
+    N = cfg.grid_size
+    mask = O.boundary_mask(N, cfg.boundary_fraction) if ev.has_mask else np.zeros(N ** 3)
+    st = O.OracleState(lvl, N, pr, pi, np.full(N ** 3, cfg.chi0), mask, oracle_params(cfg))
+    st.step(steps)
+    return ev, st
+
+
+def check_fields(ev, st, tol, tag):
+    errs = {"psi_r": O.rel_l2(ev.get_psi_real(), st.psi_r),
+            "psi_r_prev": O.rel_l2(ev.get_psi_real_prev(), st.psi_r_prev),
+            "chi": O.rel_l2(ev.get_chi(), st.chi),
+            "chi_prev": O.rel_l2(ev.get_chi_prev(), st.chi_prev)}
+    if st.psi_i is not None:
+        errs["psi_i"] = O.rel_l2(ev.get_psi_imag(), st.psi_i)
+        errs["psi_i_prev"] = O.rel_l2(ev.get_psi_imag_prev(), st.psi_i_prev)
+    if st.sa_fields is not None:
+        errs["sa"] = O.rel_l2(ev.get_sa_fields(), st.sa_fields)
+    log(f"{tag}: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+    for k, v in errs.items():
+        assert v <= tol, (tag, k, v)
+
+
+# ----------------------------------------------- Evolver vs FP64 oracle
+@pytest.mark.parametrize("direct", [False, True], ids=["tiled", "direct"])
+@pytest.mark.parametrize("case", [
+    ("real64_gravity_only", 0, 64), ("real24", 0, 24),
+    ("complex48_gravity_em", 1, 48), ("complex32", 1, 32),
+    ("color32_full_physics", 2, 32), ("color16_full_physics", 2, 16)])
+def test_evolver_vs_fp64_oracle(gpu, case, direct):
+    import lfm_hip as L
+    name, lvl, N = case
+    log(f"case {name} direct={direct}")
+    cfg = [L.gravity_only, L.gravity_em, L.full_physics][lvl](N)
+    ev, st = run_pair(cfg, 100, direct=direct)
+    check_fields(ev, st, TOL64, name)
+
+
+@pytest.mark.parametrize("lvl", [0, 1, 2])
+def test_periodic_mask_free(gpu, lvl):
+    import lfm_hip as L
+    N = 32
+    log(f"periodic mask-free level {lvl}")
+    cfg = [L.gravity_only, L.gravity_em, L.full_physics][lvl](
+        N, boundary_type=L.BoundaryType.PERIODIC)
+    ev, st = run_pair(cfg, 100, mask_policy="boundary_type")
+    assert not ev.has_mask
+    check_fields(ev, st, TOL64, f"periodic{lvl}")
+
+
+@pytest.mark.parametrize("N", [8, 20, 40, 72])
+def test_tiled_equals_direct_ragged_sizes(gpu, N):
+    """One step on random fields; N not a multiple of the 64-wide tile."""
+    import lfm_hip as L
+    import torch
+    log(f"tiled vs direct N={N}")
+    rng = np.random.default_rng(N)
+    for lvl, mk in ((0, L.gravity_only), (1, L.gravity_em), (2, L.full_physics)):
+        cfg = mk(N)
+        outs = []
+        for direct in (False, True):
+            ev = L.Evolver(cfg, backend="hip")
+            r = np.random.default_rng(N + lvl)
+            shape = (3, N, N, N) if lvl == 2 else (N, N, N)
+            ev.set_psi_real(r.standard_normal(shape).astype(np.float32))
+            if lvl:
+                ev.set_psi_imag(r.standard_normal(shape).astype(np.float32))
+            ev.set_chi((19 + 0.1 * r.standard_normal((N,) * 3)).astype(np.float32))
+            ev._step(direct=direct)
+            ev._step(direct=direct)
+            torch.cuda.synchronize()
+            outs.append([ev.get_psi_real(), ev.get_chi()] + ([ev.get_psi_imag()] if lvl else []))
+        for a, b in zip(*outs):
+            assert O.rel_l2(a, b) < 1e-6, (N, lvl, O.rel_l2(a, b))
+    del rng
+
+
+# ------------------------------------- HipBackend four-output contract
+def _ab_loop(be, step_fn, A, B, steps):
+    for _ in range(steps):
+        step_fn(A, B)
+        A, B = B, A
+    return A
+
+
+@pytest.mark.parametrize("lvl", [0, 1, 2])
+def test_backend_contract_vs_fp32_oracle(gpu, lvl):
+    """Reference test_gpu_parity.py: N=16, 50 steps, bf 0.15, atol 2e-4, rtol 1e-4."""
+    import torch
+    import lfm_hip as L
+    log(f"backend contract level {lvl}")
+    N, steps, bf = 16, 50, 0.15
+    be = L.HipBackend()
+    mask_np = O.boundary_mask(N, bf)
+    mask = be.create_boundary_mask(N, bf)
+    np.testing.assert_array_equal(be.to_numpy(mask), mask_np)
+    cfg = [L.gravity_only, L.gravity_em, L.full_physics][lvl](N, boundary_fraction=bf)
+    pr, pi = make_ics(lvl, N)
+    n = N ** 3
+    C = 3 if lvl == 2 else 1
+    chi0 = np.full(n, 19.0, np.float32)
+
+    def bufs(xp_from, r, i):
+        d = {"r": xp_from(r.ravel()), "rp": xp_from(r.ravel()), "c": xp_from(chi0),
+             "cp": xp_from(chi0)}
+        if lvl:
+            d["i"], d["ip"] = xp_from(i.ravel()), xp_from(i.ravel())
+        if lvl == 2:
+            d["sa"] = xp_from(np.zeros(3 * n, np.float32))
+        return d
+    gA, gB = bufs(be.from_numpy, pr, pi), bufs(be.from_numpy, np.zeros_like(pr),
+                                               None if pi is None else np.zeros_like(pi))
+    cA = bufs(lambda a: a.astype(np.float32).copy(), pr, pi)
+    cB = bufs(lambda a: a.astype(np.float32).copy(), np.zeros_like(pr),
+              None if pi is None else np.zeros_like(pi))
+    s = dict(N=N, dt2=cfg.dt ** 2, kappa=cfg.kappa, lambda_self=cfg.lambda_self, chi0=cfg.chi0,
+             e0_sq=cfg.e0_sq)
+
+    def g_step(A, B):
+        if lvl == 0:
+            be.step_real(A["r"], A["rp"], A["c"], A["cp"], mask, B["r"], B["rp"], B["c"], B["cp"], **s)
+        elif lvl == 1:
+            be.step_complex(A["r"], A["rp"], A["i"], A["ip"], A["c"], A["cp"], mask, B["r"],
+                            B["rp"], B["i"], B["ip"], B["c"], B["cp"], epsilon_w=cfg.epsilon_w, **s)
+        else:
+            be.step_color(A["r"], A["rp"], A["i"], A["ip"], A["c"], A["cp"], mask, B["r"],
+                          B["rp"], B["i"], B["ip"], B["c"], B["cp"], epsilon_w=cfg.epsilon_w,
+                          kappa_c=cfg.kappa_c, epsilon_cc=cfg.epsilon_cc,
+                          kappa_string=cfg.kappa_string, kappa_tube=cfg.kappa_tube,
+                          sa_fields_in=A["sa"], sa_fields_out=B["sa"], sa_gamma=cfg.sa_gamma,
+                          sa_d=cfg.sa_d, **s)
+
+    def c_step(A, B):
+        if lvl == 0:
+            O.step_real(A["r"], A["rp"], A["c"], A["cp"], mask_np, B["r"], B["rp"], B["c"], B["cp"], **s)
+        elif lvl == 1:
+            O.step_complex(A["r"], A["rp"], A["i"], A["ip"], A["c"], A["cp"], mask_np, B["r"],
+                           B["rp"], B["i"], B["ip"], B["c"], B["cp"], epsilon_w=cfg.epsilon_w, **s)
+        else:
+            O.step_color(A["r"], A["rp"], A["i"], A["ip"], A["c"], A["cp"], mask_np, B["r"],
+                         B["rp"], B["i"], B["ip"], B["c"], B["cp"], epsilon_w=cfg.epsilon_w,
+                         kappa_c=cfg.kappa_c, epsilon_cc=cfg.epsilon_cc,
+                         kappa_string=cfg.kappa_string, kappa_tube=cfg.kappa_tube,
+                         sa_fields_in=A["sa"], sa_fields_out=B["sa"], sa_gamma=cfg.sa_gamma,
+                         sa_d=cfg.sa_d, **s)
+
+    gF = _ab_loop(be, g_step, gA, gB, steps)
+    cF = _ab_loop(None, c_step, cA, cB, steps)
+    torch.cuda.synchronize()
+    for k in cF:
+        got = be.to_numpy(gF[k])
+        log(f"  {k}: max|d|={np.max(np.abs(got - cF[k])):.2e}")
+        np.testing.assert_allclose(got, cF[k], atol=2e-4, rtol=1e-4, err_msg=f"{k} level {lvl}")
+    del C
+
+
+def test_backend_step_color_golden(gpu):
+    """The reference's own step_color outputs (N=8, seeded, all terms on)."""
+    import lfm_hip as L
+    import torch
+    log("golden step_color N=8")
+    g = np.load(os.path.join(HERE, "golden", "step_color_N8.npz"))
+    be = L.HipBackend()
+    N = int(g["N"])
+    for name in ("full", "kc"):
+        p = {k: float(g[f"{name}_param_{k}"]) for k in
+             ("kappa", "lambda_self", "chi0", "e0_sq", "epsilon_w", "kappa_c", "epsilon_cc",
+              "kappa_string", "kappa_tube", "sa_gamma", "sa_d")}
+        A = {k: be.from_numpy(g[s]) for k, s in (("r", "pr"), ("rp", "prp"), ("i", "pi"),
+                                                  ("ip", "pip"), ("c", "chi"), ("cp", "chip"))}
+        A["sa"] = be.from_numpy(np.zeros(3 * N ** 3, np.float32))
+        B = {k: torch.zeros_like(v) for k, v in A.items()}
+        mask = be.from_numpy(g["mask"])
+        for step in range(1, 4):
+            be.step_color(A["r"], A["rp"], A["i"], A["ip"], A["c"], A["cp"], mask, B["r"], B["rp"],
+                          B["i"], B["ip"], B["c"], B["cp"], N, 0.02 ** 2, p["kappa"],
+                          p["lambda_self"], p["chi0"], p["e0_sq"], p["epsilon_w"], p["kappa_c"],
+                          p["epsilon_cc"], kappa_string=p["kappa_string"],
+                          kappa_tube=p["kappa_tube"], sa_fields_in=A["sa"],
+                          sa_fields_out=B["sa"], sa_gamma=p["sa_gamma"], sa_d=p["sa_d"])
+            A, B = B, A
+            if step in (1, 3):
+                for k in ("r", "rp", "i", "ip", "c", "cp") + (("sa",) if p["kappa_tube"] > 0 else ()):
+                    e = O.rel_l2(be.to_numpy(A[k]), g[f"{name}_s{step}_{k}"])
+                    assert e <= 2e-6, (name, step, k, e)
+
+
+# ------------------------------------------------ evolver semantics
+def test_freeze_chi_and_setters_match_reference_semantics(gpu):
+    import lfm_hip as L
+    import torch
+    log("freeze_chi / set_chi_current / set_chi_prev / set_psi_real_prev")
+    N = 24
+    cfg = L.gravity_em(N)
+    pr, pi = make_ics(1, N)
+    ev = L.Evolver(cfg, backend="hip")
+    ev.set_psi_real(pr)
+    ev.set_psi_imag(pi)
+    st = O.OracleState(1, N, pr, pi, np.full(N ** 3, 19.0), O.boundary_mask(N, 0.3),
+                       oracle_params(cfg))
+    rng = np.random.default_rng(5)
+    ev.evolve(5); st.step(5)
+    ev.evolve(3, freeze_chi=True); st.step(3, freeze_chi=True)
+    ev.evolve(4); st.step(4)
+    x = (19 + 0.01 * rng.standard_normal((N,) * 3)).astype(np.float32)
+    ev.set_chi_current(x); st.set_chi_current(x)
+    ev.evolve(2); st.step(2)
+    y = (19 + 0.01 * rng.standard_normal((N,) * 3)).astype(np.float32)
+    ev.set_chi_prev(y); st.set_chi_prev(y)
+    ev.evolve(1, freeze_chi=True); st.step(1, freeze_chi=True)
+    ev.evolve(2); st.step(2)
+    z = ev.get_psi_real() * np.float32(0.999)
+    ev.set_psi_real_prev(z); st.set_psi_real_prev(z)
+    ev.evolve(3); st.step(3)
+    torch.cuda.synchronize()
+    check_fields(ev, st, TOL64, "semantics")
+
+
+# --------------------------------------------------------- reductions
+@pytest.mark.parametrize("lvl", [0, 1, 2])
+def test_energy_and_chi_stats_fp64(gpu, lvl):
+    import lfm_hip as L
+    log(f"reductions level {lvl}")
+    N = 32
+    cfg = [L.gravity_only, L.gravity_em, L.full_physics][lvl](N)
+    ev, _ = run_pair(cfg, 7)
+    pr, pi = ev.get_psi_real().astype(np.float64), ev.get_psi_imag()
+    prp, pip = ev.get_psi_real_prev(), ev.get_psi_imag_prev()
+    chi = ev.get_chi()
+    T, G, V = O.energy_components(pr, prp, chi, cfg.dt, 1.0, pi, pip)
+    e = ev.energy_sums()
+    for k, ref in (("energy_kinetic", T.sum()), ("energy_gradient", G.sum()),
+                   ("energy_potential", V.sum())):
+        assert abs(e[k] - ref) <= 1e-10 * abs(ref) + 1e-12, (k, e[k], ref)
+    interior = O.interior_mask(N, cfg.boundary_fraction)
+    s = O.chi_statistics(chi.astype(np.float64), interior)
+    w, v = O.well_void_fraction(chi, 17.0, 18.0, interior)
+    got = ev.chi_stats()
+    assert got["chi_min"] == s["min"] and got["chi_max"] == s["max"]
+    assert abs(got["chi_mean"] - s["mean"]) <= 1e-12 * abs(s["mean"])
+    assert abs(got["chi_std"] - s["std"]) <= 1e-9 * max(s["std"], 1e-12)
+    assert got["well_fraction"] == w and got["void_fraction"] == v
+
+
+def test_simulation_run_metrics_and_checkpoint(gpu, tmp_path):
+    import lfm_hip as L
+    log("simulation run/metrics/checkpoint")
+    N = 24
+    cfg = L.gravity_em(N, report_interval=10)
+    sim = L.Simulation(cfg, backend="hip")
+    pr, pi = make_ics(1, N)
+    sim.psi_real, sim.psi_imag = pr, pi
+    sim.run(30)
+    assert len(sim.history) == 3 and sim.step == 30
+    m = sim.metrics()
+    for k in ("energy_kinetic", "energy_gradient", "energy_potential", "energy_total", "chi_min",
+              "chi_max", "chi_mean", "chi_std", "well_fraction", "void_fraction", "n_clusters",
+              "psi_sq_total"):
+        assert k in m and np.isfinite(m[k])
+    path = tmp_path / "ck.npz"
+    sim.save_checkpoint(path)
+    sim2 = L.Simulation.load_checkpoint(path, backend="hip", exact=True)
+    sim.run(5, record_metrics=False)
+    sim2.run(5, record_metrics=False)
+    assert O.rel_l2(sim2.chi, sim.chi) < 1e-7
+    assert O.rel_l2(sim2.psi_real, sim.psi_real) < 1e-7
+
+
+def test_backend_rejects_bad_tensors(gpu):
+    import torch
+    import lfm_hip as L
+    be = L.HipBackend()
+    N = 8
+    good = [torch.zeros(N ** 3, device="cuda") for _ in range(8)]
+    bad = torch.zeros(N ** 3, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="float32"):
+        be.step_real(bad, *good[1:4], None, *good[4:8], N, 4e-4, 1 / 63, 0, 19, 0)
+    with pytest.raises(L._abi.LfmError if hasattr(L, "_abi") else Exception):
+        be.step_real(good[0], *good[1:4], None, good[0], *good[5:8], N, 4e-4, 1 / 63, 0, 19, 0)

@@ -1,0 +1,137 @@
+"""CPU-side tests: the C-ABI library, host logic, configs and IC builders.
+
+No GPU compute is called here; the library is only loaded and inspected.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import lfm_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lfm_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lfm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from lfm_hip import _abi, _build
+    path = _build.library_path()
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    syms = declared_symbols()
+    assert len(syms) >= 12
+    for s in syms:
+        assert hasattr(lib, s), f"missing export {s}"
+    bound = {name for name, _, _ in _abi.SIGNATURES}
+    assert set(syms) == bound, set(syms) ^ bound
+
+
+def test_abi_version_and_error_string():
+    from lfm_hip import _abi
+    lib = _abi.load()
+    assert lib.lfm_abi_version() == 1
+    # argument validation happens before any device work: N < 8 is refused
+    p = _abi.LfmParams(4, 0, 4e-4, 1 / 63, 0, 19, 0, 0.1, 0, 0, 0, 0)
+    dummy = [1 << 20] * 4
+    rc = lib.lfm_step_real_f32(*dummy, None, *[(2 << 20) + (q << 16) for q in range(4)],
+                               ctypes.byref(p), None)
+    assert rc == -1
+    assert b"out of range" in lib.lfm_last_error()
+
+
+def test_abi_rejects_in_place_and_missing_inputs():
+    from lfm_hip import _abi
+    lib = _abi.load()
+    N = 16
+    p = _abi.LfmParams(N, 0, 4e-4, 1 / 63, 0, 19, 0, 0.1, 0, 0, 0, 0)
+    base = 1 << 24
+    E, Ep, chi, chip = base, base + 4 * N ** 3, base + 8 * N ** 3, base + 12 * N ** 3
+    # output aliases an input
+    rc = lib.lfm_step_real_f32(E, Ep, chi, chip, None, E, None, base + 16 * N ** 3, None,
+                               ctypes.byref(p), None)
+    assert rc == -1 and b"overlaps" in lib.lfm_last_error()
+    rc = lib.lfm_step_real_f32(None, Ep, chi, chip, None, base + 16 * N ** 3, None,
+                               base + 20 * N ** 3, None, ctypes.byref(p), None)
+    assert rc == -1 and b"NULL" in lib.lfm_last_error()
+    # mask flag without a mask
+    p.flags = _abi.LFM_F_MASK
+    rc = lib.lfm_step_real_f32(E, Ep, chi, chip, None, base + 16 * N ** 3, None,
+                               base + 20 * N ** 3, None, ctypes.byref(p), None)
+    assert rc == -1 and b"mask" in lib.lfm_last_error()
+    # COLOR with kappa_tube but no S_a
+    p.flags, p.kappa_tube = 0, 0.5
+    n3 = N ** 3 * 4
+    args = [base + q * 3 * n3 for q in range(4)] + [base + 12 * n3, base + 13 * n3, None]
+    outs = [base + (20 + 3 * q) * n3 for q in range(4)] + [base + 40 * n3, None]
+    rc = lib.lfm_step_color_f32(*args, *outs, None, None, None, ctypes.byref(p), None)
+    assert rc == -1 and b"S_a" in lib.lfm_last_error()
+
+
+def test_host_mask_bit_identical_to_oracle():
+    from lfm_hip.fields import boundary_mask_host
+    for N, bf in ((8, 0.3), (16, 0.15), (48, 0.3), (64, 0.3)):
+        np.testing.assert_array_equal(boundary_mask_host(N, bf), O.boundary_mask(N, bf))
+
+
+def test_config_defaults_and_validation():
+    import lfm_hip as L
+    c = L.SimulationConfig()
+    assert (c.grid_size, c.chi0, c.dt, c.boundary_fraction, c.report_interval) == (128, 19.0, 0.02, 0.3, 5000)
+    assert abs(c.kappa - 1 / 63) < 1e-15 and c.epsilon_w == pytest.approx(0.1)
+    assert c.e_amplitude == 3.6 and c.sigma == pytest.approx(128 / 12)
+    with pytest.raises(ValueError, match="CFL"):
+        L.SimulationConfig(dt=0.06)
+    with pytest.raises(ValueError):
+        L.SimulationConfig(grid_size=4)
+    with pytest.raises(ValueError):
+        L.SimulationConfig(boundary_fraction=1.0)
+    fp = L.full_physics(32)
+    assert fp.field_level == L.FieldLevel.COLOR and fp.sa_enabled
+    assert fp.kappa_tube == pytest.approx(30 / 63) and fp.lambda_self == pytest.approx(4 / 31)
+    assert fp.epsilon_cc == pytest.approx(2 / 17) and fp.kappa_string == pytest.approx(1 / 189)
+    ge = L.gravity_em(16)
+    assert ge.field_level == L.FieldLevel.COMPLEX and ge.lambda_self == pytest.approx(4 / 31)
+    sc = L.SimulationConfig.for_scale(L.PhysicsScale.NUCLEAR, grid_size=16)
+    assert sc.field_level == L.FieldLevel.COLOR and sc.kappa_c == pytest.approx(1 / 189)
+
+
+def test_get_backend_refuses_cpu_and_unknown():
+    import lfm_hip as L
+    with pytest.raises(ValueError, match="no CPU backend"):
+        L.get_backend("cpu")
+    with pytest.raises(ValueError, match="Unknown"):
+        L.get_backend("tpu")
+
+
+def test_backend_needs_gpu_when_absent():
+    import torch
+    import lfm_hip as L
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(RuntimeError, match="GPU"):
+        L.HipBackend()
+
+
+def test_ic_builders_match_formulas():
+    from lfm_hip import fields
+    N = 16
+    r, i = fields.gaussian_soliton(N, (8, 7, 9), 3.0, 2.0, phase=0.5)
+    assert r.dtype == np.float32 and r.shape == (N, N, N)
+    assert r[8, 7, 9] == pytest.approx(3.0 * np.cos(0.5), rel=1e-6)
+    assert i[8, 7, 9] == pytest.approx(3.0 * np.sin(0.5), rel=1e-6)
+    assert r[8, 7, 11] == pytest.approx(3.0 * np.cos(0.5) * np.exp(-4 / 8), rel=1e-6)
+    pr, pi = fields.place_solitons(N, [(4, 4, 4), (10, 10, 10)], 2.0, 1.5, colors=[0, 2])
+    assert pr.shape == (3, N, N, N) and np.all(pr[1] == 0)
+    assert pr[0, 4, 4, 4] == pytest.approx(2.0, rel=1e-5)
+    a = fields.seed_noise(N, 1e-3, rng=np.random.default_rng(42))[0]
+    b = (np.random.default_rng(42).standard_normal((N, N, N)) * 1e-3).astype(np.float32)
+    np.testing.assert_array_equal(a, b)
