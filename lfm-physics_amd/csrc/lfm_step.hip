@@ -79,12 +79,31 @@ __device__ __forceinline__ float* field_out(float* pr, float* pi, int f, size_t 
     return f < C ? pr + f * n3 : pi + (f - C) * n3;
 }
 
+// Site addressing.  Off32 is a byte offset shared by every field (all fields
+// have the same layout), so each access is one uniform 64-bit base held in
+// SGPRs plus one 32-bit lane offset (the saddr form of global loads); it
+// covers N <= 1023.  Off64 is a plain element index for any N.
+struct Off32 {
+    uint32_t b;
+    __device__ __forceinline__ float ld(const float* p) const {
+        return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + b);
+    }
+    __device__ __forceinline__ void st(float* p, float v) const {
+        *reinterpret_cast<float*>(reinterpret_cast<char*>(p) + b) = v;
+    }
+};
+struct Off64 {
+    size_t i;
+    __device__ __forceinline__ float ld(const float* p) const { return p[i]; }
+    __device__ __forceinline__ void st(float* p, float v) const { p[i] = v; }
+};
+
 // Per-site update, shared by both kernels.
 //  c[f]     Psi component values at the site      lap[f]  their Laplacians
 //  jsum     sum over colours and directions of j_{a,d}   (COMPLEX/COLOR)
 //  ccv      sum_d (sum_a j_{a,d}^2 - 1/3 (sum_a j_{a,d})^2)  (COLOR)
-template <int L>
-__device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, size_t idx,
+template <int L, class O>
+__device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O idx,
                                             const float (&c)[Lv<L>::NPF],
                                             const float (&lap)[Lv<L>::NPF], float chi_c,
                                             float lap_chi, float jsum, float ccv) {
@@ -92,7 +111,7 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, si
     constexpr int C = Lv<L>::NC;
     const size_t n3 = P.n3;
     const bool has_mask = (kp.flags & LFM_F_MASK) != 0;
-    const float m = has_mask ? P.mask[idx] : 0.0f;
+    const float m = has_mask ? idx.ld(P.mask) : 0.0f;
     const float a = 1.0f - m;
     const float chi_sq = chi_c * chi_c;
 
@@ -107,20 +126,20 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, si
     float nw[NPF];
 #pragma unroll
     for (int f = 0; f < NPF; ++f) {
-        float pv = field<L>(P.prp, P.pip, f, n3)[idx];
+        float pv = idx.ld(field<L>(P.prp, P.pip, f, n3));
         if (damp) pv = pv * a;
         float v = 2.0f * c[f] - pv + kp.dt2 * (lap[f] - chi_sq * c[f]);
         if constexpr (L == COLOR) {
             if (kp.ecc > 0.0f) v -= kp.dt2 * kp.ecc * chi_sq * (c[f] - avg[f / C]);
         }
         nw[f] = v * a;
-        field_out<L>(P.pr_out, P.pi_out, f, n3)[idx] = nw[f];
-        if (P.prp_out) field_out<L>(P.prp_out, P.pip_out, f, n3)[idx] = c[f] * a;
+        idx.st(field_out<L>(P.pr_out, P.pi_out, f, n3), nw[f]);
+        if (P.prp_out) idx.st(field_out<L>(P.prp_out, P.pip_out, f, n3), c[f] * a);
     }
     if constexpr (L == COLOR) {
         if (P.psisq) {
 #pragma unroll
-            for (int q = 0; q < C; ++q) P.psisq[q * n3 + idx] = nw[q] * nw[q] + nw[q + C] * nw[q + C];
+            for (int q = 0; q < C; ++q) idx.st(P.psisq + q * n3, nw[q] * nw[q] + nw[q + C] * nw[q + C]);
         }
     }
     if (kp.flags & LFM_F_FREEZE_CHI) return;
@@ -154,9 +173,9 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, si
                 float s_sum = 0.0f, s_sq = 0.0f;
 #pragma unroll
                 for (int q = 0; q < C; ++q) {
-                    float s = P.sa[q * n3 + idx];
+                    float s = idx.ld(P.sa + q * n3);
                     s = s > 0.0f ? s : 0.0f;
-                    if (P.sa_clip) P.sa_clip[q * n3 + idx] = s;
+                    if (P.sa_clip) idx.st(P.sa_clip + q * n3, s);
                     s_sum += s;
                     s_sq += s * s;
                 }
@@ -167,11 +186,11 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, si
     const float chi_self = -4.0f * kp.lam * chi_c * (chi_sq - kp.chi0 * kp.chi0);
     float accel = lap_chi - src - cv + chi_self;
     if constexpr (L == COLOR) accel = accel - kp.ks * ccv - kp.kt * scv;
-    float cn = 2.0f * chi_c - P.chip[idx] + kp.dt2 * accel;
+    float cn = 2.0f * chi_c - idx.ld(P.chip) + kp.dt2 * accel;
     if (cn < -kp.chi0) cn = -kp.chi0;   // BH excision clamp (NaN passes through)
     cn = m * kp.chi0 + a * cn;
-    P.chi_out[idx] = cn;
-    if (P.chip_out) P.chip_out[idx] = chi_c;
+    idx.st(P.chi_out, cn);
+    if (P.chip_out) idx.st(P.chip_out, chi_c);
 }
 
 // ---------------------------------------------------------------------------
@@ -238,7 +257,7 @@ __global__ __launch_bounds__(256) void step_direct(Ptrs P, KParams kp) {
             }
         }
     }
-    site_update<L>(kp, P, idx, c, lap, chi_c, lap_chi, jsum, ccv);
+    site_update<L>(kp, P, Off64{idx}, c, lap, chi_c, lap_chi, jsum, ccv);
 }
 
 // ---------------------------------------------------------------------------
@@ -252,9 +271,9 @@ __global__ __launch_bounds__(256) void step_direct(Ptrs P, KParams kp) {
 // ---------------------------------------------------------------------------
 constexpr int TK = 64;
 
-template <int L, int BJ, int RJ>
-__global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nkt, int njt,
-                                                       int CH) {
+template <int L, int BJ, int RJ, int WPE>
+__global__ __launch_bounds__(64 * BJ) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
     constexpr int NPF = Lv<L>::NPF;
     constexpr int C = Lv<L>::NC;
     constexpr int NF = NPF + 1;              // + chi
@@ -262,11 +281,17 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nk
     constexpr int LW = TK + 2;
     constexpr int LH = TJ + 2;
     static_assert(BJ >= 4, "halo loading uses four wavefronts");
-    __shared__ float lds[NF][LH][LW];
+    // Field-fastest LDS layout: the NF values of one (row, column) position are
+    // adjacent (padded to NFP), so all 9 stencil positions of all fields are
+    // reached from one per-lane base address with immediate offsets, and the
+    // 64 lanes of a wavefront touch consecutive positions (stride NFP floats).
+    constexpr int NFP = NF == 7 ? 8 : NF == 3 ? 4 : NF;
+    __shared__ float lds[LH][LW][NFP];
 
     const int N = kp.N;
     const size_t n3 = P.n3;
-    const size_t NN = size_t(N) * N;
+    const uint32_t rowB = uint32_t(N) * 4u;         // bytes per j row
+    const uint32_t planeB = rowB * uint32_t(N);     // bytes per i plane
 
     // XCD-aware block remap (bijective when gridDim.x % 8 == 0).
     const int nb = gridDim.x;
@@ -288,42 +313,56 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nk
     for (int f = 0; f < NPF; ++f) src[f] = field<L>(P.pr, P.pi, f, n3);
     src[NPF] = P.chi;
 
-    // Global offsets (within a plane) of this thread's loads.
+    // In-plane byte offsets of this thread's loads.
     //   interior: rows ty*RJ + rr, column tx
     //   halo (wave 0: row -1, wave 1: row nj, wave 2: column -1, wave 3: column nk)
     const bool kin = tx < nk;
-    int hoff = -1, hrow = 0, hcol = 0;  // LDS destination of the halo value
+    bool hin = false;
+    // lanes without a halo job load a valid in-plane address and discard it,
+    // so every global load is unconditional (no per-load branches)
+    uint32_t hoff = uint32_t(min(j0, N - 1)) * rowB + uint32_t(min(k0 + tx, N - 1)) * 4u;
+    int hrow = 0, hcol = 0;  // LDS destination of the halo value
     if (ty == 0 && kin) {
-        hoff = ((j0 - 1 + N) % N) * N + k0 + tx; hrow = 0; hcol = tx + 1;
+        hin = true; hoff = ((j0 - 1 + N) % N) * rowB + (k0 + tx) * 4u; hrow = 0; hcol = tx + 1;
     } else if (ty == 1 && kin) {
-        hoff = ((j0 + nj) % N) * N + k0 + tx; hrow = nj + 1; hcol = tx + 1;
+        hin = true; hoff = ((j0 + nj) % N) * rowB + (k0 + tx) * 4u; hrow = nj + 1; hcol = tx + 1;
     } else if (ty == 2 && tx < nj + 2) {
-        hoff = ((j0 - 1 + tx + N) % N) * N + (k0 - 1 + N) % N; hrow = tx; hcol = 0;
+        hin = true; hoff = ((j0 - 1 + tx + N) % N) * rowB + ((k0 - 1 + N) % N) * 4u;
+        hrow = tx; hcol = 0;
     } else if (ty == 3 && tx < nj + 2) {
-        hoff = ((j0 - 1 + tx + N) % N) * N + (k0 + nk) % N; hrow = tx; hcol = nk + 1;
+        hin = true; hoff = ((j0 - 1 + tx + N) % N) * rowB + ((k0 + nk) % N) * 4u;
+        hrow = tx; hcol = nk + 1;
     }
     bool rin[RJ];
+    uint32_t loff[RJ];   // load offsets, clamped into the lattice for idle lanes
+    const uint32_t ioff0 = uint32_t(j0 + ty * RJ) * rowB + uint32_t(k0 + tx) * 4u;
 #pragma unroll
-    for (int rr = 0; rr < RJ; ++rr) rin[rr] = kin && (ty * RJ + rr) < nj;
+    for (int rr = 0; rr < RJ; ++rr) {
+        rin[rr] = kin && (ty * RJ + rr) < nj;
+        loff[rr] = uint32_t(min(j0 + ty * RJ + rr, N - 1)) * rowB + uint32_t(min(k0 + tx, N - 1)) * 4u;
+    }
 
     float pre[NF][RJ + 1];
     auto prefetch = [&](int q) {
-        const size_t base = size_t((q + N) % N) * NN;
+        const uint32_t pb = uint32_t((q + N) % N) * planeB;
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
 #pragma unroll
             for (int rr = 0; rr < RJ; ++rr)
-                pre[f][rr] = rin[rr] ? src[f][base + size_t(j0 + ty * RJ + rr) * N + k0 + tx] : 0.0f;
-            pre[f][RJ] = hoff >= 0 ? src[f][base + hoff] : 0.0f;
+                pre[f][rr] = Off32{pb + loff[rr]}.ld(src[f]);
+            pre[f][RJ] = Off32{pb + hoff}.ld(src[f]);
         }
     };
     auto stage = [&]() {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
+        for (int rr = 0; rr < RJ; ++rr)
+            if (rin[rr]) {
 #pragma unroll
-            for (int rr = 0; rr < RJ; ++rr)
-                if (rin[rr]) lds[f][ty * RJ + rr + 1][tx + 1] = pre[f][rr];
-            if (hoff >= 0) lds[f][hrow][hcol] = pre[f][RJ];
+                for (int f = 0; f < NF; ++f) lds[ty * RJ + rr + 1][tx + 1][f] = pre[f][rr];
+            }
+        if (hin) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) lds[hrow][hcol][f] = pre[f][RJ];
         }
     };
 
@@ -352,6 +391,7 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nk
         stage();
         __syncthreads();
         if (s + 1 < cnt) prefetch(q + 1);
+        const uint32_t pend_pb = uint32_t((q - 1 + N) % N) * planeB;
 #pragma unroll
         for (int rr = 0; rr < RJ; ++rr) {
             const int r = ty * RJ + rr + 1;  // LDS row of this site
@@ -359,12 +399,12 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nk
             float A[NF], B[NF], cq[NF], dy[NPF], dz[NPF];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
-                const float ctr = lds[f][r][x];
-                const float nn = lds[f][r - 1][x], ss = lds[f][r + 1][x];
-                const float ww = lds[f][r][x - 1], ee = lds[f][r][x + 1];
+                const float ctr = lds[r][x][f];
+                const float nn = lds[r - 1][x][f], ss = lds[r + 1][x][f];
+                const float ww = lds[r][x - 1][f], ee = lds[r][x + 1][f];
                 const float faces = (nn + ss) + (ww + ee);
-                const float diag = (lds[f][r - 1][x - 1] + lds[f][r - 1][x + 1]) +
-                                   (lds[f][r + 1][x - 1] + lds[f][r + 1][x + 1]);
+                const float diag = (lds[r - 1][x - 1][f] + lds[r - 1][x + 1][f]) +
+                                   (lds[r + 1][x - 1][f] + lds[r + 1][x + 1][f]);
                 A[f] = WF * faces + WE * diag - 4.0f * ctr;
                 B[f] = WF * ctr + WE * faces;
                 cq[f] = ctr;
@@ -404,9 +444,8 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nk
                         if (kp.ks > 0.0f) ccv = ccvyz[rr] + (qx - (1.0f / 3.0f) * sx * sx);
                     }
                 }
-                const size_t idx = size_t((q - 1 + N) % N) * NN +
-                                   size_t(j0 + ty * RJ + rr) * N + k0 + tx;
-                site_update<L>(kp, P, idx, cpend, lap, chim1[rr], lap_chi, jsum, ccv);
+                site_update<L>(kp, P, Off32{pend_pb + ioff0 + rr * rowB}, cpend, lap, chim1[rr],
+                               lap_chi, jsum, ccv);
             }
             // Advance the queue.
 #pragma unroll
@@ -433,7 +472,7 @@ int choose_chunk(int N, int tiles) {
     return (N + nch - 1) / nch;
 }
 
-template <int L, int BJ, int RJ>
+template <int L, int BJ, int RJ, int WPE>
 int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
     constexpr int TJ = BJ * RJ;
     const int N = kp.N;
@@ -442,7 +481,7 @@ int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
     const int CH = choose_chunk(N, nkt * njt);
     const int nch = (N + CH - 1) / CH;
     const long long nblocks = (long long)nkt * njt * nch;
-    hipLaunchKernelGGL((step_tiled<L, BJ, RJ>), dim3((unsigned)nblocks), dim3(64 * BJ), 0, st, P,
+    hipLaunchKernelGGL((step_tiled<L, BJ, RJ, WPE>), dim3((unsigned)nblocks), dim3(64 * BJ), 0, st, P,
                        kp, nkt, njt, CH);
     LFM_LAUNCH_CHECK("step_tiled");
     return LFM_OK;
@@ -450,14 +489,14 @@ int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
 
 template <int L>
 int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
-    if (kp.flags & LFM_F_DIRECT) {
+    if ((kp.flags & LFM_F_DIRECT) || kp.N > 1023) {
         const unsigned nb = unsigned((P.n3 + 255) / 256);
         hipLaunchKernelGGL((step_direct<L>), dim3(nb), dim3(256), 0, st, P, kp);
         LFM_LAUNCH_CHECK("step_direct");
         return LFM_OK;
     }
-    if constexpr (L == COLOR) return launch_tiled<L, 4, 2>(P, kp, st);
-    else return launch_tiled<L, 4, 4>(P, kp, st);
+    if constexpr (L == COLOR) return launch_tiled<L, 4, 2, 1>(P, kp, st);
+    else return launch_tiled<L, 4, 4, 1>(P, kp, st);
 }
 
 KParams to_k(const lfm_params_t* p) {
