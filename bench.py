@@ -144,10 +144,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="color512", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="color512", choices=sorted(WORKLOADS) + ["copy"])
     ap.add_argument("--size", type=int, default=None, help="override the lattice size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--direct", action="store_true", help="use the untiled kernel")
+    ap.add_argument("--tile", default="auto", help="tile shape 1..6, 0 default, or auto (tune)")
+    ap.add_argument("--chunk", type=int, default=0, help="i planes per workgroup (0 auto)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the leapfrog path here runs on a single MI355X "
@@ -157,6 +159,24 @@ def main():
 
     import torch
     import lfm_hip as L
+
+    if args.workload == "copy":
+        # Device-to-device copy of a known size: the HBM calibration for the
+        # rocprof FETCH_SIZE / WRITE_SIZE counters and the measured copy rate.
+        n = (args.size or 512) ** 3
+        x = torch.ones(n, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(args.warmup):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        print(json.dumps({"workload": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                          "ms": ms, "GBps": 8 * n / (ms * 1e-3) / 1e9}), flush=True)
+        return
 
     w = WORKLOADS[args.workload]
     N = args.size or w["N"]
@@ -170,6 +190,11 @@ def main():
     del pr, pi
     torch.cuda.synchronize()
 
+    tuning = None
+    if args.tile == "auto" and not args.direct:
+        tuning = ev.autotune(chunks=(args.chunk,))
+    elif not args.direct:
+        ev.tile, ev.chunk = int(args.tile), args.chunk
     step = (lambda: ev._step(direct=True)) if args.direct else ev._step
     for _ in range(args.warmup):
         step()
@@ -209,6 +234,8 @@ def main():
                      "kernel": f"step_tiled<{['REAL', 'COMPLEX', 'COLOR'][w['level']]}>",
                      "kernel_ms": kms, "bytes_per_site": w["b_alg"]},
         "finite": ok,
+        "tile": {"shape": ev.tile, "chunk": ev.chunk,
+                 "tuned_ms": {f"{t}/{c}": round(v, 4) for (t, c), v in tuning.items()} if tuning else None},
     }
     if not args.no_cpu_baseline and args.size is None:
         cb = cpu_baseline(w)

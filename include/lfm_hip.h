@@ -45,14 +45,24 @@ extern "C" {
                                  (three-buffer rotation; see DESIGN) */
 #define LFM_F_FREEZE_CHI 0x04 /* GOV-01 only: chi is read, never written */
 #define LFM_F_DIRECT 0x08     /* force the untiled one-site-per-thread kernel */
+#define LFM_F_CHI_DELTA 0x10  /* every chi buffer (in and out) holds chi - chi0: the
+                                 leapfrog accumulates the small deviation, which
+                                 float32 resolves ~1e6 times finer than chi ~ 19 */
+
+/* Number of tile shapes of the tiled kernel (lfm_params_t.tile = 1..this). */
+#define LFM_TILE_VARIANTS 6
 
 /* Scalar parameters of one step (the Evolver's per-step arguments,
- * lfm/core/evolver.py:201-279).  Unused terms are 0. */
+ * lfm/core/evolver.py:201-279).  Unused terms are 0.
+ * tile:  0 = the level's default tile shape, 1..LFM_TILE_VARIANTS a specific
+ *        one ({4,1} {4,2} {4,4} {8,1} {8,2} {16,1} wavefronts x rows/thread).
+ * chunk: 0 = automatic, else the number of i planes one workgroup marches. */
 typedef struct lfm_params {
     int32_t N;
     int32_t flags;
     float dt2, kappa, lambda_self, chi0, e0_sq, epsilon_w;
     float kappa_c, epsilon_cc, kappa_string, kappa_tube;
+    int32_t tile, chunk;
 } lfm_params_t;
 
 int lfm_abi_version(void);
@@ -99,6 +109,15 @@ int lfm_scv_apply_psisq(void* plan, const float* e, float gamma, float D, float*
 /* e computed from pr, pi first (into the plan's own buffer). */
 int lfm_scv_apply(void* plan, const float* pr, const float* pi, float gamma, float D,
                   float* sa_out, void* stream);
+/* GOV-04 chi equilibration (lfm/fields/equilibrium.py:26-59,62-100, used by
+ * Simulation.equilibrate, lfm/simulation.py:877-963): solves
+ * lap(delta) = kappa (|Psi|^2 - e0_sq) spectrally with -1/k^2 (DC dropped),
+ * chi_out = chi0 + delta, and chi_out = chi0 outside the interior ball when
+ * boundary_fraction > 0 (FROZEN boundary).  |Psi|^2 sums ncomp components of
+ * pr (and pi when not NULL).  Needs a plan created with ncolors = 1. */
+int lfm_poisson_chi_f32(void* plan, const float* pr, const float* pi, int32_t ncomp,
+                        float kappa, float e0_sq, float chi0, double boundary_fraction,
+                        float* chi_out, void* stream);
 /* Elementwise max(x, 0) into y (n floats). */
 int lfm_clip_nonneg_f32(const float* x, float* y, int64_t n, void* stream);
 
@@ -110,13 +129,15 @@ int lfm_clip_nonneg_f32(const float* x, float* y, int64_t n, void* stream);
  * out (device, 4 doubles): sum T, sum G, sum V, sum |Psi|^2. */
 int64_t lfm_reduce_partials_len(int32_t N);
 int lfm_energy_f64(const float* pr, const float* pr_prev, const float* pi,
-                   const float* pi_prev, const float* chi, const float* mask,
-                   int32_t damp_prev, int32_t ncomp, int32_t N, double dt, double c,
-                   double* partials, double* out, void* stream);
-/* chi statistics over the interior ball dist((i,j,k), N/2) <= (1-bf) N/2.
+                   const float* pi_prev, const float* chi, double chi_offset,
+                   const float* mask, int32_t damp_prev, int32_t ncomp, int32_t N, double dt,
+                   double c, double* partials, double* out, void* stream);
+/* chi_offset is added to every stored chi value (chi0 for deviation
+ * storage, else 0).
+ * chi statistics over the interior ball dist((i,j,k), N/2) <= (1-bf) N/2.
  * out (device, 7 doubles): min, max, mean, std, well_fraction, void_fraction,
  * interior count. */
-int lfm_chi_stats_f64(const float* chi, int32_t N, double boundary_fraction,
+int lfm_chi_stats_f64(const float* chi, double chi_offset, int32_t N, double boundary_fraction,
                       double well_threshold, double void_threshold, double* partials,
                       double* out, void* stream);
 

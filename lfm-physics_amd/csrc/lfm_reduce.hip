@@ -69,7 +69,7 @@ __device__ __forceinline__ double grad1(int p, int n, double fm, double f0, doub
 
 struct EArgs {
     const float* pr; const float* prp; const float* pi; const float* pip;
-    const float* chi; const float* mask;
+    const float* chi; double chi_off; const float* mask;
     int damp, ncomp, N;
     double dt, c;
 };
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(RB) void energy_partial(EArgs a, double* partials) 
                 psq += f * f;
             }
         }
-        const double x = a.chi[idx];
+        const double x = a.chi_off + double(a.chi[idx]);
         T += t;
         G += 0.5 * a.c * a.c * g;
         V += 0.5 * (x * x) * psq;
@@ -148,15 +148,15 @@ __device__ __forceinline__ bool interior(int i, int j, int k, int N, double bf) 
 }
 
 // pass 1: count, sum, min, max, wells, voids
-__global__ __launch_bounds__(RB) void chi_pass1(const float* chi, int N, double bf, double wt,
-                                                double vt, double* partials) {
+__global__ __launch_bounds__(RB) void chi_pass1(const float* chi, double off, int N, double bf,
+                                                double wt, double vt, double* partials) {
     const size_t n3 = size_t(N) * N * N;
     double cnt = 0, sum = 0, mn = DBL_MAX, mx = -DBL_MAX, wl = 0, vd = 0;
     for (size_t idx = size_t(blockIdx.x) * RB + threadIdx.x; idx < n3;
          idx += size_t(gridDim.x) * RB) {
         const int i = int(idx / (size_t(N) * N)), j = int((idx / N) % N), k = int(idx % N);
         if (!interior(i, j, k, N, bf)) continue;
-        const double x = chi[idx];
+        const double x = off + double(chi[idx]);
         cnt += 1.0; sum += x; mn = fmin(mn, x); mx = fmax(mx, x);
         wl += x < wt ? 1.0 : 0.0;
         vd += x > vt ? 1.0 : 0.0;
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(RB) void chi_final1(const double* partials, int nbl
 }
 
 // pass 2: sum (x - mean)^2
-__global__ __launch_bounds__(RB) void chi_pass2(const float* chi, int N, double bf,
+__global__ __launch_bounds__(RB) void chi_pass2(const float* chi, double off, int N, double bf,
                                                 const double* out, double* partials) {
     const size_t n3 = size_t(N) * N * N;
     const double mean = out[2];
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(RB) void chi_pass2(const float* chi, int N, double 
          idx += size_t(gridDim.x) * RB) {
         const int i = int(idx / (size_t(N) * N)), j = int((idx / N) % N), k = int(idx % N);
         if (!interior(i, j, k, N, bf)) continue;
-        const double d = double(chi[idx]) - mean;
+        const double d = (off + double(chi[idx])) - mean;
         s += d * d;
     }
     double v[1] = {s};
@@ -234,9 +234,9 @@ extern "C" int64_t lfm_reduce_partials_len(int32_t N) {
 }
 
 extern "C" int lfm_energy_f64(const float* pr, const float* pr_prev, const float* pi,
-                              const float* pi_prev, const float* chi, const float* mask,
-                              int32_t damp_prev, int32_t ncomp, int32_t N, double dt, double c,
-                              double* partials, double* out, void* stream) {
+                              const float* pi_prev, const float* chi, double chi_offset,
+                              const float* mask, int32_t damp_prev, int32_t ncomp, int32_t N,
+                              double dt, double c, double* partials, double* out, void* stream) {
     if (!pr || !pr_prev || !chi || !partials || !out || ncomp < 1 || N < 2 || !(dt > 0.0)) {
         set_error("lfm_energy_f64: bad argument");
         return LFM_E_ARG;
@@ -245,7 +245,7 @@ extern "C" int lfm_energy_f64(const float* pr, const float* pr_prev, const float
         set_error("lfm_energy_f64: pi and pi_prev must both be given or both NULL");
         return LFM_E_ARG;
     }
-    EArgs a{pr, pr_prev, pi, pi_prev, chi, mask, damp_prev, ncomp, N, dt, c};
+    EArgs a{pr, pr_prev, pi, pi_prev, chi, chi_offset, mask, damp_prev, ncomp, N, dt, c};
     hipStream_t st = (hipStream_t)stream;
     const int nb = nblocks_for(N);
     hipLaunchKernelGGL(energy_partial, dim3(nb), dim3(RB), 0, st, a, partials);
@@ -255,7 +255,8 @@ extern "C" int lfm_energy_f64(const float* pr, const float* pr_prev, const float
     return LFM_OK;
 }
 
-extern "C" int lfm_chi_stats_f64(const float* chi, int32_t N, double boundary_fraction,
+extern "C" int lfm_chi_stats_f64(const float* chi, double chi_offset, int32_t N,
+                                 double boundary_fraction,
                                  double well_threshold, double void_threshold, double* partials,
                                  double* out, void* stream) {
     if (!chi || !partials || !out || N < 2) {
@@ -264,12 +265,12 @@ extern "C" int lfm_chi_stats_f64(const float* chi, int32_t N, double boundary_fr
     }
     hipStream_t st = (hipStream_t)stream;
     const int nb = nblocks_for(N);
-    hipLaunchKernelGGL(chi_pass1, dim3(nb), dim3(RB), 0, st, chi, N, boundary_fraction,
+    hipLaunchKernelGGL(chi_pass1, dim3(nb), dim3(RB), 0, st, chi, chi_offset, N, boundary_fraction,
                        well_threshold, void_threshold, partials);
     LFM_LAUNCH_CHECK("chi_pass1");
     hipLaunchKernelGGL(chi_final1, dim3(1), dim3(RB), 0, st, (const double*)partials, nb, out);
     LFM_LAUNCH_CHECK("chi_final1");
-    hipLaunchKernelGGL(chi_pass2, dim3(nb), dim3(RB), 0, st, chi, N, boundary_fraction,
+    hipLaunchKernelGGL(chi_pass2, dim3(nb), dim3(RB), 0, st, chi, chi_offset, N, boundary_fraction,
                        (const double*)out, partials);
     LFM_LAUNCH_CHECK("chi_pass2");
     hipLaunchKernelGGL(chi_final2, dim3(1), dim3(RB), 0, st, (const double*)partials, nb, out);

@@ -80,6 +80,65 @@ __global__ __launch_bounds__(256) void helmholtz_filter(float2* __restrict__ spe
     }
 }
 
+// rhs = kappa * (sum_c (pr_c^2 + pi_c^2) - e0_sq)   (GOV-04 source)
+__global__ __launch_bounds__(256) void poisson_rhs(const float* __restrict__ pr,
+                                                   const float* __restrict__ pi, int ncomp,
+                                                   float kappa, float e0_sq,
+                                                   float* __restrict__ rhs, size_t n3) {
+    for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n3;
+         t += size_t(gridDim.x) * blockDim.x) {
+        float s = 0.0f;
+        for (int c = 0; c < ncomp; ++c) {
+            const float r = pr[c * n3 + t];
+            s += r * r;
+            if (pi) { const float m = pi[c * n3 + t]; s += m * m; }
+        }
+        rhs[t] = kappa * (s - e0_sq);
+    }
+}
+
+// Multiply the half spectrum by -1/(N^3 k^2) with the DC term set to 0.
+__global__ __launch_bounds__(256) void poisson_filter(float2* __restrict__ spec, int N,
+                                                      float inv_n3) {
+    const int H = N / 2 + 1;
+    const size_t n = size_t(N) * N * H;
+    const float two_pi_over_n = 6.283185307179586f / float(N);
+    for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+         t += size_t(gridDim.x) * blockDim.x) {
+        const int kk = int(t % H);
+        const int j = int((t / H) % N);
+        const int i = int(t / (size_t(H) * N));
+        const int fi = i < (N + 1) / 2 ? i : i - N;
+        const int fj = j < (N + 1) / 2 ? j : j - N;
+        const float kx = two_pi_over_n * float(fi), ky = two_pi_over_n * float(fj);
+        const float kz = two_pi_over_n * float(kk);
+        const float k2 = kx * kx + ky * ky + kz * kz;
+        const float h = t == 0 ? 0.0f : -inv_n3 / k2;
+        float2 v = spec[t];
+        v.x *= h;
+        v.y *= h;
+        spec[t] = v;
+    }
+}
+
+// chi = chi0 + delta; outside the interior ball (dist > (1-bf) N/2, float64
+// distance as lfm/analysis/structure.py:148-171) chi = chi0 when bf > 0.
+__global__ __launch_bounds__(256) void poisson_finish(float* __restrict__ chi, int N, float chi0,
+                                                      double bf) {
+    const size_t n3 = size_t(N) * N * N;
+    const double c = N / 2.0;
+    for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n3;
+         t += size_t(gridDim.x) * blockDim.x) {
+        float v = chi0 + chi[t];
+        if (bf > 0.0) {
+            const int i = int(t / (size_t(N) * N)), j = int((t / N) % N), k = int(t % N);
+            const double dx = double(i) - c, dy = double(j) - c, dz = double(k) - c;
+            if (!(sqrt(dx * dx + dy * dy + dz * dz) <= c * (1.0 - bf))) v = chi0;
+        }
+        chi[t] = v;
+    }
+}
+
 __global__ __launch_bounds__(256) void clip_kernel(const float* __restrict__ x,
                                                    float* __restrict__ y, size_t n) {
     for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
@@ -219,5 +278,39 @@ extern "C" int lfm_clip_nonneg_f32(const float* x, float* y, int64_t n, void* st
     hipLaunchKernelGGL(clip_kernel, dim3(grid_for(size_t(n))), dim3(256), 0, (hipStream_t)stream,
                        x, y, size_t(n));
     LFM_LAUNCH_CHECK("clip_kernel");
+    return LFM_OK;
+}
+
+extern "C" int lfm_poisson_chi_f32(void* plan, const float* pr, const float* pi, int32_t ncomp,
+                                   float kappa, float e0_sq, float chi0,
+                                   double boundary_fraction, float* chi_out, void* stream) {
+    auto* p = static_cast<ScvPlan*>(plan);
+    if (!p || !pr || !chi_out || ncomp < 1) {
+        set_error("lfm_poisson_chi_f32: bad argument");
+        return LFM_E_ARG;
+    }
+    if (p->nc != 1) {
+        set_error("lfm_poisson_chi_f32: needs a plan created with ncolors = 1 (got %d)", p->nc);
+        return LFM_E_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n3 = size_t(p->N) * p->N * p->N;
+    hipLaunchKernelGGL(poisson_rhs, dim3(grid_for(n3)), dim3(256), 0, st, pr, pi, ncomp, kappa,
+                       e0_sq, p->e, n3);
+    LFM_LAUNCH_CHECK("poisson_rhs");
+    LFM_FFT_TRY(rocfft_execution_info_set_stream(p->info, st));
+    void* in_f[1] = {p->e};
+    void* out_f[1] = {p->spec};
+    LFM_FFT_TRY(rocfft_execute(p->fwd, in_f, out_f, p->info));
+    const size_t ns = size_t(p->N) * p->N * (p->N / 2 + 1);
+    hipLaunchKernelGGL(poisson_filter, dim3(grid_for(ns)), dim3(256), 0, st, p->spec, p->N,
+                       float(1.0 / double(n3)));
+    LFM_LAUNCH_CHECK("poisson_filter");
+    void* in_i[1] = {p->spec};
+    void* out_i[1] = {chi_out};
+    LFM_FFT_TRY(rocfft_execute(p->inv, in_i, out_i, p->info));
+    hipLaunchKernelGGL(poisson_finish, dim3(grid_for(n3)), dim3(256), 0, st, chi_out, p->N, chi0,
+                       boundary_fraction);
+    LFM_LAUNCH_CHECK("poisson_finish");
     return LFM_OK;
 }

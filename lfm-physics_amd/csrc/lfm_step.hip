@@ -41,6 +41,7 @@ template <> struct Lv<COLOR> { static constexpr int NPF = 6, NC = 3; };
 struct KParams {
     int N, flags;
     float dt2, kappa, lam, chi0, e0sq, epsw, kc, ecc, ks, kt;
+    int tile, chunk;
 };
 
 struct Ptrs {
@@ -102,14 +103,18 @@ struct Off64 {
 //  c[f]     Psi component values at the site      lap[f]  their Laplacians
 //  jsum     sum over colours and directions of j_{a,d}   (COMPLEX/COLOR)
 //  ccv      sum_d (sum_a j_{a,d}^2 - 1/3 (sum_a j_{a,d})^2)  (COLOR)
+//  chi_st, lap_chi  chi as stored and the Laplacian of the stored field
 template <int L, class O>
 __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O idx,
                                             const float (&c)[Lv<L>::NPF],
-                                            const float (&lap)[Lv<L>::NPF], float chi_c,
+                                            const float (&lap)[Lv<L>::NPF], float chi_st,
                                             float lap_chi, float jsum, float ccv) {
     constexpr int NPF = Lv<L>::NPF;
     constexpr int C = Lv<L>::NC;
     const size_t n3 = P.n3;
+    // chi_st is chi as stored: chi itself, or chi - chi0 with LFM_F_CHI_DELTA.
+    const bool delta = (kp.flags & LFM_F_CHI_DELTA) != 0;
+    const float chi_c = delta ? kp.chi0 + chi_st : chi_st;
     const bool has_mask = (kp.flags & LFM_F_MASK) != 0;
     const float m = has_mask ? idx.ld(P.mask) : 0.0f;
     const float a = 1.0f - m;
@@ -183,14 +188,19 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
             }
         }
     }
-    const float chi_self = -4.0f * kp.lam * chi_c * (chi_sq - kp.chi0 * kp.chi0);
+    // chi_self = -4 lam chi (chi^2 - chi0^2); with deviation storage the
+    // factor chi^2 - chi0^2 is formed as d (2 chi0 + d), free of cancellation.
+    const float hat = delta ? chi_st * (2.0f * kp.chi0 + chi_st) : chi_sq - kp.chi0 * kp.chi0;
+    const float chi_self = -4.0f * kp.lam * chi_c * hat;
     float accel = lap_chi - src - cv + chi_self;
     if constexpr (L == COLOR) accel = accel - kp.ks * ccv - kp.kt * scv;
-    float cn = 2.0f * chi_c - idx.ld(P.chip) + kp.dt2 * accel;
-    if (cn < -kp.chi0) cn = -kp.chi0;   // BH excision clamp (NaN passes through)
-    cn = m * kp.chi0 + a * cn;
+    // leapfrog in storage units (a constant offset drops out of 2c - c_prev and of lap)
+    float cn = 2.0f * chi_st - idx.ld(P.chip) + kp.dt2 * accel;
+    const float floor_v = delta ? -2.0f * kp.chi0 : -kp.chi0;
+    if (cn < floor_v) cn = floor_v;   // BH excision clamp chi >= -chi0 (NaN passes through)
+    cn = delta ? a * cn : m * kp.chi0 + a * cn;   // frozen boundary: chi -> chi0
     idx.st(P.chi_out, cn);
-    if (P.chip_out) idx.st(P.chip_out, chi_c);
+    if (P.chip_out) idx.st(P.chip_out, chi_st);
 }
 
 // ---------------------------------------------------------------------------
@@ -478,7 +488,7 @@ int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
     const int N = kp.N;
     const int nkt = (N + TK - 1) / TK;
     const int njt = (N + TJ - 1) / TJ;
-    const int CH = choose_chunk(N, nkt * njt);
+    const int CH = kp.chunk > 0 ? min(kp.chunk, N) : choose_chunk(N, nkt * njt);
     const int nch = (N + CH - 1) / CH;
     const long long nblocks = (long long)nkt * njt * nch;
     hipLaunchKernelGGL((step_tiled<L, BJ, RJ, WPE>), dim3((unsigned)nblocks), dim3(64 * BJ), 0, st, P,
@@ -495,8 +505,19 @@ int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
         LFM_LAUNCH_CHECK("step_direct");
         return LFM_OK;
     }
-    if constexpr (L == COLOR) return launch_tiled<L, 4, 2, 1>(P, kp, st);
-    else return launch_tiled<L, 4, 4, 1>(P, kp, st);
+    // Tile shapes (wavefronts BJ x rows per thread RJ); 0 picks the level's default.
+    int tile = kp.tile > 0 ? kp.tile : (L == COLOR ? 2 : 3);
+    switch (tile) {
+        case 1: return launch_tiled<L, 4, 1, 1>(P, kp, st);
+        case 2: return launch_tiled<L, 4, 2, 1>(P, kp, st);
+        case 3: return launch_tiled<L, 4, 4, 1>(P, kp, st);
+        case 4: return launch_tiled<L, 8, 1, 1>(P, kp, st);
+        case 5: return launch_tiled<L, 8, 2, 1>(P, kp, st);
+        case 6: return launch_tiled<L, 16, 1, 1>(P, kp, st);
+        default:
+            set_error("tile variant %d unknown (1..%d)", tile, LFM_TILE_VARIANTS);
+            return LFM_E_ARG;
+    }
 }
 
 KParams to_k(const lfm_params_t* p) {
@@ -504,6 +525,7 @@ KParams to_k(const lfm_params_t* p) {
     k.N = p->N; k.flags = p->flags; k.dt2 = p->dt2; k.kappa = p->kappa; k.lam = p->lambda_self;
     k.chi0 = p->chi0; k.e0sq = p->e0_sq; k.epsw = p->epsilon_w; k.kc = p->kappa_c;
     k.ecc = p->epsilon_cc; k.ks = p->kappa_string; k.kt = p->kappa_tube;
+    k.tile = p->tile; k.chunk = p->chunk;
     return k;
 }
 

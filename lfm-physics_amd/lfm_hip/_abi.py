@@ -22,6 +22,8 @@ LFM_F_MASK = 0x01
 LFM_F_DAMP_PREV = 0x02
 LFM_F_FREEZE_CHI = 0x04
 LFM_F_DIRECT = 0x08
+LFM_F_CHI_DELTA = 0x10
+LFM_TILE_VARIANTS = 6
 
 # (name, restype, argtypes) for every entry point of include/lfm_hip.h
 _STEP_REAL = [_vp] * 9 + [_vp, _vp]
@@ -37,11 +39,12 @@ SIGNATURES = [
     ("lfm_scv_plan_destroy", ctypes.c_int, [_vp]),
     ("lfm_scv_apply_psisq", ctypes.c_int, [_vp, _vp, _f32, _f32, _vp, _vp]),
     ("lfm_scv_apply", ctypes.c_int, [_vp, _vp, _vp, _f32, _f32, _vp, _vp]),
+    ("lfm_poisson_chi_f32", ctypes.c_int, [_vp, _vp, _vp, _i32, _f32, _f32, _f32, _f64, _vp, _vp]),
     ("lfm_clip_nonneg_f32", ctypes.c_int, [_vp, _vp, _i64, _vp]),
     ("lfm_reduce_partials_len", _i64, [_i32]),
     ("lfm_energy_f64", ctypes.c_int,
-     [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _f64, _f64, _vp, _vp, _vp]),
-    ("lfm_chi_stats_f64", ctypes.c_int, [_vp, _i32, _f64, _f64, _f64, _vp, _vp, _vp]),
+     [_vp, _vp, _vp, _vp, _vp, _f64, _vp, _i32, _i32, _i32, _f64, _f64, _vp, _vp, _vp]),
+    ("lfm_chi_stats_f64", ctypes.c_int, [_vp, _f64, _i32, _f64, _f64, _f64, _vp, _vp, _vp]),
 ]
 
 
@@ -51,7 +54,8 @@ class LfmParams(ctypes.Structure):
     _fields_ = [("N", _i32), ("flags", _i32),
                 ("dt2", _f32), ("kappa", _f32), ("lambda_self", _f32), ("chi0", _f32),
                 ("e0_sq", _f32), ("epsilon_w", _f32), ("kappa_c", _f32),
-                ("epsilon_cc", _f32), ("kappa_string", _f32), ("kappa_tube", _f32)]
+                ("epsilon_cc", _f32), ("kappa_string", _f32), ("kappa_tube", _f32),
+                ("tile", _i32), ("chunk", _i32)]
 
 
 class LfmError(RuntimeError):

@@ -40,12 +40,14 @@ def _require(t, n: int, what: str) -> None:
 
 
 def params(N, dt2, kappa, lambda_self, chi0, e0_sq, epsilon_w=0.0, kappa_c=0.0,
-           epsilon_cc=0.0, kappa_string=0.0, kappa_tube=0.0, flags=0) -> _abi.LfmParams:
+           epsilon_cc=0.0, kappa_string=0.0, kappa_tube=0.0, flags=0, tile=0,
+           chunk=0) -> _abi.LfmParams:
     """lfm_params_t with every scalar rounded to float32, as the CuPy backend
     passes them (cupy_backend.py:160-165,213-219,298-309)."""
     return _abi.LfmParams(int(N), int(flags), float(dt2), float(kappa), float(lambda_self),
                           float(chi0), float(e0_sq), float(epsilon_w), float(kappa_c),
-                          float(epsilon_cc), float(kappa_string), float(kappa_tube))
+                          float(epsilon_cc), float(kappa_string), float(kappa_tube),
+                          int(tile), int(chunk))
 
 
 def stream_handle() -> int:

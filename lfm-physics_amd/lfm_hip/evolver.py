@@ -54,9 +54,17 @@ class Evolver:
     mask_policy: "reference" (default) applies the cos^2 mask whatever the
     boundary type, as lfm/core/evolver.py:126 does; "boundary_type" drops it
     for BoundaryType.PERIODIC (truly periodic, mask-free kernel).
+
+    chi_storage: "delta" (default) keeps chi - chi0 in the device buffers.
+    float32 resolves chi ~ 19 only to ~2e-6 absolute, and the leapfrog
+    accumulates that rounding step after step (the dominant float32 error
+    at the REAL level); the deviation is resolved ~1e6 times finer.  Getters
+    and setters still speak absolute chi.  "absolute" stores chi as the
+    reference does.
     """
 
-    def __init__(self, config: SimulationConfig, backend="auto", mask_policy="reference"):
+    def __init__(self, config: SimulationConfig, backend="auto", mask_policy="reference",
+                 chi_storage="delta"):
         self.config = config
         self.backend = get_backend(backend)
         self._lib = _abi.load()
@@ -78,8 +86,12 @@ class Evolver:
         f32 = torch.float32
         self._pr = [torch.zeros(n, dtype=f32, device=dev) for _ in range(3)]
         self._pi = [torch.zeros(n, dtype=f32, device=dev) for _ in range(3)] if self._has_imag else None
-        self._chi = [torch.full((self.total,), float(config.chi0), dtype=f32, device=dev)
-                     for _ in range(3)]
+        if chi_storage not in ("delta", "absolute"):
+            raise ValueError(f"chi_storage must be 'delta' or 'absolute', got {chi_storage!r}")
+        self.chi_storage = chi_storage
+        self._chi_off = float(np.float32(config.chi0)) if chi_storage == "delta" else 0.0
+        self._chi = [torch.full((self.total,), float(config.chi0) - self._chi_off, dtype=f32,
+                                device=dev) for _ in range(3)]
         self._cur, self._prev = 0, 1          # Psi slots
         self._damped = False                   # Psi prev needs (1-m) on read
         self._cc, self._cp, self._oc, self._op = 0, 1, 0, 1   # chi references
@@ -99,6 +111,8 @@ class Evolver:
         else:
             self._sa = None
         self._params = {}
+        self.tile = 0      # tiled-kernel shape (0 = default, see autotune())
+        self.chunk = 0     # i planes per workgroup (0 = automatic)
         self._partials = None
         self._red_out = None
         # Optional list: when set, each step appends a (start, end) event pair
@@ -107,20 +121,55 @@ class Evolver:
 
     # ------------------------------------------------------------------ steps
     def _p(self, flags):
-        p = self._params.get(flags)
+        key = (flags, self.tile, self.chunk)
+        p = self._params.get(key)
         if p is None:
             c = self.config
             p = params(self.N, self._dt2, c.kappa, c.lambda_self, c.chi0, c.e0_sq, c.epsilon_w,
                        c.kappa_c, c.epsilon_cc, c.kappa_string,
-                       c.kappa_tube if self.sa_enabled else 0.0, flags)
-            self._params[flags] = p
+                       c.kappa_tube if self.sa_enabled else 0.0, flags, self.tile, self.chunk)
+            self._params[key] = p
         return p
+
+    def autotune(self, reps: int = 6, chunks=(0,)) -> dict:
+        """Time every tile shape (and the given chunk sizes) of the stencil
+        kernel on the current state and keep the fastest.  The state is
+        saved and restored, so tuning does not advance the simulation.
+        Returns {(tile, chunk): ms per kernel}."""
+        saved = ([t.clone() for t in self._pr], [t.clone() for t in self._pi] if self._pi else None,
+                 [t.clone() for t in self._chi], (self._cur, self._prev, self._damped, self._cc,
+                                                  self._cp, self._oc, self._op))
+        results = {}
+        old = (self.tile, self.chunk, self.kernel_events)
+        for tile in range(1, _abi.LFM_TILE_VARIANTS + 1):
+            for ch in chunks:
+                self.tile, self.chunk = tile, ch
+                self.kernel_events = []
+                for _ in range(reps + 1):
+                    self._step()
+                torch.cuda.synchronize(self.device)
+                ts = [a.elapsed_time(b) for a, b in self.kernel_events[1:]]
+                results[(tile, ch)] = sorted(ts)[len(ts) // 2]
+        best = min(results, key=results.get)
+        self.tile, self.chunk = best
+        self.kernel_events = old[2]
+        for dst, src in zip(self._pr, saved[0]):
+            dst.copy_(src)
+        if saved[1]:
+            for dst, src in zip(self._pi, saved[1]):
+                dst.copy_(src)
+        for dst, src in zip(self._chi, saved[2]):
+            dst.copy_(src)
+        (self._cur, self._prev, self._damped, self._cc, self._cp, self._oc, self._op) = saved[3]
+        if self.sa_enabled:
+            self._psisq_valid = False
+        return results
 
     def _step(self, freeze_chi: bool = False, direct: bool = False) -> None:
         lib = self._lib
         cur, prev = self._cur, self._prev
         nxt = 3 - cur - prev
-        flags = 0
+        flags = _abi.LFM_F_CHI_DELTA if self._chi_off else 0
         if self.has_mask:
             flags |= _abi.LFM_F_MASK
             if self._damped:
@@ -202,10 +251,16 @@ class Evolver:
         return self._pi[self._cur] if self._has_imag else None
 
     def native_chi(self) -> torch.Tensor:
+        """Live device view of the current chi buffer, in storage units
+        (chi - chi0 when chi_storage == "delta")."""
         return self._chi[self._cc]
 
     def native_chi_prev(self) -> torch.Tensor:
         return self._chi[self._cp]
+
+    def chi_tensor(self) -> torch.Tensor:
+        """Absolute chi as a new device tensor."""
+        return self._chi[self._cc] + self._chi_off if self._chi_off else self._chi[self._cc].clone()
 
     def _prev_view(self, pool) -> torch.Tensor:
         t = pool[self._prev]
@@ -220,11 +275,17 @@ class Evolver:
             return flat.reshape(self._n_psi, N, N, N)
         return flat.reshape(N, N, N)
 
+    def _abs_chi(self, slot) -> np.ndarray:
+        t = self._chi[slot]
+        if self._chi_off:
+            t = t + self._chi_off
+        return t.cpu().numpy().reshape(self.N, self.N, self.N)
+
     def get_chi(self) -> np.ndarray:
-        return self._chi[self._cc].cpu().numpy().reshape(self.N, self.N, self.N)
+        return self._abs_chi(self._cc)
 
     def get_chi_prev(self) -> np.ndarray:
-        return self._chi[self._cp].cpu().numpy().reshape(self.N, self.N, self.N)
+        return self._abs_chi(self._cp)
 
     def get_psi_real(self) -> np.ndarray:
         return self._shape(self._pr[self._cur].cpu().numpy())
@@ -252,6 +313,10 @@ class Evolver:
         return torch.clamp_min(self._sa, 0.0).cpu().numpy().reshape(3, self.N, self.N, self.N)
 
     # ------------------------------------------------------------- setters
+    def _dev_chi(self, arr) -> torch.Tensor:
+        d = self._dev(arr)
+        return d - self._chi_off if self._chi_off else d
+
     def _dev(self, arr) -> torch.Tensor:
         if isinstance(arr, torch.Tensor):
             return arr.detach().to(device=self.device, dtype=torch.float32).reshape(-1).contiguous()
@@ -311,19 +376,19 @@ class Evolver:
 
     def set_chi(self, arr) -> None:
         """All four reference chi buffers := arr."""
-        self._chi[0].copy_(self._dev(arr))
+        self._chi[0].copy_(self._dev_chi(arr))
         self._cc = self._cp = self._oc = self._op = 0
 
     def set_chi_current(self, arr) -> None:
         """Both sets' current chi := arr; prev buffers unchanged."""
         s = next(q for q in range(3) if q != self._cp and q != self._op)
-        self._chi[s].copy_(self._dev(arr))
+        self._chi[s].copy_(self._dev_chi(arr))
         self._cc = self._oc = s
 
     def set_chi_prev(self, arr) -> None:
         """Both sets' prev chi := arr; current buffers unchanged."""
         s = next(q for q in range(3) if q != self._cc and q != self._oc)
-        self._chi[s].copy_(self._dev(arr))
+        self._chi[s].copy_(self._dev_chi(arr))
         self._cp = self._op = s
 
     def set_sa_fields(self, arr) -> None:
@@ -356,7 +421,7 @@ class Evolver:
         rc = self._lib.lfm_energy_f64(
             self._pr[self._cur].data_ptr(), prev_r.data_ptr(), pi.data_ptr() if pi is not None else None,
             prev_i.data_ptr() if prev_i is not None else None, self._chi[self._cc].data_ptr(),
-            self.boundary_mask.data_ptr(), damp, self._n_psi, self.N, float(self.config.dt),
+            self._chi_off, self.boundary_mask.data_ptr(), damp, self._n_psi, self.N, float(self.config.dt),
             float(c), part.data_ptr(), out.data_ptr(), stream_handle())
         _abi.check(rc, "lfm_energy_f64")
         T, G, V, S = out[:4].tolist()
@@ -370,7 +435,7 @@ class Evolver:
         """chi min/max/mean/std and well/void fractions over the interior ball
         (lfm/analysis/structure.py:26-108,148-171), FP64 on the GPU."""
         part, out = self._red_bufs()
-        rc = self._lib.lfm_chi_stats_f64(self._chi[self._cc].data_ptr(), self.N,
+        rc = self._lib.lfm_chi_stats_f64(self._chi[self._cc].data_ptr(), self._chi_off, self.N,
                                          float(self.config.boundary_fraction),
                                          float(well_threshold), float(void_threshold),
                                          part.data_ptr(), out.data_ptr(), stream_handle())

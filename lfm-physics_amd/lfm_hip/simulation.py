@@ -22,15 +22,20 @@ from .evolver import Evolver
 
 class Simulation:
     def __init__(self, config: SimulationConfig | None = None, backend="auto",
-                 mask_policy: str = "reference") -> None:
+                 mask_policy: str = "reference", chi_storage: str = "delta") -> None:
         self.config = config if config is not None else SimulationConfig()
-        self._evolver = Evolver(self.config, backend=backend, mask_policy=mask_policy)
+        self._evolver = Evolver(self.config, backend=backend, mask_policy=mask_policy,
+                                chi_storage=chi_storage)
         self._history: list[dict[str, float]] = []
         # device snapshot of Psi at run() start / last callback (the reference's
         # _psi_r_prev / _psi_i_prev, lfm/simulation.py:1168-1182)
         self._snap_r: torch.Tensor | None = None
         self._snap_i: torch.Tensor | None = None
         self._interior = None
+        self._solitons_placed = False
+        self._equilibrated = False
+        self._velocity_boosts: list = []
+        self._poisson = None
 
     # -- fields ---------------------------------------------------------------
     @property
@@ -109,6 +114,101 @@ class Simulation:
     def set_chi(self, v):
         self._evolver.set_chi(v)
 
+    # -- initial conditions -------------------------------------------------
+    def place_soliton(self, position, amplitude=None, sigma=None, phase=0.0, velocity=None):
+        """Add a Gaussian soliton to Psi (lfm/simulation.py:495-625).  At the
+        COLOR level the single-component soliton is added to every colour
+        (NumPy broadcasting in the reference).  A velocity adds the carrier
+        phase k = chi0 v / c and a time-shifted prev field."""
+        from .fields import gaussian_soliton
+        cfg = self.config
+        N = cfg.grid_size
+        amp = amplitude if amplitude is not None else cfg.e_amplitude
+        sig = sigma if sigma is not None else cfg.sigma
+        ev = self._evolver
+        if velocity is not None:
+            import math
+            vx, vy, vz = velocity
+            speed = math.sqrt(vx * vx + vy * vy + vz * vz)
+            if cfg.chi0 * speed / cfg.c > 0.8 * math.pi:
+                raise ValueError(f"Velocity |v|={speed:.4f}c gives carrier k={cfg.chi0 * speed / cfg.c:.2f} "
+                                 f"rad/cell, exceeding 80% of Nyquist; maximum safe speed is "
+                                 f"{0.8 * math.pi * cfg.c / cfg.chi0:.4f}c.")
+            kx, ky, kz = (cfg.chi0 * v / cfg.c for v in velocity)
+            x = np.arange(N, dtype=np.float32)
+            X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+            px, py, pz = position
+
+            def wave(cx, cy, cz, extra):
+                r2 = (X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2
+                env = (amp * np.exp(-r2 / (2.0 * sig ** 2))).astype(np.float32)
+                ph = (phase + kx * (X - cx) + ky * (Y - cy) + kz * (Z - cz) + extra).astype(np.float32)
+                return env, (env * np.cos(ph)).astype(np.float32), (env * np.sin(ph)).astype(np.float32)
+
+            env, pr, pi = wave(px, py, pz, 0.0)
+            omega = float(np.sqrt(kx * kx + ky * ky + kz * kz + cfg.chi0 ** 2))
+            _, pr_prev, pi_prev = wave(px - vx * cfg.dt, py - vy * cfg.dt, pz - vz * cfg.dt,
+                                      omega * cfg.dt)
+        else:
+            pr, pi = gaussian_soliton(N, position, amp, sig, phase)
+        cur_r = ev.get_psi_real()
+        ev.set_psi_real(cur_r + pr)
+        if velocity is not None:
+            ev.set_psi_real_prev(cur_r + pr_prev)
+        if cfg.field_level != FieldLevel.REAL:
+            cur_i = ev.get_psi_imag()
+            ev.set_psi_imag(cur_i + pi)
+            if velocity is not None:
+                ev.set_psi_imag_prev(cur_i + pi_prev)
+        self._solitons_placed = True
+        self._equilibrated = False
+        if velocity is not None:
+            self._velocity_boosts.append((tuple(velocity), env ** 2))
+
+    def equilibrate(self) -> None:
+        """Poisson-equilibrate chi from |Psi|^2 (GOV-04) on the GPU
+        (lfm/simulation.py:877-963, lfm/fields/equilibrium.py:26-100)."""
+        import ctypes
+        from . import _abi
+        from .backend import ScvPlan, stream_handle
+        if not self._solitons_placed:
+            warnings.warn("equilibrate() called before any solitons were placed -- chi will stay "
+                          "flat at chi0.  Call place_soliton() first.", stacklevel=2)
+        cfg = self.config
+        ev = self._evolver
+        if self._poisson is None:
+            self._poisson = ScvPlan(cfg.grid_size, 1)
+        chi_eq = torch.empty(cfg.grid_size ** 3, dtype=torch.float32, device=ev.device)
+        bf = cfg.boundary_fraction if cfg.boundary_type == BoundaryType.FROZEN else -1.0
+        pi = ev.native_psi_imag()
+        rc = ev._lib.lfm_poisson_chi_f32(self._poisson.handle, ev.native_psi_real().data_ptr(),
+                                         pi.data_ptr() if pi is not None else None, ev._n_psi,
+                                         float(cfg.kappa), float(cfg.e0_sq), float(cfg.chi0),
+                                         float(bf), chi_eq.data_ptr(), stream_handle())
+        _abi.check(rc, "lfm_poisson_chi_f32")
+        if self._velocity_boosts:
+            ce = chi_eq.cpu().numpy().reshape((cfg.grid_size,) * 3)
+            ev.set_chi_current(ce)
+            g = [np.gradient(ce, axis=a).astype(np.float32) for a in range(3)]
+            vdg = np.zeros_like(ce)
+            for (vx, vy, vz), rho in self._velocity_boosts:
+                vdg += rho * (vx * g[0] + vy * g[1] + vz * g[2])
+            total = sum(rho for _, rho in self._velocity_boosts)
+            sel = total > 1e-10 * total.max()
+            vdg[sel] /= total[sel]
+            vdg[~sel] = 0.0
+            ev.set_chi_prev(ce + cfg.dt * vdg)
+            self._velocity_boosts.clear()
+        else:
+            ev.set_chi(chi_eq)
+        self._equilibrated = True
+
+    def _auto_equilibrate(self) -> None:
+        if self._solitons_placed and not self._equilibrated:
+            warnings.warn("Solitons were placed but equilibrate() was never called -- "
+                          "auto-equilibrating chi now.", stacklevel=3)
+            self.equilibrate()
+
     # -- evolution ------------------------------------------------------------
     def _snapshot(self) -> None:
         ev = self._evolver
@@ -120,6 +220,7 @@ class Simulation:
             evolve_chi: bool = True) -> None:
         """Advance `steps` steps; a metrics snapshot goes to :attr:`history`
         every report_interval steps (lfm/simulation.py:1123-1188)."""
+        self._auto_equilibrate()
         self._snapshot()
 
         def _cb(ev, step):
@@ -204,7 +305,8 @@ class Simulation:
         np.savez_compressed(str(path), **data)
 
     @classmethod
-    def load_checkpoint(cls, path, backend="auto", exact: bool = False) -> "Simulation":
+    def load_checkpoint(cls, path, backend="auto", exact: bool = False,
+                        **kwargs) -> "Simulation":
         """Restore a checkpoint (lfm/simulation.py:1516-1565).  With
         exact=True and the true prev buffers present the run resumes with
         its time derivatives; otherwise current = prev, as the reference."""
@@ -218,7 +320,7 @@ class Simulation:
             if k in cfg and cfg[k] is not None:
                 from . import config as _c
                 cfg[k] = (_c.PhysicsScale if k == "physical_scale" else _c.ChiMode)(cfg[k])
-        sim = cls(SimulationConfig(**cfg), backend=backend)
+        sim = cls(SimulationConfig(**cfg), backend=backend, **kwargs)
         ev = sim._evolver
         ev.set_psi_real(data["psi_real"])
         if "psi_imag" in data:

@@ -365,6 +365,24 @@ class OracleState:
         return self.sa[self.cur] if self.sa else None
 
 
+def poisson_chi(psi_sq, N, chi0, kappa, e0_sq=0.0, frozen_bf=None):
+    """GOV-04 equilibrated chi: lap(delta) = kappa (|Psi|^2 - E0^2) solved with
+    -1/k^2 (DC dropped), chi = chi0 + delta, chi = chi0 outside the interior
+    ball for a FROZEN boundary.  Follows lfm/fields/equilibrium.py:26-100.
+    """
+    src = kappa * (np.asarray(psi_sq, dtype=np.float64).reshape(N, N, N) - e0_sq)
+    kx = np.fft.fftfreq(N) * 2.0 * np.pi
+    kz = np.fft.rfftfreq(N) * 2.0 * np.pi
+    K2 = kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2
+    K2[0, 0, 0] = 1.0
+    phi_hat = -np.fft.rfftn(src) / K2
+    phi_hat[0, 0, 0] = 0.0
+    chi = chi0 + np.fft.irfftn(phi_hat, s=(N, N, N), axes=(0, 1, 2))
+    if frozen_bf is not None:
+        chi[~interior_mask(N, frozen_bf)] = chi0
+    return chi
+
+
 # ---------------------------------------------------------------------------
 # FP64 analysis reductions (lfm/analysis/energy.py, structure.py)
 # ---------------------------------------------------------------------------

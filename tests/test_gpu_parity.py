@@ -56,12 +56,12 @@ def make_ics(level, N, seed=42):
                                    colors=[0, 1, 2])
 
 
-def run_pair(cfg, steps, mask_policy="reference", direct=False, seed=42):
+def run_pair(cfg, steps, mask_policy="reference", direct=False, seed=42, chi_storage="delta"):
     import lfm_hip as L
     import torch
     lvl = int(cfg.field_level)
     pr, pi = make_ics(lvl, cfg.grid_size, seed)
-    ev = L.Evolver(cfg, backend="hip", mask_policy=mask_policy)
+    ev = L.Evolver(cfg, backend="hip", mask_policy=mask_policy, chi_storage=chi_storage)
     ev.set_psi_real(pr)
     if pi is not None:
         ev.set_psi_imag(pi)
@@ -77,18 +77,26 @@ def run_pair(cfg, steps, mask_policy="reference", direct=False, seed=42):
 
 
 def check_fields(ev, st, tol, tag):
-    errs = {"psi_r": O.rel_l2(ev.get_psi_real(), st.psi_r),
-            "psi_r_prev": O.rel_l2(ev.get_psi_real_prev(), st.psi_r_prev),
-            "chi": O.rel_l2(ev.get_chi(), st.chi),
-            "chi_prev": O.rel_l2(ev.get_chi_prev(), st.chi_prev)}
-    if st.psi_i is not None:
-        errs["psi_i"] = O.rel_l2(ev.get_psi_imag(), st.psi_i)
-        errs["psi_i_prev"] = O.rel_l2(ev.get_psi_imag_prev(), st.psi_i_prev)
+    """Relative L2 of every field against the oracle.  For complex levels Psi
+    is measured as one complex field (real and imaginary parts together): a
+    part whose own norm is small would otherwise inflate its relative error."""
+    cat = lambda *a: np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in a])
+    if st.psi_i is None:
+        errs = {"psi": O.rel_l2(ev.get_psi_real(), st.psi_r),
+                "psi_prev": O.rel_l2(ev.get_psi_real_prev(), st.psi_r_prev)}
+    else:
+        errs = {"psi": O.rel_l2(cat(ev.get_psi_real(), ev.get_psi_imag()),
+                                cat(st.psi_r, st.psi_i)),
+                "psi_prev": O.rel_l2(cat(ev.get_psi_real_prev(), ev.get_psi_imag_prev()),
+                                     cat(st.psi_r_prev, st.psi_i_prev))}
+    errs["chi"] = O.rel_l2(ev.get_chi(), st.chi)
+    errs["chi_prev"] = O.rel_l2(ev.get_chi_prev(), st.chi_prev)
     if st.sa_fields is not None:
         errs["sa"] = O.rel_l2(ev.get_sa_fields(), st.sa_fields)
     log(f"{tag}: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
     for k, v in errs.items():
         assert v <= tol, (tag, k, v)
+    return errs
 
 
 # ----------------------------------------------- Evolver vs FP64 oracle
@@ -104,6 +112,32 @@ def test_evolver_vs_fp64_oracle(gpu, case, direct):
     cfg = [L.gravity_only, L.gravity_em, L.full_physics][lvl](N)
     ev, st = run_pair(cfg, 100, direct=direct)
     check_fields(ev, st, TOL64, name)
+
+
+def test_absolute_chi_storage_matches_fp32_reference_floor(gpu):
+    """chi stored as absolute float32 (the reference's layout) carries the
+    reference's own float32 error: on REAL 64^3 the float32 NumPy oracle
+    is ~1.2e-4 from float64 after 100 steps.  The GPU must match that float32
+    path closely, and the default deviation storage must beat it by >10x."""
+    import lfm_hip as L
+    log("absolute chi storage, REAL 64^3")
+    cfg = L.gravity_only(64)
+    ev, st = run_pair(cfg, 100, chi_storage="absolute")
+    pr, _ = make_ics(0, 64)
+    p = oracle_params(cfg)
+    p["dt2"] = float(np.float32(p["dt2"]))
+    st32 = O.OracleState(0, 64, pr, None, np.full(64 ** 3, 19.0),
+                         O.boundary_mask(64, cfg.boundary_fraction), p, dtype=np.float32)
+    st32.step(100)
+    e_gpu = O.rel_l2(ev.get_psi_real(), st.psi_r)
+    e_f32 = O.rel_l2(st32.psi_r, st.psi_r)
+    e_pair = O.rel_l2(ev.get_psi_real(), st32.psi_r)
+    ev2, _ = run_pair(cfg, 100)
+    e_delta = O.rel_l2(ev2.get_psi_real(), st.psi_r)
+    log(f"  gpu-abs vs f64 {e_gpu:.2e}  f32-oracle vs f64 {e_f32:.2e}  gpu-abs vs f32 {e_pair:.2e}"
+        f"  gpu-delta vs f64 {e_delta:.2e}")
+    assert e_gpu <= 1.5 * e_f32
+    assert e_delta * 10 <= e_f32 and e_delta <= TOL64
 
 
 @pytest.mark.parametrize("lvl", [0, 1, 2])
@@ -347,3 +381,41 @@ def test_backend_rejects_bad_tensors(gpu):
         be.step_real(bad, *good[1:4], None, *good[4:8], N, 4e-4, 1 / 63, 0, 19, 0)
     with pytest.raises(L._abi.LfmError if hasattr(L, "_abi") else Exception):
         be.step_real(good[0], *good[1:4], None, good[0], *good[5:8], N, 4e-4, 1 / 63, 0, 19, 0)
+
+
+@pytest.mark.parametrize("lvl", [0, 1, 2])
+def test_equilibrate_matches_oracle(gpu, lvl):
+    import lfm_hip as L
+    log(f"equilibrate level {lvl}")
+    N = 24
+    cfg = [L.gravity_only, L.gravity_em, L.full_physics][lvl](N)
+    sim = L.Simulation(cfg, backend="hip")
+    sim.place_soliton((12, 11, 12.5), amplitude=3.0, sigma=2.5, phase=0.4)
+    sim.place_soliton((8, 14, 12), amplitude=2.0, sigma=2.0)
+    sim.equilibrate()
+    psq = sim.get_energy_density()
+    ref = O.poisson_chi(psq, N, cfg.chi0, cfg.kappa, cfg.e0_sq, frozen_bf=cfg.boundary_fraction)
+    err = O.rel_l2(sim.chi - cfg.chi0, ref - cfg.chi0)
+    log(f"  delta-chi rel-L2 {err:.2e}")
+    assert err < 1e-5
+    np.testing.assert_array_equal(sim._evolver.get_chi_prev(), sim.chi)
+
+
+def test_high_level_simulation_matches_oracle(gpu):
+    """The reference's end-to-end check (test_gpu_parity.py:446-480): REAL
+    16^3, e_amplitude 6, soliton at the centre, equilibrate, 100 steps."""
+    import lfm_hip as L
+    log("high-level simulation 16^3")
+    cfg = L.SimulationConfig(grid_size=16, e_amplitude=6.0, field_level=L.FieldLevel.REAL)
+    sim = L.Simulation(cfg, backend="hip")
+    sim.place_soliton((8.0, 8.0, 8.0))
+    sim.equilibrate()
+    E0 = sim.psi_real.copy()
+    chi_eq = sim.chi.copy()
+    sim.run(steps=100)
+    st = O.OracleState(0, 16, E0, None, chi_eq, O.boundary_mask(16, cfg.boundary_fraction),
+                       oracle_params(cfg))
+    st.step(100)
+    np.testing.assert_allclose(sim.chi.ravel(), st.chi, atol=5e-4, rtol=1e-3)
+    assert O.rel_l2(sim.chi, st.chi) < TOL64
+    assert O.rel_l2(sim.psi_real, st.psi_r) < TOL64

@@ -301,3 +301,17 @@ def test_interior_mask_and_stats_brute_force():
     w, v = O.well_void_fraction(chi, 17.0, 18.0, m)
     assert w == sum(x < 17 for x in vals) / len(vals)
     assert v == sum(x > 18 for x in vals) / len(vals)
+
+
+def test_poisson_single_mode():
+    """psi_sq = A + B cos(k.x): delta chi = -kappa B / |k|^2 cos(k.x), DC dropped."""
+    N, kappa = 16, 1 / 63
+    c, k = plane(N, (1, 3, 2), 0.7)
+    A, B = 3.0, 1.5
+    chi = O.poisson_chi(A + B * c, N, 19.0, kappa)
+    k2 = sum(q * q for q in k)
+    np.testing.assert_allclose(chi, 19.0 - kappa * B / k2 * c, atol=1e-12)
+    frozen = O.poisson_chi(A + B * c, N, 19.0, kappa, frozen_bf=0.3)
+    outside = ~O.interior_mask(N, 0.3)
+    assert np.all(frozen[outside] == 19.0)
+    np.testing.assert_allclose(frozen[~outside], chi[~outside], atol=0)
