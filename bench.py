@@ -139,12 +139,105 @@ def cpu_baseline(w, budget_s=12.0):
                       f"({steps / dt:.2f} steps/s, one thread)"}
 
 
+def run_batch(args):
+    """Batched sweep workload: 64 independent COMPLEX lattices with uneven
+    sizes from {32,48,64,96,128}^3, kappa_b = kappa * U(0.5, 2), one soliton
+    (amplitude U(2,8), width U(2,4)) at the centre plus N(0, 1e-3) noise on both
+    parts, default cos^2 mask, float32, 500 steps.  Batched: one launch per
+    step for all lattices, replayed from a HIP graph, autotuned tile/chunk.
+    Sequential: the same 64 lattices through the plain Evolver, one after
+    another, one launch per step each (IC generation excluded in both)."""
+    import numpy as np
+    import torch
+    import lfm_hip as L
+    rng = np.random.default_rng(42)
+    nlat = 64
+    cfgs, ics = [], []
+    for _ in range(nlat):
+        N = int(rng.choice([32, 48, 64, 96, 128]))
+        kap = (1 / 63) * rng.uniform(0.5, 2)
+        amp, sig = rng.uniform(2, 8), rng.uniform(2, 4)
+        cfgs.append(L.SimulationConfig(grid_size=N, field_level=L.FieldLevel.COMPLEX, kappa=kap))
+        r, i = L.fields.gaussian_soliton(N, (N // 2, N // 2, N // 2), amp, sig)
+        nr, ni = L.fields.seed_noise(N, 1e-3, rng=rng)
+        ics.append((r + nr, i + ni))
+    steps = args.steps
+    sites = sum(c.grid_size ** 3 for c in cfgs)
+    batch = L.LatticeBatch(cfgs, backend="hip")
+    for b, (r, i) in enumerate(ics):
+        batch.set_psi_real(b, r)
+        batch.set_psi_imag(b, i)
+    tuning = batch.autotune() if args.tile == "auto" else None
+    batch.evolve(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.evolve(steps)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    # kernel-only time: events around eager launches on the launch stream
+    evs = []
+    for _ in range(12):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        batch._step()
+        e1.record()
+        evs.append((e0, e1))
+    torch.cuda.synchronize()
+    kms = sorted(a.elapsed_time(b) for a, b in evs[2:])[len(evs[2:]) // 2]
+    ok = all(bool(torch.isfinite(t).all()) for t in batch._chi)
+    chi_b0 = batch.get_chi(0)
+    del batch
+    torch.cuda.empty_cache()
+    # sequential plain runs
+    evols = []
+    for c, (r, i) in zip(cfgs, ics):
+        ev = L.Evolver(c, backend="hip")
+        ev.set_psi_real(r)
+        ev.set_psi_imag(i)
+        evols.append(ev)
+    for ev in evols:
+        ev.evolve(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for ev in evols:
+        ev.evolve(steps)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    same = bool(np.array_equal(evols[0].get_chi(), chi_b0))
+    b_alg = 40   # COMPLEX with mask: 7 reads + 3 writes of float32
+    achieved = b_alg * sites / (kms * 1e-3) / 1e9
+    out = {"metric": METRIC, "value": nlat / t_batch, "unit": "lattices/s", "n_gpus": 1,
+           "steps": steps, "warmup": args.warmup, "ms_per_step": t_batch * 1e3 / steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": "batched sweep: 64 COMPLEX lattices, N in {32,48,64,96,128}, "
+                                  f"{steps} steps, mask on", "lattices": nlat, "sites": sites},
+           "batched": {"lattices_per_s": nlat / t_batch, "GLUPS": sites * steps / t_batch / 1e9,
+                       "wall_s": t_batch, "tile": batch_tile(tuning)},
+           "sequential": {"lattices_per_s": nlat / t_seq, "GLUPS": sites * steps / t_seq / 1e9,
+                          "wall_s": t_seq},
+           "speedup_vs_sequential": t_seq / t_batch,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "step_tiled_batch<COMPLEX>", "kernel_ms": kms,
+                        "bytes_per_site": b_alg},
+           "finite": ok, "batch_equals_sequential_lattice0": same}
+    print(json.dumps(out), flush=True)
+
+
+def batch_tile(tuning):
+    if not tuning:
+        return None
+    best = min(tuning, key=tuning.get)
+    return {"best": list(best), "ms": {f"{t}/{c}": round(v, 4) for (t, c), v in tuning.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="color512", choices=sorted(WORKLOADS) + ["copy"])
+    ap.add_argument("--workload", default="color512", choices=sorted(WORKLOADS) + ["copy", "batch64"])
     ap.add_argument("--size", type=int, default=None, help="override the lattice size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--direct", action="store_true", help="use the untiled kernel")
@@ -160,6 +253,9 @@ def main():
     import torch
     import lfm_hip as L
 
+    if args.workload == "batch64":
+        run_batch(args)
+        return
     if args.workload == "copy":
         # Device-to-device copy of a known size: the HBM calibration for the
         # rocprof FETCH_SIZE / WRITE_SIZE counters and the measured copy rate.

@@ -54,7 +54,7 @@ extern "C" {
 
 /* Scalar parameters of one step (the Evolver's per-step arguments,
  * lfm/core/evolver.py:201-279).  Unused terms are 0.
- * tile:  0 = the level's default tile shape, 1..LFM_TILE_VARIANTS a specific
+ * tile:  0 = the default tile shape (1), 1..LFM_TILE_VARIANTS a specific
  *        one ({4,1} {4,2} {4,4} {8,1} {8,2} {16,1} wavefronts x rows/thread).
  * chunk: 0 = automatic, else the number of i planes one workgroup marches. */
 typedef struct lfm_params {
@@ -120,6 +120,26 @@ int lfm_poisson_chi_f32(void* plan, const float* pr, const float* pi, int32_t nc
                         float* chi_out, void* stream);
 /* Elementwise max(x, 0) into y (n floats). */
 int lfm_clip_nonneg_f32(const float* x, float* y, int64_t n, void* stream);
+
+/* Batched sweep (replaces the serial per-value runs of lfm.sweep.sweep /
+ * sweep_2d, lfm/sweep.py:14-157).  nlat independent lattices of one field
+ * level, sizes params[b].N in [8, 1023], each with its own scalars (flags may
+ * carry LFM_F_MASK per lattice), are packed in arenas: lattice b starts at
+ * scalar element offset off_b = sum_{c<b} N_c^3 in the chi and mask arenas and
+ * at ncomp*off_b in the Psi arenas (ncomp = 3 for COLOR, else 1; colour-major
+ * inside a lattice).  One lfm_batch_step_f32 call advances every lattice by one
+ * step with a single launch (rotation-mode outputs only); `flags` adds
+ * LFM_F_DAMP_PREV / LFM_F_FREEZE_CHI / LFM_F_CHI_DELTA to every lattice.
+ * COLOR lattices must have kappa_tube == 0 (the S_a transform is per lattice). */
+int lfm_batch_plan_create(int32_t level, int32_t nlat, const lfm_params_t* params, int32_t tile,
+                          int32_t chunk, void** plan);
+int lfm_batch_plan_destroy(void* plan);
+int64_t lfm_batch_total_sites(void* plan);
+int32_t lfm_batch_workgroups(void* plan);
+int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_prev, const float* pi,
+                       const float* pi_prev, const float* chi, const float* chi_prev,
+                       const float* mask, float* pr_out, float* pi_out, float* chi_out,
+                       int32_t flags, void* stream);
 
 /* FP64 reductions (lfm/analysis/energy.py:26-135, metrics.py:29-111,
  * structure.py:26-171).  `ncomp` Psi components of N^3 each (1 REAL,

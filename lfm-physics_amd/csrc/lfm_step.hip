@@ -23,6 +23,9 @@
 //   chi' = max(chi', -chi0); chi' = m chi0 + (1-m) chi'; Psi' *= (1-m).
 #include "lfm_internal.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace lfm {
 namespace {
 
@@ -99,6 +102,34 @@ struct Off64 {
     __device__ __forceinline__ void st(float* p, float v) const { p[i] = v; }
 };
 
+// Pointwise inputs of one site update (read once, not stencilled): the prev
+// Psi components, chi_prev, the mask and the three S_a values.  They are
+// loaded separately from the update so the tiled kernel can issue them one
+// plane ahead and hide their latency behind the current plane's work.
+template <int L>
+struct PointData {
+    float prev[Lv<L>::NPF];
+    float chip, m;
+    float sa[3];
+};
+
+template <int L, class O>
+__device__ __forceinline__ PointData<L> load_point(const KParams& kp, const Ptrs& P, O idx) {
+    constexpr int NPF = Lv<L>::NPF;
+    PointData<L> d;
+#pragma unroll
+    for (int f = 0; f < NPF; ++f) d.prev[f] = idx.ld(field<L>(P.prp, P.pip, f, P.n3));
+    d.m = (kp.flags & LFM_F_MASK) ? idx.ld(P.mask) : 0.0f;
+    d.chip = (kp.flags & LFM_F_FREEZE_CHI) ? 0.0f : idx.ld(P.chip);
+    if constexpr (L == COLOR) {
+        if (P.sa) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) d.sa[q] = idx.ld(P.sa + q * P.n3);
+        }
+    }
+    return d;
+}
+
 // Per-site update, shared by both kernels.
 //  c[f]     Psi component values at the site      lap[f]  their Laplacians
 //  jsum     sum over colours and directions of j_{a,d}   (COMPLEX/COLOR)
@@ -106,6 +137,7 @@ struct Off64 {
 //  chi_st, lap_chi  chi as stored and the Laplacian of the stored field
 template <int L, class O>
 __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O idx,
+                                            const PointData<L>& pd,
                                             const float (&c)[Lv<L>::NPF],
                                             const float (&lap)[Lv<L>::NPF], float chi_st,
                                             float lap_chi, float jsum, float ccv) {
@@ -116,7 +148,7 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
     const bool delta = (kp.flags & LFM_F_CHI_DELTA) != 0;
     const float chi_c = delta ? kp.chi0 + chi_st : chi_st;
     const bool has_mask = (kp.flags & LFM_F_MASK) != 0;
-    const float m = has_mask ? idx.ld(P.mask) : 0.0f;
+    const float m = has_mask ? pd.m : 0.0f;
     const float a = 1.0f - m;
     const float chi_sq = chi_c * chi_c;
 
@@ -131,7 +163,7 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
     float nw[NPF];
 #pragma unroll
     for (int f = 0; f < NPF; ++f) {
-        float pv = idx.ld(field<L>(P.prp, P.pip, f, n3));
+        float pv = pd.prev[f];
         if (damp) pv = pv * a;
         float v = 2.0f * c[f] - pv + kp.dt2 * (lap[f] - chi_sq * c[f]);
         if constexpr (L == COLOR) {
@@ -178,7 +210,7 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
                 float s_sum = 0.0f, s_sq = 0.0f;
 #pragma unroll
                 for (int q = 0; q < C; ++q) {
-                    float s = idx.ld(P.sa + q * n3);
+                    float s = pd.sa[q];
                     s = s > 0.0f ? s : 0.0f;
                     if (P.sa_clip) idx.st(P.sa_clip + q * n3, s);
                     s_sum += s;
@@ -195,7 +227,7 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
     float accel = lap_chi - src - cv + chi_self;
     if constexpr (L == COLOR) accel = accel - kp.ks * ccv - kp.kt * scv;
     // leapfrog in storage units (a constant offset drops out of 2c - c_prev and of lap)
-    float cn = 2.0f * chi_st - idx.ld(P.chip) + kp.dt2 * accel;
+    float cn = 2.0f * chi_st - pd.chip + kp.dt2 * accel;
     const float floor_v = delta ? -2.0f * kp.chi0 : -kp.chi0;
     if (cn < floor_v) cn = floor_v;   // BH excision clamp chi >= -chi0 (NaN passes through)
     cn = delta ? a * cn : m * kp.chi0 + a * cn;   // frozen boundary: chi -> chi0
@@ -267,7 +299,8 @@ __global__ __launch_bounds__(256) void step_direct(Ptrs P, KParams kp) {
             }
         }
     }
-    site_update<L>(kp, P, Off64{idx}, c, lap, chi_c, lap_chi, jsum, ccv);
+    site_update<L>(kp, P, Off64{idx}, load_point<L>(kp, P, Off64{idx}), c, lap, chi_c, lap_chi,
+                   jsum, ccv);
 }
 
 // ---------------------------------------------------------------------------
@@ -281,9 +314,11 @@ __global__ __launch_bounds__(256) void step_direct(Ptrs P, KParams kp) {
 // ---------------------------------------------------------------------------
 constexpr int TK = 64;
 
-template <int L, int BJ, int RJ, int WPE>
-__global__ __launch_bounds__(64 * BJ) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
-void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
+// One workgroup's work: the column tile at (k0, j0) over planes [i0, i1) of
+// one lattice (P, kp).  Shared by the single-lattice and batched kernels.
+template <int L, int BJ, int RJ>
+__device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int k0, int j0,
+                                          int i0, int i1) {
     constexpr int NPF = Lv<L>::NPF;
     constexpr int C = Lv<L>::NC;
     constexpr int NF = NPF + 1;              // + chi
@@ -302,18 +337,6 @@ void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
     const size_t n3 = P.n3;
     const uint32_t rowB = uint32_t(N) * 4u;         // bytes per j row
     const uint32_t planeB = rowB * uint32_t(N);     // bytes per i plane
-
-    // XCD-aware block remap (bijective when gridDim.x % 8 == 0).
-    const int nb = gridDim.x;
-    const int b = blockIdx.x;
-    const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
-    const int kt = lb % nkt;
-    const int rest = lb / nkt;
-    const int jt = rest % njt;
-    const int ch = rest / njt;
-    const int k0 = kt * TK, j0 = jt * TJ;
-    const int i0 = ch * CH;
-    const int i1 = min(i0 + CH, N);
     const int nk = min(TK, N - k0), nj = min(TJ, N - j0);
     const int tx = threadIdx.x & 63;
     const int ty = threadIdx.x >> 6;
@@ -393,6 +416,8 @@ void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
         chim1[rr] = 0.0f; jyz[rr] = 0.0f; ccvyz[rr] = 0.0f;
     }
 
+    // Pointwise inputs of the pending sites, loaded one plane ahead.
+    PointData<L> ptc[RJ], ptn[RJ];
     const int cnt = (i1 - i0) + 2;  // planes i0-1 .. i1
     prefetch(i0 - 1);
     for (int s = 0; s < cnt; ++s) {
@@ -401,6 +426,12 @@ void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
         stage();
         __syncthreads();
         if (s + 1 < cnt) prefetch(q + 1);
+        if (s >= 1 && q < i1) {
+            // site q is finalised in the next iteration: fetch its pointwise data now
+            const uint32_t qb = uint32_t(q) * planeB;
+#pragma unroll
+            for (int rr = 0; rr < RJ; ++rr) ptn[rr] = load_point<L>(kp, P, Off32{qb + loff[rr]});
+        }
         const uint32_t pend_pb = uint32_t((q - 1 + N) % N) * planeB;
 #pragma unroll
         for (int rr = 0; rr < RJ; ++rr) {
@@ -454,8 +485,8 @@ void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
                         if (kp.ks > 0.0f) ccv = ccvyz[rr] + (qx - (1.0f / 3.0f) * sx * sx);
                     }
                 }
-                site_update<L>(kp, P, Off32{pend_pb + ioff0 + rr * rowB}, cpend, lap, chim1[rr],
-                               lap_chi, jsum, ccv);
+                site_update<L>(kp, P, Off32{pend_pb + ioff0 + rr * rowB}, ptc[rr], cpend, lap,
+                               chim1[rr], lap_chi, jsum, ccv);
             }
             // Advance the queue.
 #pragma unroll
@@ -468,18 +499,74 @@ void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
             chim1[rr] = cq[NPF];
             jyz[rr] = jyz_new;
             ccvyz[rr] = ccv_new;
+            ptc[rr] = ptn[rr];
         }
     }
 }
 
+template <int L, int BJ, int RJ, int WPE>
+__global__ __launch_bounds__(64 * BJ) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
+    // XCD-aware block remap: blocks are dispatched round-robin over the 8
+    // XCDs; this makes consecutive tiles of one i chunk share an XCD (and its
+    // L2) so the halo rows neighbouring tiles both read hit in L2.
+    const int nb = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+    const int kt = lb % nkt;
+    const int rest = lb / nkt;
+    const int jt = rest % njt;
+    const int ch = rest / njt;
+    const int i0 = ch * CH;
+    tile_body<L, BJ, RJ>(P, kp, kt * TK, jt * BJ * RJ, i0, min(i0 + CH, kp.N));
+}
+
+// Batched sweep: one launch advances many independent lattices.  Each
+// workgroup reads its tile descriptor {lattice, k0, j0, i0} and the
+// lattice's record (arena offset, N, chunk, scalar parameters).
+struct BatchLattice {
+    long long off;   // element offset of the lattice in every arena
+    int N, CH, ncomp, pad;
+    KParams kp;
+};
+
+struct BatchArenas {
+    const float* pr; const float* prp; const float* pi; const float* pip;
+    const float* chi; const float* chip; const float* mask;
+    float* pr_out; float* pi_out; float* chi_out;
+};
+
+template <int L, int BJ, int RJ>
+__global__ __launch_bounds__(64 * BJ) void step_tiled_batch(BatchArenas A,
+                                                             const BatchLattice* __restrict__ lat,
+                                                             const int4* __restrict__ desc,
+                                                             int flags_or) {
+    const int4 d = desc[blockIdx.x];
+    if (d.x < 0) return;   // padding descriptor (whole workgroup exits together)
+    const BatchLattice& Lb = lat[d.x];
+    KParams kp = Lb.kp;
+    kp.flags |= flags_or;
+    const long long o = Lb.off;
+    const size_t n3 = size_t(Lb.N) * Lb.N * Lb.N;
+    // colour fields of one lattice are colour-major within its own span
+    const long long oc = o * Lb.ncomp;
+    Ptrs P{};
+    P.pr = A.pr + oc; P.prp = A.prp + oc; P.chi = A.chi + o; P.chip = A.chip + o;
+    if (A.pi) { P.pi = A.pi + oc; P.pip = A.pip + oc; P.pi_out = A.pi_out + oc; }
+    P.mask = (kp.flags & LFM_F_MASK) ? A.mask + o : nullptr;
+    P.pr_out = A.pr_out + oc; P.chi_out = A.chi_out + o;
+    P.n3 = n3;
+    tile_body<L, BJ, RJ>(P, kp, d.y, d.z, d.w, min(d.w + Lb.CH, Lb.N));
+}
+
 int choose_chunk(int N, int tiles) {
-    // Aim for >= 2048 workgroups (8 per CU on 256 CUs); keep >= 4 planes per
-    // chunk so the two halo planes cost at most half a chunk.
-    int nch = (2048 + tiles - 1) / tiles;
-    int maxch = N / 4 > 0 ? N / 4 : 1;
-    if (nch > maxch) nch = maxch;
-    if (nch < 1) nch = 1;
-    return (N + nch - 1) / nch;
+    // Aim for ~8192 workgroups (32 per CU on 256 CUs) so every CU keeps
+    // several tiles in flight; keep >= 8 planes per chunk so the two halo
+    // planes a chunk re-reads cost at most a quarter of its traffic.
+    const int nch = (8192 + tiles - 1) / tiles;
+    int ch = (N + nch - 1) / nch;
+    if (ch < 8) ch = 8;
+    return ch < N ? ch : N;
 }
 
 template <int L, int BJ, int RJ, int WPE>
@@ -506,7 +593,7 @@ int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
         return LFM_OK;
     }
     // Tile shapes (wavefronts BJ x rows per thread RJ); 0 picks the level's default.
-    int tile = kp.tile > 0 ? kp.tile : (L == COLOR ? 2 : 3);
+    int tile = kp.tile > 0 ? kp.tile : 1;
     switch (tile) {
         case 1: return launch_tiled<L, 4, 1, 1>(P, kp, st);
         case 2: return launch_tiled<L, 4, 2, 1>(P, kp, st);
@@ -519,6 +606,34 @@ int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
             return LFM_E_ARG;
     }
 }
+
+constexpr int tile_tj(int tile) {
+    return tile == 1 ? 4 : tile == 2 ? 8 : tile == 3 ? 16 : tile == 4 ? 8 : tile == 5 ? 16 : 16;
+}
+
+template <int L>
+int launch_batch(const BatchArenas& A, const BatchLattice* lat, const int4* desc, int ndesc,
+                 int tile, int flags_or, hipStream_t st) {
+    const dim3 g(ndesc);
+    switch (tile) {
+        case 1: hipLaunchKernelGGL((step_tiled_batch<L, 4, 1>), g, dim3(256), 0, st, A, lat, desc, flags_or); break;
+        case 2: hipLaunchKernelGGL((step_tiled_batch<L, 4, 2>), g, dim3(256), 0, st, A, lat, desc, flags_or); break;
+        case 3: hipLaunchKernelGGL((step_tiled_batch<L, 4, 4>), g, dim3(256), 0, st, A, lat, desc, flags_or); break;
+        case 4: hipLaunchKernelGGL((step_tiled_batch<L, 8, 1>), g, dim3(512), 0, st, A, lat, desc, flags_or); break;
+        case 5: hipLaunchKernelGGL((step_tiled_batch<L, 8, 2>), g, dim3(512), 0, st, A, lat, desc, flags_or); break;
+        case 6: hipLaunchKernelGGL((step_tiled_batch<L, 16, 1>), g, dim3(1024), 0, st, A, lat, desc, flags_or); break;
+        default: set_error("tile variant %d unknown", tile); return LFM_E_ARG;
+    }
+    LFM_LAUNCH_CHECK("step_tiled_batch");
+    return LFM_OK;
+}
+
+struct BatchPlan {
+    int level = 0, nlat = 0, tile = 0, ndesc = 0;
+    long long total = 0;
+    BatchLattice* d_lat = nullptr;
+    int4* d_desc = nullptr;
+};
 
 KParams to_k(const lfm_params_t* p) {
     KParams k;
@@ -667,4 +782,124 @@ extern "C" int lfm_step_color_f32(const float* pr, const float* pr_prev, const f
     P.psisq = psisq_out;
     P.n3 = n3;
     return launch<COLOR>(P, to_k(p), (hipStream_t)stream);
+}
+
+extern "C" int lfm_batch_plan_create(int32_t level, int32_t nlat, const lfm_params_t* params,
+                                     int32_t tile, int32_t chunk, void** plan) {
+    if (!plan || !params || nlat < 1 || level < 0 || level > 2) {
+        set_error("lfm_batch_plan_create: bad argument");
+        return LFM_E_ARG;
+    }
+    *plan = nullptr;
+    if (tile <= 0) tile = 1;
+    if (tile > LFM_TILE_VARIANTS) { set_error("tile variant %d unknown", tile); return LFM_E_ARG; }
+    const int TJ = tile_tj(tile);
+    const int ncomp = level == 2 ? 3 : 1;
+    std::vector<BatchLattice> lat(nlat);
+    std::vector<int4> desc;
+    long long off = 0;
+    for (int b = 0; b < nlat; ++b) {
+        const lfm_params_t& p = params[b];
+        if (p.N < 8 || p.N > 1023) {
+            set_error("lfm_batch_plan_create: lattice %d has N=%d outside [8, 1023]", b, p.N);
+            return LFM_E_ARG;
+        }
+        if (level == 2 && p.kappa_tube > 0.0f) {
+            set_error("lfm_batch_plan_create: the S_a transform is not batched (lattice %d has kappa_tube > 0)", b);
+            return LFM_E_ARG;
+        }
+        BatchLattice& L = lat[b];
+        L.off = off;
+        L.N = p.N;
+        L.ncomp = ncomp;
+        L.CH = chunk > 0 ? std::min<int>(chunk, p.N) : std::min<int>(16, p.N);
+        L.kp = to_k(&p);
+        off += (long long)p.N * p.N * p.N;
+    }
+    // Largest lattices first (their tiles are the fullest), tiles of one
+    // lattice chunk contiguous.
+    std::vector<int> order(nlat);
+    for (int b = 0; b < nlat; ++b) order[b] = b;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lat[a].N > lat[b].N; });
+    for (int b : order) {
+        const int N = lat[b].N, CH = lat[b].CH;
+        for (int i0 = 0; i0 < N; i0 += CH)
+            for (int j0 = 0; j0 < N; j0 += TJ)
+                for (int k0 = 0; k0 < N; k0 += TK) desc.push_back(make_int4(b, k0, j0, i0));
+    }
+    // XCD grouping: physical block p runs on XCD p % 8; place logical tile l at
+    // p so that each XCD takes a contiguous run of tiles.
+    const int n = int((desc.size() + 7) / 8 * 8);
+    std::vector<int4> phys(n, make_int4(-1, 0, 0, 0));
+    for (int l = 0; l < int(desc.size()); ++l) {
+        const int x = l / (n / 8), m = l % (n / 8);
+        phys[m * 8 + x] = desc[l];
+    }
+    auto* P = new BatchPlan();
+    P->level = level; P->nlat = nlat; P->tile = tile; P->ndesc = n; P->total = off;
+    hipError_t e = hipMalloc((void**)&P->d_lat, sizeof(BatchLattice) * nlat);
+    if (e == hipSuccess) e = hipMalloc((void**)&P->d_desc, sizeof(int4) * n);
+    if (e == hipSuccess) e = hipMemcpy(P->d_lat, lat.data(), sizeof(BatchLattice) * nlat, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(P->d_desc, phys.data(), sizeof(int4) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error("lfm_batch_plan_create: %s", hipGetErrorString(e));
+        if (P->d_lat) (void)hipFree(P->d_lat);
+        if (P->d_desc) (void)hipFree(P->d_desc);
+        delete P;
+        return LFM_E_HIP;
+    }
+    *plan = P;
+    return LFM_OK;
+}
+
+extern "C" int64_t lfm_batch_total_sites(void* plan) {
+    return plan ? static_cast<BatchPlan*>(plan)->total : -1;
+}
+
+extern "C" int32_t lfm_batch_workgroups(void* plan) {
+    return plan ? static_cast<BatchPlan*>(plan)->ndesc : -1;
+}
+
+extern "C" int lfm_batch_plan_destroy(void* plan) {
+    auto* P = static_cast<BatchPlan*>(plan);
+    if (!P) return LFM_OK;
+    (void)hipFree(P->d_lat);
+    (void)hipFree(P->d_desc);
+    delete P;
+    return LFM_OK;
+}
+
+extern "C" int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_prev,
+                                  const float* pi, const float* pi_prev, const float* chi,
+                                  const float* chi_prev, const float* mask, float* pr_out,
+                                  float* pi_out, float* chi_out, int32_t flags, void* stream) {
+    auto* P = static_cast<BatchPlan*>(plan);
+    if (!P || !pr || !pr_prev || !chi || !chi_prev || !mask || !pr_out || !chi_out) {
+        set_error("lfm_batch_step_f32: required pointer is NULL");
+        return LFM_E_ARG;
+    }
+    if (P->level > 0 && (!pi || !pi_prev || !pi_out)) {
+        set_error("lfm_batch_step_f32: complex levels need pi, pi_prev and pi_out");
+        return LFM_E_ARG;
+    }
+    const size_t ns = size_t(P->total) * (P->level == 2 ? 3 : 1);
+    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask};
+    const size_t nin[] = {ns, ns, ns, ns, size_t(P->total), size_t(P->total), size_t(P->total)};
+    float* outs[] = {pr_out, pi_out, chi_out};
+    const size_t nout[] = {ns, ns, size_t(P->total)};
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 7; ++b)
+            if (outs[a] && overlaps(outs[a], nout[a] * 4, ins[b], nin[b] * 4)) {
+                set_error("lfm_batch_step_f32: output %d overlaps input %d", a, b);
+                return LFM_E_ARG;
+            }
+    BatchArenas A{pr, pr_prev, P->level ? pi : nullptr, P->level ? pi_prev : nullptr, chi, chi_prev,
+                  mask, pr_out, P->level ? pi_out : nullptr, chi_out};
+    const int fo = flags & (LFM_F_DAMP_PREV | LFM_F_FREEZE_CHI | LFM_F_CHI_DELTA);
+    hipStream_t st = (hipStream_t)stream;
+    switch (P->level) {
+        case 0: return launch_batch<REAL>(A, P->d_lat, P->d_desc, P->ndesc, P->tile, fo, st);
+        case 1: return launch_batch<COMPLEX>(A, P->d_lat, P->d_desc, P->ndesc, P->tile, fo, st);
+        default: return launch_batch<COLOR>(A, P->d_lat, P->d_desc, P->ndesc, P->tile, fo, st);
+    }
 }

@@ -10,9 +10,10 @@ from .config import (BoundaryType, ChiMode, FieldLevel, PhysicsScale, Simulation
 from .backend import HipBackend
 from .evolver import Evolver, get_backend
 from .simulation import Simulation
+from .batch import LatticeBatch
 from . import fields
 
 __all__ = ["BoundaryType", "ChiMode", "FieldLevel", "PhysicsScale", "SimulationConfig",
            "full_physics", "gravity_em", "gravity_only", "spinor_field", "HipBackend",
-           "Evolver", "get_backend", "Simulation", "fields"]
+           "Evolver", "get_backend", "Simulation", "LatticeBatch", "fields"]
 __version__ = "0.1.0"

@@ -1,0 +1,420 @@
+"""Batched form: many independent lattices advanced by one launch per step.
+
+The reference's parameter sweep (lfm/sweep.py:14-157) builds one
+``Simulation`` per value and runs them one after another, one kernel launch
+per step per lattice.  :class:`LatticeBatch` packs every lattice of a sweep
+into shared arenas; ``lfm_batch_step_f32`` advances all of them with a single
+launch whose workgroups each take one (lattice, column tile, plane chunk)
+descriptor, and the per-step launches are captured in a HIP graph and
+replayed.  :func:`sweep` / :func:`sweep_2d` keep the reference's call
+signature and result rows.
+
+Lattices of one batch share a field level; COLOR lattices must not use the
+S_a transform (kappa_tube == 0), which runs per lattice.
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+from copy import deepcopy
+
+import numpy as np
+import torch
+
+from . import _abi
+from .backend import ScvPlan, params, stream_handle
+from .config import BoundaryType, FieldLevel, SimulationConfig
+from .evolver import get_backend
+from .fields import boundary_mask_host, gaussian_soliton
+
+
+class LatticeBatch:
+    """State of `len(configs)` lattices in packed arenas (three-slot rotation,
+    chi stored as chi - chi0 per lattice, as :class:`Evolver` does)."""
+
+    def __init__(self, configs, backend="auto", tile: int = 0, chunk: int = 0,
+                 mask_policy: str = "reference"):
+        if not configs:
+            raise ValueError("LatticeBatch needs at least one config")
+        levels = {int(c.field_level) for c in configs}
+        if len(levels) != 1:
+            raise ValueError("all lattices of a batch must share one field level")
+        self.level = levels.pop()
+        if self.level == 2 and any(c.kappa_tube > 0 for c in configs):
+            raise NotImplementedError("COLOR lattices with kappa_tube > 0 need the per-lattice "
+                                      "S_a transform; use Evolver for them")
+        if self.level == 2 and any(c.n_colors != 3 for c in configs):
+            raise NotImplementedError("the COLOR step is defined for 3 colours")
+        self.backend = get_backend(backend)
+        self.device = self.backend.device
+        self._lib = _abi.load()
+        self.configs = list(configs)
+        self.nlat = len(configs)
+        self.sizes = [c.grid_size for c in configs]
+        self.ncomp = 3 if self.level == 2 else 1
+        n3 = [N ** 3 for N in self.sizes]
+        self.offsets = np.concatenate([[0], np.cumsum(n3)[:-1]]).astype(np.int64)
+        self.total = int(sum(n3))
+        f32 = torch.float32
+        dev = self.device
+        npsi = self.ncomp * self.total
+        self._pr = [torch.zeros(npsi, dtype=f32, device=dev) for _ in range(3)]
+        self._pi = ([torch.zeros(npsi, dtype=f32, device=dev) for _ in range(3)]
+                    if self.level > 0 else None)
+        self._chi = [torch.zeros(self.total, dtype=f32, device=dev) for _ in range(3)]
+        self._chi0 = [float(np.float32(c.chi0)) for c in configs]
+        self.mask = torch.zeros(self.total, dtype=f32, device=dev)
+        self.has_mask = []
+        for b, c in enumerate(configs):
+            m = not (mask_policy == "boundary_type" and c.boundary_type == BoundaryType.PERIODIC)
+            self.has_mask.append(m)
+            if m:
+                self._view(self.mask, b).copy_(torch.from_numpy(
+                    boundary_mask_host(c.grid_size, c.boundary_fraction)))
+        self._cur, self._prev, self._chi_cur, self._chi_prev = 0, 1, 0, 1
+        self._damped = False
+        self.step = 0
+        self._graph = None
+        self._plan = None
+        self.tile, self.chunk = 0, 0
+        self._make_plan(tile, chunk)
+
+    # ------------------------------------------------------------- plan
+    def _make_plan(self, tile: int, chunk: int) -> None:
+        arr = (_abi.LfmParams * self.nlat)()
+        for b, c in enumerate(self.configs):
+            arr[b] = params(c.grid_size, c.dt ** 2, c.kappa, c.lambda_self, c.chi0, c.e0_sq,
+                            c.epsilon_w, c.kappa_c, c.epsilon_cc, c.kappa_string, 0.0,
+                            _abi.LFM_F_MASK if self.has_mask[b] else 0)
+        h = ctypes.c_void_p()
+        _abi.check(self._lib.lfm_batch_plan_create(self.level, self.nlat, arr, int(tile),
+                                                   int(chunk), ctypes.byref(h)),
+                   "lfm_batch_plan_create")
+        self._free_plan()
+        self._plan = h
+        self.tile, self.chunk = tile, chunk
+        self._graph = None
+
+    def _free_plan(self):
+        if self._plan is not None and self._plan.value:
+            self._lib.lfm_batch_plan_destroy(self._plan)
+        self._plan = None
+
+    def __del__(self):
+        try:
+            self._free_plan()
+        except Exception:
+            pass
+
+    @property
+    def workgroups(self) -> int:
+        return int(self._lib.lfm_batch_workgroups(self._plan))
+
+    # ------------------------------------------------------------ views
+    def _view(self, arena: torch.Tensor, b: int, psi: bool = False) -> torch.Tensor:
+        n3 = self.sizes[b] ** 3
+        k = self.ncomp if psi else 1
+        o = int(self.offsets[b]) * k
+        return arena[o:o + k * n3]
+
+    def _shape(self, b):
+        N = self.sizes[b]
+        return (3, N, N, N) if self.level == 2 else (N, N, N)
+
+    def _undamp(self):
+        if self._damped:
+            for b in range(self.nlat):
+                if self.has_mask[b]:
+                    a = 1.0 - self._view(self.mask, b)
+                    if self.ncomp > 1:
+                        a = a.repeat(self.ncomp)
+                    self._view(self._pr[self._prev], b, True).mul_(a)
+                    if self._pi is not None:
+                        self._view(self._pi[self._prev], b, True).mul_(a)
+        self._damped = False
+        self._graph = None
+
+    def _dev(self, arr) -> torch.Tensor:
+        if isinstance(arr, torch.Tensor):
+            return arr.detach().to(self.device, torch.float32).reshape(-1)
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(arr, np.float32).ravel())).to(self.device)
+
+    # ---------------------------------------------------- setters/getters
+    def set_psi_real(self, b: int, arr) -> None:
+        """Current and prev Psi_r of lattice b := arr (evolver.py:355-375)."""
+        d = self._dev(arr)
+        self._undamp()
+        self._view(self._pr[self._cur], b, True).copy_(d)
+        self._view(self._pr[self._prev], b, True).copy_(d)
+
+    def set_psi_imag(self, b: int, arr) -> None:
+        if self._pi is None:
+            raise ValueError("Cannot set imaginary part for REAL field level")
+        d = self._dev(arr)
+        self._undamp()
+        self._view(self._pi[self._cur], b, True).copy_(d)
+        self._view(self._pi[self._prev], b, True).copy_(d)
+
+    def set_chi(self, b: int, arr) -> None:
+        """Current and prev chi of lattice b := arr."""
+        d = self._dev(arr) - self._chi0[b]
+        self._view(self._chi[self._chi_cur], b).copy_(d)
+        self._view(self._chi[self._chi_prev], b).copy_(d)
+        self._graph = None
+
+    def get_psi_real(self, b: int) -> np.ndarray:
+        return self._view(self._pr[self._cur], b, True).cpu().numpy().reshape(self._shape(b))
+
+    def get_psi_imag(self, b: int):
+        if self._pi is None:
+            return None
+        return self._view(self._pi[self._cur], b, True).cpu().numpy().reshape(self._shape(b))
+
+    def get_chi(self, b: int) -> np.ndarray:
+        N = self.sizes[b]
+        return (self._view(self._chi[self._chi_cur], b) + self._chi0[b]).cpu().numpy().reshape(N, N, N)
+
+    def get_energy_density(self, b: int) -> np.ndarray:
+        pr = self.get_psi_real(b).astype(np.float32)
+        e = np.sum(pr ** 2, axis=0) if pr.ndim == 4 else pr ** 2
+        pi = self.get_psi_imag(b)
+        if pi is not None:
+            e = e + (np.sum(pi ** 2, axis=0) if pi.ndim == 4 else pi ** 2)
+        return e
+
+    # -------------------------------------------------------------- steps
+    def _step(self) -> None:
+        cur, prev = self._cur, self._prev
+        nxt = 3 - cur - prev
+        cc, cp = self._chi_cur, self._chi_prev
+        cn = 3 - cc - cp if cc != cp else (cc + 1) % 3
+        flags = _abi.LFM_F_CHI_DELTA | (_abi.LFM_F_DAMP_PREV if self._damped else 0)
+        pi = self._pi
+        rc = self._lib.lfm_batch_step_f32(
+            self._plan, self._pr[cur].data_ptr(), self._pr[prev].data_ptr(),
+            pi[cur].data_ptr() if pi else None, pi[prev].data_ptr() if pi else None,
+            self._chi[cc].data_ptr(), self._chi[cp].data_ptr(), self.mask.data_ptr(),
+            self._pr[nxt].data_ptr(), pi[nxt].data_ptr() if pi else None,
+            self._chi[cn].data_ptr(), flags, stream_handle())
+        _abi.check(rc, "lfm_batch_step_f32")
+        self._prev, self._cur = cur, nxt
+        self._chi_prev, self._chi_cur = cc, cn
+        self._damped = True
+
+    def evolve(self, steps: int, graph: bool = True) -> None:
+        """Advance every lattice `steps` steps.  With graph=True the launches
+        of one full three-slot rotation are captured once in a HIP graph and
+        replayed (steps // 3 replays, the remainder launched directly)."""
+        steps = int(steps)
+        if steps <= 0:
+            return
+        if not graph:
+            for _ in range(steps):
+                self._step()
+            self.step += steps
+            return
+        done = 0
+        # reach the steady state: prev damped and the chi slots distinct
+        while done < steps and (not self._damped or self._chi_cur == self._chi_prev):
+            self._step()
+            done += 1
+        reps = (steps - done) // 3
+        if reps:
+            if self._graph is None or self._graph_key != self._state_key():
+                self._capture()
+            for _ in range(reps):
+                self._graph.replay()
+            done += 3 * reps
+        while done < steps:
+            self._step()
+            done += 1
+        self.step += steps
+
+    def _state_key(self):
+        return (self._cur, self._prev, self._chi_cur, self._chi_prev, self._damped)
+
+    def _capture(self) -> None:
+        key = self._state_key()
+        saved = (self._cur, self._prev, self._chi_cur, self._chi_prev)
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                for _ in range(3):
+                    self._step()
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        # capture only recorded the launches: restore the slot bookkeeping
+        self._cur, self._prev, self._chi_cur, self._chi_prev = saved
+        self._graph, self._graph_key = g, key
+
+    def autotune(self, steps: int = 6, tiles=None, chunks=(8, 16, 32)) -> dict:
+        """Time tile shapes x plane chunks of the batched kernel on the
+        current state (restored afterwards) and keep the fastest."""
+        tiles = tiles or range(1, _abi.LFM_TILE_VARIANTS + 1)
+        snap = ([t.clone() for t in self._pr], [t.clone() for t in self._pi] if self._pi else None,
+                [t.clone() for t in self._chi], self._state_key())
+        res = {}
+        for tile, ch in itertools.product(tiles, chunks):
+            self._make_plan(tile, ch)
+            self._step()
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record()
+            for _ in range(steps):
+                self._step()
+            ev1.record()
+            torch.cuda.synchronize(self.device)
+            res[(tile, ch)] = ev0.elapsed_time(ev1) / steps
+            for dst, src in zip(self._pr, snap[0]):
+                dst.copy_(src)
+            if snap[1]:
+                for dst, src in zip(self._pi, snap[1]):
+                    dst.copy_(src)
+            for dst, src in zip(self._chi, snap[2]):
+                dst.copy_(src)
+            self._cur, self._prev, self._chi_cur, self._chi_prev, self._damped = snap[3]
+        best = min(res, key=res.get)
+        self._make_plan(*best)
+        return res
+
+    # ----------------------------------------------------------- analysis
+    def _red(self):
+        if not hasattr(self, "_partials"):
+            n = int(self._lib.lfm_reduce_partials_len(max(self.sizes)))
+            self._partials = torch.empty(n, dtype=torch.float64, device=self.device)
+            self._red_out = torch.empty(8, dtype=torch.float64, device=self.device)
+        return self._partials, self._red_out
+
+    def metrics(self, b: int, clusters: bool = True) -> dict:
+        """compute_metrics keys for lattice b, with the leapfrog prev as the
+        kinetic reference (FP64 reductions on the GPU)."""
+        part, out = self._red()
+        c = self.configs[b]
+        N = self.sizes[b]
+        pr, prp = self._view(self._pr[self._cur], b, True), self._view(self._pr[self._prev], b, True)
+        pi = self._view(self._pi[self._cur], b, True) if self._pi else None
+        pip = self._view(self._pi[self._prev], b, True) if self._pi else None
+        rc = self._lib.lfm_energy_f64(
+            pr.data_ptr(), prp.data_ptr(), pi.data_ptr() if pi is not None else None,
+            pip.data_ptr() if pip is not None else None,
+            self._view(self._chi[self._chi_cur], b).data_ptr(), self._chi0[b],
+            self._view(self.mask, b).data_ptr(), int(self._damped and self.has_mask[b]),
+            self.ncomp, N, float(c.dt), float(c.c), part.data_ptr(), out.data_ptr(),
+            stream_handle())
+        _abi.check(rc, "lfm_energy_f64")
+        T, G, V, S = out[:4].tolist()
+        rc = self._lib.lfm_chi_stats_f64(self._view(self._chi[self._chi_cur], b).data_ptr(),
+                                         self._chi0[b], N, float(c.boundary_fraction), 17.0,
+                                         18.0, part.data_ptr(), out.data_ptr(), stream_handle())
+        _abi.check(rc, "lfm_chi_stats_f64")
+        v = out[:7].tolist()
+        m = {"energy_kinetic": T, "energy_gradient": G, "energy_potential": V,
+             "energy_total": T + G + V, "chi_min": v[0], "chi_max": v[1], "chi_mean": v[2],
+             "chi_std": v[3], "well_fraction": v[4], "void_fraction": v[5],
+             "n_clusters": float("nan"), "psi_sq_total": S}
+        if clusters:
+            from scipy import ndimage
+            psq = self.get_energy_density(b).astype(np.float64)
+            x = np.arange(N, dtype=np.float64)
+            X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+            h = N / 2.0
+            interior = np.sqrt((X - h) ** 2 + (Y - h) ** 2 + (Z - h) ** 2) <= h * (1 - c.boundary_fraction)
+            vals = psq[interior]
+            if vals.size:
+                binary = (psq > np.percentile(vals, 90.0)) & interior
+                m["n_clusters"] = float(ndimage.label(binary)[1])
+            else:
+                m["n_clusters"] = 0.0
+        return m
+
+    # ------------------------------------------------------------- setup
+    def place_soliton(self, b: int, position=None, amplitude=None, sigma=None, phase=0.0) -> None:
+        """Add a static Gaussian soliton to lattice b (Simulation.place_soliton,
+        lfm/simulation.py:495-625, without velocity)."""
+        c = self.configs[b]
+        N = c.grid_size
+        if position is None:
+            position = (N // 2, N // 2, N // 2)
+        pr, pi = gaussian_soliton(N, position, c.e_amplitude if amplitude is None else amplitude,
+                                  c.sigma if sigma is None else sigma, phase)
+        self.set_psi_real(b, self.get_psi_real(b) + pr)
+        if self._pi is not None:
+            self.set_psi_imag(b, self.get_psi_imag(b) + pi)
+
+    def equilibrate(self, b: int) -> None:
+        """GOV-04 Poisson chi for lattice b on the GPU (lfm/simulation.py:877-963)."""
+        c = self.configs[b]
+        N = c.grid_size
+        if not hasattr(self, "_poisson"):
+            self._poisson = {}
+        if N not in self._poisson:
+            self._poisson[N] = ScvPlan(N, 1)
+        chi = torch.empty(N ** 3, dtype=torch.float32, device=self.device)
+        bf = c.boundary_fraction if c.boundary_type == BoundaryType.FROZEN else -1.0
+        pi = self._view(self._pi[self._cur], b, True) if self._pi else None
+        rc = self._lib.lfm_poisson_chi_f32(
+            self._poisson[N].handle, self._view(self._pr[self._cur], b, True).data_ptr(),
+            pi.data_ptr() if pi is not None else None, self.ncomp, float(c.kappa),
+            float(c.e0_sq), float(c.chi0), float(bf), chi.data_ptr(), stream_handle())
+        _abi.check(rc, "lfm_poisson_chi_f32")
+        self.set_chi(b, chi)
+
+
+def _batched_sweep(cfgs, rows, steps, metric_names, equilibrate, soliton, backend, tune):
+    batch = LatticeBatch(cfgs, backend=backend)
+    for b, c in enumerate(cfgs):
+        if soliton is not None:
+            kw = dict(soliton)
+            pos = kw.pop("position", None)
+            if "velocity" in kw:
+                raise NotImplementedError("moving solitons are not batched")
+            batch.place_soliton(b, pos, **kw)
+        if equilibrate:
+            batch.equilibrate(b)
+    if tune:
+        batch.autotune()
+    batch.evolve(steps)
+    out = []
+    for b, row in enumerate(rows):
+        m = batch.metrics(b)
+        if metric_names is not None:
+            row.update({k: m.get(k) for k in metric_names})
+        else:
+            row.update(m)
+        out.append(row)
+    return out
+
+
+def sweep(config: SimulationConfig, param: str, values, steps: int, metric_names=None, *,
+          equilibrate: bool = True, soliton=None, backend="auto", tune: bool = False):
+    """lfm.sweep.sweep with every value run in one batch (lfm/sweep.py:14-82).
+
+    As in the reference, each value is set on a deep copy of `config` with
+    setattr (derived fields such as sigma keep the base config's value), an
+    optional soliton is placed (at the centre when no position is given),
+    chi is equilibrated, `steps` steps are run and the final metrics are
+    returned as {param: value, **metrics} rows.  Metrics use the true leapfrog
+    prev for the kinetic term.
+    """
+    cfgs, rows = [], []
+    for v in values:
+        c = deepcopy(config)
+        setattr(c, param, v)
+        cfgs.append(c)
+        rows.append({param: v})
+    return _batched_sweep(cfgs, rows, steps, metric_names, equilibrate, soliton, backend, tune)
+
+
+def sweep_2d(config: SimulationConfig, param1: str, values1, param2: str, values2, steps: int,
+             metric_names=None, *, equilibrate: bool = True, soliton=None, backend="auto",
+             tune: bool = False):
+    """lfm.sweep.sweep_2d as one batch (lfm/sweep.py:85-157)."""
+    cfgs, rows = [], []
+    for v1 in values1:
+        for v2 in values2:
+            c = deepcopy(config)
+            setattr(c, param1, v1)
+            setattr(c, param2, v2)
+            cfgs.append(c)
+            rows.append({param1: v1, param2: v2})
+    return _batched_sweep(cfgs, rows, steps, metric_names, equilibrate, soliton, backend, tune)

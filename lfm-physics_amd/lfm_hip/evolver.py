@@ -391,6 +391,17 @@ class Evolver:
         self._chi[s].copy_(self._dev_chi(arr))
         self._cp = self._op = s
 
+    def get_chi_storage(self) -> tuple[np.ndarray, np.ndarray]:
+        """Raw stored (current, prev) chi buffers (chi - chi0 for deviation
+        storage): the exact state, for bit-exact checkpoints."""
+        return self._chi[self._cc].cpu().numpy(), self._chi[self._cp].cpu().numpy()
+
+    def set_chi_storage(self, cur, prev) -> None:
+        """Restore raw stored chi buffers written by get_chi_storage."""
+        self._chi[0].copy_(self._dev(cur))
+        self._chi[1].copy_(self._dev(prev))
+        self._cc, self._cp, self._oc, self._op = 0, 1, 1, 1
+
     def set_sa_fields(self, arr) -> None:
         if self._sa is None:
             raise ValueError("SA fields not allocated -- set kappa_tube > 0 in SimulationConfig"

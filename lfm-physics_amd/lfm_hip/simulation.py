@@ -286,8 +286,10 @@ class Simulation:
         path = Path(path)
         path.parent.mkdir(parents=True, exist_ok=True)
         ev = self._evolver
+        cs, cps = ev.get_chi_storage()
         data = {"step": np.int64(self.step), "chi": ev.get_chi(), "psi_real": ev.get_psi_real(),
-                "psi_real_prev_true": ev.get_psi_real_prev(), "chi_prev_true": ev.get_chi_prev()}
+                "psi_real_prev_true": ev.get_psi_real_prev(), "chi_prev_true": ev.get_chi_prev(),
+                "chi_storage": np.array(ev.chi_storage), "chi_stored": cs, "chi_prev_stored": cps}
         if ev.get_psi_imag() is not None:
             data["psi_imag"] = ev.get_psi_imag()
             data["psi_imag_prev_true"] = ev.get_psi_imag_prev()
@@ -331,6 +333,8 @@ class Simulation:
             if "psi_imag_prev_true" in data:
                 ev.set_psi_imag_prev(data["psi_imag_prev_true"])
             ev.set_chi_prev(data["chi_prev_true"])
+            if "chi_stored" in data and str(data["chi_storage"]) == ev.chi_storage:
+                ev.set_chi_storage(data["chi_stored"], data["chi_prev_stored"])
         ev.step = int(data["step"])
         if "sa_fields" in data and ev.sa_enabled:
             ev.set_sa_fields(data["sa_fields"])
