@@ -35,6 +35,23 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) == bound, set(syms) ^ bound
 
 
+def declared_arity():
+    """Parameter count of every function declared in the header."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for name, args in re.findall(r"\b(lfm_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
+        args = args.strip()
+        out[name] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_binding_arity_matches_header():
+    from lfm_hip import _abi
+    ar = declared_arity()
+    for name, _, argtypes in _abi.SIGNATURES:
+        assert len(argtypes) == ar[name], (name, len(argtypes), ar[name])
+
+
 def test_abi_version_and_error_string():
     from lfm_hip import _abi
     lib = _abi.load()
