@@ -174,6 +174,7 @@ def run_batch(args):
     batch.evolve(steps)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    chi_b0 = batch.get_chi(0)
     # kernel-only time: events around eager launches on the launch stream
     evs = []
     for _ in range(12):
@@ -185,7 +186,6 @@ def run_batch(args):
     torch.cuda.synchronize()
     kms = sorted(a.elapsed_time(b) for a, b in evs[2:])[len(evs[2:]) // 2]
     ok = all(bool(torch.isfinite(t).all()) for t in batch._chi)
-    chi_b0 = batch.get_chi(0)
     del batch
     torch.cuda.empty_cache()
     # sequential plain runs
@@ -225,6 +225,71 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def run_adjoint(args):
+    """Differentiable workload: COMPLEX 128^3, 400 steps, float32, mask on.
+    Loss sum((chi_T - chi_target)^2) with chi_target from a run at 1.05 kappa;
+    gradients w.r.t. Psi_0 (two solitons, amplitude 5, width 3) and kappa.
+    Cost ratio = time(forward with tape + adjoint backward) / time(plain
+    forward of the same steps), for the keep-every-state tape and for the
+    sqrt(T)-checkpoint tape."""
+    import math
+    import numpy as np
+    import torch
+    import lfm_hip as L
+    from lfm_hip.adjoint import evolve_chi
+    N = args.size or 128
+    T = args.steps
+    cfg = L.SimulationConfig(grid_size=N, field_level=L.FieldLevel.COMPLEX)
+    rng = np.random.default_rng(42)
+    pr = np.zeros((N,) * 3, np.float32)
+    pi = np.zeros((N,) * 3, np.float32)
+    for _ in range(2):
+        r, i = L.fields.gaussian_soliton(N, tuple(N / 2 + rng.uniform(-0.2, 0.2, 3) * N), 5.0, 3.0,
+                                         rng.uniform(0, 2 * np.pi))
+        pr += r
+        pi += i
+    dev = torch.device("cuda")
+    t = lambda a: torch.tensor(a, device=dev)
+    with torch.no_grad():
+        target = evolve_chi(t(pr), t(pi), 1.05 * cfg.kappa, cfg, T, stride=T)
+    # plain forward: the Evolver, T steps
+    ev = L.Evolver(cfg, backend="hip")
+    ev.set_psi_real(pr)
+    ev.set_psi_imag(pi)
+    ev.evolve(3)
+    ev.set_psi_real(pr)
+    ev.set_psi_imag(pi)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev.evolve(T)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    res = {}
+    for name, stride in (("keep_all", 1), ("sqrtT", math.ceil(math.sqrt(T)))):
+        pr_t, pi_t = t(pr).requires_grad_(True), t(pi).requires_grad_(True)
+        k = torch.tensor(cfg.kappa, dtype=torch.float64, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        chiT = evolve_chi(pr_t, pi_t, k, cfg, T, stride=stride)
+        loss = ((chiT - target) ** 2).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[name] = {"stride": stride, "wall_s": dt, "cost_ratio": dt / t_fwd,
+                     "loss": float(loss), "grad_kappa": float(k.grad),
+                     "grad_psi_norm": float(torch.sqrt((pr_t.grad ** 2).sum() + (pi_t.grad ** 2).sum())),
+                     "finite": bool(torch.isfinite(pr_t.grad).all() and torch.isfinite(pi_t.grad).all())}
+        del pr_t, pi_t, chiT, loss
+        torch.cuda.empty_cache()
+    best = min(r["cost_ratio"] for r in res.values())
+    out = {"metric": METRIC, "value": best, "unit": "x forward (adjoint cost ratio)", "n_gpus": 1,
+           "steps": T, "warmup": 3, "ms_per_step": t_fwd * 1e3 / T, "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"adjoint COMPLEX {N}^3, {T} steps, mask on", "grid": N},
+           "forward_s": t_fwd, "forward_GLUPS": N ** 3 * T / t_fwd / 1e9, "tapes": res}
+    print(json.dumps(out), flush=True)
+
+
 def batch_tile(tuning):
     if not tuning:
         return None
@@ -237,7 +302,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="color512", choices=sorted(WORKLOADS) + ["copy", "batch64"])
+    ap.add_argument("--workload", default="color512", choices=sorted(WORKLOADS) + ["copy", "batch64", "adjoint128"])
     ap.add_argument("--size", type=int, default=None, help="override the lattice size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--direct", action="store_true", help="use the untiled kernel")
@@ -255,6 +320,9 @@ def main():
 
     if args.workload == "batch64":
         run_batch(args)
+        return
+    if args.workload == "adjoint128":
+        run_adjoint(args)
         return
     if args.workload == "copy":
         # Device-to-device copy of a known size: the HBM calibration for the

@@ -75,4 +75,4 @@ def grads(pr0, pi0, chi_init, kappa, mask, steps, chi_target, **kw):
     _, _, chiT = forward_complex(pr, pi, t(chi_init), k, t(mask), steps, **kw)
     loss = ((chiT - t(chi_target)) ** 2).sum()
     loss.backward()
-    return float(loss), pr.grad.numpy(), pi.grad.numpy(), float(k.grad)
+    return float(loss.detach()), pr.grad.numpy(), pi.grad.numpy(), float(k.grad)

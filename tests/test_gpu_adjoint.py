@@ -1,0 +1,103 @@
+"""GPU tests of the differentiable form (adjoint leapfrog kernel).
+
+The gradients of L = sum((chi_T - chi_target)^2) with respect to Psi_0 and
+kappa from the hand-written adjoint are compared with the FP64 torch
+autograd reference (oracle/adjoint_ref.py, itself pinned by central finite
+differences in tests/test_oracle_adjoint.py) on the same float32 inputs.
+Tolerance: relative L2 of each gradient field <= 2e-3, kappa gradient
+relative error <= 2e-3 (float32 forward and adjoint over the whole horizon).
+Checkpoint strides must not change the result at all.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import adjoint_ref as AR
+from oracle import lfm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def log(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def _setup(N, lam):
+    import lfm_hip as L
+    cfg = L.SimulationConfig(grid_size=N, field_level=L.FieldLevel.COMPLEX, lambda_self=lam)
+    rng = np.random.default_rng(17)
+    pr = np.zeros((N,) * 3, np.float32)
+    pi = np.zeros((N,) * 3, np.float32)
+    for _ in range(2):
+        pos = tuple(N / 2 + rng.uniform(-0.15, 0.15, 3) * N)
+        r, i = L.fields.gaussian_soliton(N, pos, 5.0, 3.0, rng.uniform(0, 2 * np.pi))
+        pr += r
+        pi += i
+    return cfg, pr, pi
+
+
+@pytest.mark.parametrize("lam", [0.0, 4 / 31])
+def test_adjoint_matches_fp64_autograd(gpu, lam):
+    import torch
+    from lfm_hip.adjoint import evolve_chi
+    N, T = 32, 60
+    log(f"adjoint vs FP64 autograd N={N} T={T} lam={lam}")
+    cfg, pr, pi = _setup(N, lam)
+    dev = torch.device("cuda")
+    t = lambda a: torch.tensor(a, device=dev)
+    with torch.no_grad():
+        target = evolve_chi(t(pr), t(pi), 1.05 * cfg.kappa, cfg, T).cpu().numpy()
+    pr_t, pi_t = t(pr).requires_grad_(True), t(pi).requires_grad_(True)
+    k = torch.tensor(cfg.kappa, dtype=torch.float64, device=dev, requires_grad=True)
+    chiT = evolve_chi(pr_t, pi_t, k, cfg, T)
+    loss = ((chiT - t(target)) ** 2).sum()
+    loss.backward()
+    mask = O.boundary_mask(N, cfg.boundary_fraction).reshape((N,) * 3).astype(np.float64)
+    L64, gr, gi, gk = AR.grads(pr, pi, np.full((N,) * 3, 19.0), cfg.kappa, mask, T, target,
+                               lam=lam)
+    e_r = O.rel_l2(pr_t.grad.cpu().numpy(), gr)
+    e_i = O.rel_l2(pi_t.grad.cpu().numpy(), gi)
+    e_k = abs(float(k.grad) - gk) / abs(gk)
+    log(f"  loss gpu {float(loss):.6e} fp64 {L64:.6e}; grad rel-L2 pr {e_r:.2e} pi {e_i:.2e} kappa {e_k:.2e}")
+    assert abs(float(loss) - L64) <= 1e-3 * L64
+    assert e_r <= 2e-3 and e_i <= 2e-3 and e_k <= 2e-3
+
+
+def test_checkpoint_stride_is_exact(gpu):
+    import torch
+    from lfm_hip.adjoint import evolve_chi
+    N, T = 24, 37
+    log("adjoint stride 1 vs sqrt(T) vs 5")
+    cfg, pr, pi = _setup(N, 0.0)
+    dev = torch.device("cuda")
+    out = []
+    for stride in (1, 7, 5):
+        pr_t = torch.tensor(pr, device=dev, requires_grad=True)
+        pi_t = torch.tensor(pi, device=dev, requires_grad=True)
+        k = torch.tensor(cfg.kappa, dtype=torch.float64, device=dev, requires_grad=True)
+        chiT = evolve_chi(pr_t, pi_t, k, cfg, T, stride=stride)
+        ((chiT - 19.0) ** 2).sum().backward()
+        out.append((pr_t.grad.cpu().numpy(), pi_t.grad.cpu().numpy(), float(k.grad)))
+    for o in out[1:]:
+        np.testing.assert_array_equal(o[0], out[0][0])
+        np.testing.assert_array_equal(o[1], out[0][1])
+        assert o[2] == out[0][2]
+
+
+def test_forward_equals_evolver(gpu):
+    import torch
+    import lfm_hip as L
+    from lfm_hip.adjoint import evolve_chi
+    N, T = 24, 30
+    log("adjoint forward vs Evolver")
+    cfg, pr, pi = _setup(N, 4 / 31)
+    ev = L.Evolver(cfg, backend="hip")
+    ev.set_psi_real(pr)
+    ev.set_psi_imag(pi)
+    ev.evolve(T)
+    with torch.no_grad():
+        chiT = evolve_chi(torch.tensor(pr, device="cuda"), torch.tensor(pi, device="cuda"),
+                          cfg.kappa, cfg, T, stride=4).cpu().numpy()
+    np.testing.assert_array_equal(chiT, ev.get_chi())
