@@ -150,9 +150,10 @@ int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_prev, const 
  * LFM_F_CHI_DELTA as used forward) and the adjoints U = dL/dPsi, X = dL/dchi
  * of step n+1, it writes Ut = (1-m) U and Xt = (1-m) 1[no clamp] X of step
  * n+1 and the adjoints Un, Xn of step n; Ut2 / Xt2 are the Ut / Xt written
- * by the previous call (step n+2), NULL for the last step.  dL/dkappa of this
- * step is added to *kappa_grad (device double); partials needs
- * lfm_adjoint_partials_len(N) doubles. */
+ * by the previous call (step n+2), NULL for the last step.  The per-workgroup
+ * terms of dL/dkappa of this step land in partials (lfm_adjoint_partials_len(N)
+ * doubles); when kappa_grad (device double) is not NULL their sum is also
+ * added to *kappa_grad. */
 int64_t lfm_adjoint_partials_len(int32_t N);
 int lfm_adjoint_complex_step_f32(const float* pr, const float* pi, const float* chi,
                                  const float* chi_prev, const float* mask, const float* U_r,

@@ -188,7 +188,7 @@ extern "C" int lfm_adjoint_complex_step_f32(
     const float* Ut2_r, const float* Ut2_i, const float* Xt2, float* Un_r, float* Un_i,
     float* Xn, double* partials, double* kappa_grad, const lfm_params_t* p, void* stream) {
     if (!p || !pr || !pi || !chi || !chi_prev || !U_r || !U_i || !X || !Ut_r || !Ut_i || !Xt ||
-        !Un_r || !Un_i || !Xn || !partials || !kappa_grad) {
+        !Un_r || !Un_i || !Xn || !partials) {
         set_error("lfm_adjoint_complex_step_f32: required pointer is NULL");
         return LFM_E_ARG;
     }
@@ -214,8 +214,10 @@ extern "C" int lfm_adjoint_complex_step_f32(
     LFM_LAUNCH_CHECK("adj_prep");
     hipLaunchKernelGGL(adj_step, dim3(nb), dim3(AB), 0, st, A);
     LFM_LAUNCH_CHECK("adj_step");
-    hipLaunchKernelGGL(sum_into, dim3(1), dim3(AB), 0, st, (const double*)partials, int(nb),
-                       kappa_grad);
-    LFM_LAUNCH_CHECK("sum_into");
+    if (kappa_grad) {
+        hipLaunchKernelGGL(sum_into, dim3(1), dim3(AB), 0, st, (const double*)partials, int(nb),
+                           kappa_grad);
+        LFM_LAUNCH_CHECK("sum_into");
+    }
     return LFM_OK;
 }

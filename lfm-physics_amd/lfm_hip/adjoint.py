@@ -83,56 +83,65 @@ class ComplexTape:
         self.kappa = float(kappa)
         n3 = self.N ** 3
         f32 = dict(dtype=torch.float32, device=self.device)
-        pr0 = pr0.detach().to(**f32).reshape(-1).contiguous()
-        pi0 = pi0.detach().to(**f32).reshape(-1).contiguous()
-        chi_d = (chi_init.detach().to(**f32).reshape(-1) - self.chi_off).contiguous()
-        self.ckpt = {}
+        pr0 = pr0.detach().to(**f32).reshape(-1)
+        pi0 = pi0.detach().to(**f32).reshape(-1)
+        chi_d = chi_init.detach().to(**f32).reshape(-1) - self.chi_off
         if self.K == 1:
-            # every state kept: each step writes straight into its tape slot
-            R, I, C = [pr0], [pi0], [chi_d]
+            # every state kept: one allocation, each step writes into its slot
+            R = torch.empty((self.T + 1, n3), **f32)
+            I = torch.empty((self.T + 1, n3), **f32)
+            C = torch.empty((self.T + 1, n3), **f32)
+            R[0].copy_(pr0)
+            I[0].copy_(pi0)
+            C[0].copy_(chi_d)
             for n in range(self.T):
-                R.append(torch.empty(n3, **f32))
-                I.append(torch.empty(n3, **f32))
-                C.append(torch.empty(n3, **f32))
-                prev_r, prev_i = (R[n - 1], I[n - 1]) if n else (R[0], I[0])
-                self._fwd(R[n], I[n], prev_r, prev_i, C[n], C[n - 1] if n else C[0], n > 0,
-                          R[n + 1], I[n + 1], C[n + 1])
+                m = n - 1 if n else 0
+                self._fwd(R[n], I[n], R[m], I[m], C[n], C[m], n > 0, R[n + 1], I[n + 1], C[n + 1])
             self.tape = (R, I, C)
             return C[self.T] + self.chi_off
-        # three-slot rotation for Psi and chi, checkpoint every K steps
-        R = [pr0.clone(), pr0.clone(), torch.empty(n3, **f32)]
-        I = [pi0.clone(), pi0.clone(), torch.empty(n3, **f32)]
-        Cs = [chi_d.clone(), chi_d.clone(), torch.empty(n3, **f32)]
+        # three-slot rotation; checkpoint (cur, prev, chi, chi_prev) every K steps
+        nck = (self.T + self.K - 1) // self.K
+        self.ck = torch.empty((nck, 6, n3), **f32)
+        self.ck_damped = [False] * nck
+        S = torch.empty((3, 3, n3), **f32)     # [slot][pr, pi, chi]
+        S[0, 0].copy_(pr0); S[1, 0].copy_(pr0)
+        S[0, 1].copy_(pi0); S[1, 1].copy_(pi0)
+        S[0, 2].copy_(chi_d); S[1, 2].copy_(chi_d)
         cur, prev, cc, cp = 0, 1, 0, 1
         damped = False
         for n in range(self.T):
             if n % self.K == 0:
-                self.ckpt[n] = (R[cur].clone(), I[cur].clone(), R[prev].clone(), I[prev].clone(),
-                                Cs[cc].clone(), Cs[cp].clone(), damped)
+                c = self.ck[n // self.K]
+                c[0].copy_(S[cur, 0]); c[1].copy_(S[cur, 1])
+                c[2].copy_(S[prev, 0]); c[3].copy_(S[prev, 1])
+                c[4].copy_(S[cc, 2]); c[5].copy_(S[cp, 2])
+                self.ck_damped[n // self.K] = damped
             nxt = 3 - cur - prev
             cn = 3 - cc - cp
-            self._fwd(R[cur], I[cur], R[prev], I[prev], Cs[cc], Cs[cp], damped, R[nxt], I[nxt], Cs[cn])
+            self._fwd(S[cur, 0], S[cur, 1], S[prev, 0], S[prev, 1], S[cc, 2], S[cp, 2], damped,
+                      S[nxt, 0], S[nxt, 1], S[cn, 2])
             prev, cur, cp, cc = cur, nxt, cc, cn
             damped = True
-        return Cs[cc] + self.chi_off
+        self.seg = torch.empty((self.K + 1, 3, n3), **f32)
+        return S[cc, 2] + self.chi_off
 
     def _segment(self, s: int, e: int):
         """States s..e-1 as (pr, pi, chi, chi_prev): from the tape, or
-        recomputed from checkpoint s."""
+        recomputed from checkpoint s into the segment buffer."""
         if self.K == 1:
             R, I, C = self.tape
             return [(R[n], I[n], C[n], C[n - 1] if n else C[0]) for n in range(s, e)]
-        cr, ci, pr_, pi_, c, cp, damped = self.ckpt[s]
-        n3 = self.N ** 3
-        f32 = dict(dtype=torch.float32, device=self.device)
-        states = [(cr, ci, c, cp)]
-        for n in range(s, e - 1):
-            nr, ni, nc = torch.empty(n3, **f32), torch.empty(n3, **f32), torch.empty(n3, **f32)
-            self._fwd(cr, ci, pr_, pi_, c, cp, damped, nr, ni, nc)
-            pr_, pi_, cp = cr, ci, c
-            cr, ci, c = nr, ni, nc
+        c = self.ck[s // self.K]
+        B = self.seg
+        B[0, 0].copy_(c[0]); B[0, 1].copy_(c[1]); B[0, 2].copy_(c[4])
+        states = [(B[0, 0], B[0, 1], B[0, 2], c[5])]
+        pr_, pi_, damped = c[2], c[3], self.ck_damped[s // self.K]
+        for t in range(e - s - 1):
+            cr, ci, ch, cp = states[-1]
+            self._fwd(cr, ci, pr_, pi_, ch, cp, damped, B[t + 1, 0], B[t + 1, 1], B[t + 1, 2])
+            pr_, pi_ = cr, ci
             damped = True
-            states.append((cr, ci, c, cp))
+            states.append((B[t + 1, 0], B[t + 1, 1], B[t + 1, 2], ch))
         return states
 
     # ----------------------------------------------------------- backward
@@ -140,46 +149,52 @@ class ComplexTape:
         """Adjoint sweep; returns (dL/dpr0, dL/dpi0, dL/dkappa, dL/dchi_init)."""
         n3 = self.N ** 3
         f32 = dict(dtype=torch.float32, device=self.device)
-        X = g_chiT.detach().to(**f32).reshape(-1).contiguous().clone()
-        Ur = (g_prT.detach().to(**f32).reshape(-1).clone() if g_prT is not None
-              else torch.zeros(n3, **f32))
-        Ui = (g_piT.detach().to(**f32).reshape(-1).clone() if g_piT is not None
-              else torch.zeros(n3, **f32))
-        Unr, Uni, Xn = torch.empty(n3, **f32), torch.empty(n3, **f32), torch.empty(n3, **f32)
-        Ut = [(torch.empty(n3, **f32), torch.empty(n3, **f32), torch.empty(n3, **f32)) for _ in range(2)]
-        part = torch.empty(int(self._lib.lfm_adjoint_partials_len(self.N)), dtype=torch.float64,
-                           device=self.device)
-        kgrad = torch.zeros(1, dtype=torch.float64, device=self.device)
+        A = torch.empty((11, n3), **f32)
+        Ur, Ui, X, Unr, Uni, Xn = A[0], A[1], A[2], A[3], A[4], A[5]
+        Ut = [(A[6], A[7], A[8]), (A[9], A[10], None)]
+        Xt1 = torch.empty(n3, **f32)
+        Ut[1] = (A[9], A[10], Xt1)
+        X.copy_(g_chiT.detach().to(**f32).reshape(-1))
+        if g_prT is not None:
+            Ur.copy_(g_prT.detach().to(**f32).reshape(-1))
+        else:
+            Ur.zero_()
+        if g_piT is not None:
+            Ui.copy_(g_piT.detach().to(**f32).reshape(-1))
+        else:
+            Ui.zero_()
+        nb = int(self._lib.lfm_adjoint_partials_len(self.N))
+        part = torch.empty((self.T, nb), dtype=torch.float64, device=self.device)
         p = self._p(self._flags(False) & ~_abi.LFM_F_DAMP_PREV)
+        mask = self.mask.data_ptr() if self.has_mask else None
+        st = stream_handle()
         slot, have2 = 0, False
-        starts = list(range(0, self.T, self.K))
-        for s in reversed(starts):
+        for s in reversed(range(0, self.T, self.K)):
             e = min(s + self.K, self.T)
             states = self._segment(s, e)
             for n in range(e - 1, s - 1, -1):
                 cr, ci, c, cp = states[n - s]
                 u1 = Ut[slot]
-                u2 = Ut[1 - slot] if have2 else (None, None, None)
+                u2 = Ut[1 - slot]
                 rc = self._lib.lfm_adjoint_complex_step_f32(
-                    cr.data_ptr(), ci.data_ptr(), c.data_ptr(), cp.data_ptr(),
-                    self.mask.data_ptr() if self.has_mask else None, Ur.data_ptr(), Ui.data_ptr(),
-                    X.data_ptr(), u1[0].data_ptr(), u1[1].data_ptr(), u1[2].data_ptr(),
-                    u2[0].data_ptr() if have2 else None, u2[1].data_ptr() if have2 else None,
-                    u2[2].data_ptr() if have2 else None, Unr.data_ptr(), Uni.data_ptr(),
-                    Xn.data_ptr(), part.data_ptr(), kgrad.data_ptr(), ctypes.byref(p),
-                    stream_handle())
+                    cr.data_ptr(), ci.data_ptr(), c.data_ptr(), cp.data_ptr(), mask,
+                    Ur.data_ptr(), Ui.data_ptr(), X.data_ptr(), u1[0].data_ptr(),
+                    u1[1].data_ptr(), u1[2].data_ptr(), u2[0].data_ptr() if have2 else None,
+                    u2[1].data_ptr() if have2 else None, u2[2].data_ptr() if have2 else None,
+                    Unr.data_ptr(), Uni.data_ptr(), Xn.data_ptr(), part[n].data_ptr(), None,
+                    ctypes.byref(p), st)
                 _abi.check(rc, "lfm_adjoint_complex_step_f32")
                 Ur, Unr = Unr, Ur
                 Ui, Uni = Uni, Ui
                 X, Xn = Xn, X
                 slot, have2 = 1 - slot, True
-            del states
         # the last call wrote Ut, Xt of step 1 into Ut[1 - slot]
         u1 = Ut[1 - slot]
         g_pr0 = Ur - u1[0]        # dL/dcur_0 + dL/dprev_0, prev_0 = Psi_0
         g_pi0 = Ui - u1[1]
         g_chi = X - u1[2]         # dL/dchi_0 + dL/dchi_prev_0
-        return g_pr0, g_pi0, float(kgrad.item()), g_chi
+        g_k = float(part.sum().item())
+        return g_pr0, g_pi0, g_k, g_chi
 
 
 class _ChiRun(torch.autograd.Function):

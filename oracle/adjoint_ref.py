@@ -76,3 +76,15 @@ def grads(pr0, pi0, chi_init, kappa, mask, steps, chi_target, **kw):
     loss = ((chiT - t(chi_target)) ** 2).sum()
     loss.backward()
     return float(loss.detach()), pr.grad.numpy(), pi.grad.numpy(), float(k.grad)
+
+
+def vjp(pr0, pi0, chi_init, kappa, mask, steps, cotangent, **kw):
+    """FP64 vector-Jacobian product of chi_T with a fixed cotangent:
+    gradients of sum(cotangent * chi_T) w.r.t. Psi_0 and kappa."""
+    t = lambda x: torch.as_tensor(np.asarray(x, dtype=np.float64))
+    pr = t(pr0).requires_grad_(True)
+    pi = t(pi0).requires_grad_(True)
+    k = torch.tensor(float(kappa), dtype=torch.float64, requires_grad=True)
+    _, _, chiT = forward_complex(pr, pi, t(chi_init), k, t(mask), steps, **kw)
+    (chiT * t(cotangent)).sum().backward()
+    return pr.grad.numpy(), pi.grad.numpy(), float(k.grad)

@@ -1,12 +1,15 @@
 """GPU tests of the differentiable form (adjoint leapfrog kernel).
 
-The gradients of L = sum((chi_T - chi_target)^2) with respect to Psi_0 and
-kappa from the hand-written adjoint are compared with the FP64 torch
-autograd reference (oracle/adjoint_ref.py, itself pinned by central finite
-differences in tests/test_oracle_adjoint.py) on the same float32 inputs.
-Tolerance: relative L2 of each gradient field <= 2e-3, kappa gradient
-relative error <= 2e-3 (float32 forward and adjoint over the whole horizon).
-Checkpoint strides must not change the result at all.
+The adjoint is checked as a vector-Jacobian product: with the cotangent
+g = 2 (chi_T - chi_target) of L = sum((chi_T - chi_target)^2) held fixed,
+the hand-written adjoint's dL/dPsi_0 and dL/dkappa are compared with the FP64
+torch autograd reference (oracle/adjoint_ref.py, itself pinned by central
+finite differences in tests/test_oracle_adjoint.py) on the same float32
+inputs.  (Feeding each side its own chi_T would compare float32 and float64
+forward residuals, which at this loss level are of the order of float32
+rounding of chi.)  Tolerance: relative L2 of each gradient field <= 2e-3 and
+kappa relative error <= 2e-3.  Checkpoint strides must not change the
+result at all.
 """
 import sys
 
@@ -52,16 +55,15 @@ def test_adjoint_matches_fp64_autograd(gpu, lam):
     pr_t, pi_t = t(pr).requires_grad_(True), t(pi).requires_grad_(True)
     k = torch.tensor(cfg.kappa, dtype=torch.float64, device=dev, requires_grad=True)
     chiT = evolve_chi(pr_t, pi_t, k, cfg, T)
-    loss = ((chiT - t(target)) ** 2).sum()
-    loss.backward()
+    cot = 2.0 * (chiT.detach() - t(target))
+    chiT.backward(cot)
     mask = O.boundary_mask(N, cfg.boundary_fraction).reshape((N,) * 3).astype(np.float64)
-    L64, gr, gi, gk = AR.grads(pr, pi, np.full((N,) * 3, 19.0), cfg.kappa, mask, T, target,
-                               lam=lam)
+    cot64 = cot.double().cpu().numpy()
+    gr, gi, gk = AR.vjp(pr, pi, np.full((N,) * 3, 19.0), cfg.kappa, mask, T, cot64, lam=lam)
     e_r = O.rel_l2(pr_t.grad.cpu().numpy(), gr)
     e_i = O.rel_l2(pi_t.grad.cpu().numpy(), gi)
     e_k = abs(float(k.grad) - gk) / abs(gk)
-    log(f"  loss gpu {float(loss):.6e} fp64 {L64:.6e}; grad rel-L2 pr {e_r:.2e} pi {e_i:.2e} kappa {e_k:.2e}")
-    assert abs(float(loss) - L64) <= 1e-3 * L64
+    log(f"  |cot| {np.linalg.norm(cot64):.3e}; grad rel-L2 pr {e_r:.2e} pi {e_i:.2e} kappa {e_k:.2e}")
     assert e_r <= 2e-3 and e_i <= 2e-3 and e_k <= 2e-3
 
 
