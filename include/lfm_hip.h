@@ -102,6 +102,12 @@ int lfm_step_color_f32(const float* pr, const float* pr_prev, const float* pi,
  * for a < ncolors, written unclipped.  A plan owns rocFFT plans and the
  * spectrum/work buffers for one (N, ncolors); it is not thread-safe. */
 int lfm_scv_plan_create(int32_t N, int32_t ncolors, void** plan);
+/* method 0 = automatic; 1 = rocFFT 3-D R2C/C2R plans + a k-space filter
+ * kernel; 2 = rocFFT 1-D R2C/C2R along k plus hand-written column FFT passes
+ * along j and i, the i pass fusing forward transform, filter and inverse
+ * transform (power-of-two N in [16, 1024]; automatic picks it there). */
+int lfm_scv_plan_create_ex(int32_t N, int32_t ncolors, int32_t method, void** plan);
+int32_t lfm_scv_plan_method(void* plan);
 int lfm_scv_plan_destroy(void* plan);
 /* e = |Psi_a|^2 given directly (ncolors*N^3). */
 int lfm_scv_apply_psisq(void* plan, const float* e, float gamma, float D, float* sa_out,

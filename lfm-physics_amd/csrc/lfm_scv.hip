@@ -21,12 +21,14 @@ namespace {
 
 struct ScvPlan {
     int N = 0, nc = 0;
+    int method = 0;          // 1: rocFFT 3-D plans; 2: 1-D rocFFT along k + column passes
     rocfft_plan fwd = nullptr, inv = nullptr;
     rocfft_execution_info info = nullptr;
     void* work = nullptr;
     size_t work_bytes = 0;
     float2* spec = nullptr;  // nc * N * N * (N/2+1)
     float* e = nullptr;      // nc * N^3 (|Psi_a|^2 when computed here)
+    float2* tw = nullptr;    // exp(-2 pi i m / N), m < N (column passes)
 };
 
 std::once_flag g_fft_once;
@@ -139,6 +141,216 @@ __global__ __launch_bounds__(256) void poisson_finish(float* __restrict__ chi, i
     }
 }
 
+// ---------------------------------------------------------------------------
+// Column passes: 1-D FFTs of length N along the j or the i axis of the half
+// spectrum [a][i][j][kk] (kk < H = N/2+1 contiguous).  A workgroup loads
+// NCOL adjacent kk columns of one (a, i) or (a, j) slab (NCOL contiguous
+// complex values per row, rows `rs` apart) into LDS, runs a Stockham
+// autosort FFT per column (radix-8 stages, one radix-2/4 stage when log2 N is
+// not a multiple of 3; every thread owns 8 values per stage) and writes the
+// columns back.  The fused mode transforms along i, multiplies by the
+// spectral filter and transforms back, so the i axis is read and written
+// once for the forward transform, the filter and the inverse transform.
+// ---------------------------------------------------------------------------
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by -i (forward, sign = -1) or +i (inverse, sign = +1)
+__device__ __forceinline__ float2 rot(float2 a, int sign) {
+    return sign < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+template <int R>
+__device__ __forceinline__ void dft(float2 (&x)[R], int sign) {
+    if constexpr (R == 2) {
+        const float2 a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        const float2 t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]);
+        const float2 t2 = cadd(x[1], x[3]), t3 = rot(csub(x[1], x[3]), sign);
+        x[0] = cadd(t0, t2);
+        x[2] = csub(t0, t2);
+        x[1] = cadd(t1, t3);
+        x[3] = csub(t1, t3);
+    } else {
+        const float c = 0.70710678118654752f;
+        const float2 t0 = cadd(x[0], x[4]), t1 = csub(x[0], x[4]);
+        const float2 t2 = cadd(x[2], x[6]), t3 = rot(csub(x[2], x[6]), sign);
+        const float2 t4 = cadd(x[1], x[5]), t5 = csub(x[1], x[5]);
+        const float2 t6 = cadd(x[3], x[7]), t7 = rot(csub(x[3], x[7]), sign);
+        const float2 e0 = cadd(t0, t2), e2 = csub(t0, t2), e1 = cadd(t1, t3), e3 = csub(t1, t3);
+        const float2 o0 = cadd(t4, t6), o2 = csub(t4, t6), o1 = cadd(t5, t7), o3 = csub(t5, t7);
+        // W8^1 = (c, sign c), W8^2 = sign i, W8^3 = (-c, sign c)
+        const float2 w1o1 = make_float2(c * (o1.x - sign * o1.y), c * (o1.y + sign * o1.x));
+        const float2 w2o2 = rot(o2, sign);
+        const float2 w3o3 = make_float2(-c * (o3.x + sign * o3.y), c * (sign * o3.x - o3.y));
+        x[0] = cadd(e0, o0);
+        x[4] = csub(e0, o0);
+        x[1] = cadd(e1, w1o1);
+        x[5] = csub(e1, w1o1);
+        x[2] = cadd(e2, w2o2);
+        x[6] = csub(e2, w2o2);
+        x[3] = cadd(e3, w3o3);
+        x[7] = csub(e3, w3o3);
+    }
+}
+
+// One Stockham stage of radix R on one column `col` (length N, LDS stride
+// CS between columns) by thread t of TC = N/8 column threads.
+template <int N, int R, int CS>
+__device__ __forceinline__ void stage(float2* lds, int col, int t, int Ns, int sign,
+                                      const float2* __restrict__ tw) {
+    constexpr int B = N / R;           // butterflies in the stage
+    constexpr int TC = N / 8;          // threads per column
+    constexpr int NB = B / TC;         // butterflies per thread (8 / R)
+    float2 x[NB][R];
+    float2* c = lds + col * CS;
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        const int j = t + u * TC;
+        const int k = j % Ns;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float2 v = c[j + r * B];
+            if (r > 0 && Ns > 1) {
+                float2 w = tw[(r * k * (N / (Ns * R))) & (N - 1)];
+                if (sign > 0) w.y = -w.y;
+                v = cmul(v, w);
+            }
+            x[u][r] = v;
+        }
+        dft<R>(x[u], sign);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        const int j = t + u * TC;
+        const int k = j % Ns;
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) c[base + r * Ns] = x[u][r];
+    }
+    __syncthreads();
+}
+
+template <int N, int CS>
+__device__ __forceinline__ void fft_col(float2* lds, int col, int t, int sign,
+                                        const float2* __restrict__ tw) {
+    constexpr int L = ilog2(N);
+    constexpr int n8 = L / 3;
+    constexpr int rem = L % 3;
+    int Ns = 1;
+#pragma unroll
+    for (int st = 0; st < n8; ++st) {
+        stage<N, 8, CS>(lds, col, t, Ns, sign, tw);
+        Ns *= 8;
+    }
+    if constexpr (rem == 1) stage<N, 2, CS>(lds, col, t, Ns, sign, tw);
+    if constexpr (rem == 2) stage<N, 4, CS>(lds, col, t, Ns, sign, tw);
+}
+
+struct ColArgs {
+    float2* spec;
+    const float2* tw;
+    int nc;
+    long long slab_stride;   // complex elements between consecutive slab indices x
+    long long row_stride;    // complex elements between consecutive rows (FFT axis)
+    int mode;                // 0 forward, 1 inverse, 2 forward + filter + inverse (i axis)
+    int filter;              // 0 Helmholtz gamma/(gamma + D k^2), 1 Poisson -1/k^2
+    float gamma, D, scale;
+};
+
+template <int N>
+constexpr int ncol_for() { return N >= 1024 ? 8 : 16; }
+
+template <int N>
+__global__ __launch_bounds__(ncol_for<N>() * (N / 8)) void fft_columns(ColArgs A) {
+    constexpr int NCOL = ncol_for<N>();
+    constexpr int TC = N / 8;
+    constexpr int H = N / 2 + 1;
+    constexpr int CS = N + 1;                 // LDS column stride (odd: spreads banks)
+    __shared__ float2 lds[NCOL * CS];
+    constexpr int NCB = (H + NCOL - 1) / NCOL;
+    const int cb = blockIdx.x % NCB;
+    const int slab = blockIdx.x / NCB;         // a * N + x
+    const int a = slab / N, x = slab % N;
+    const int kk0 = cb * NCOL;
+    const int nv = min(NCOL, H - kk0);
+    float2* base = A.spec + (long long)a * N * N * H + (long long)x * A.slab_stride + kk0;
+    const int tid = threadIdx.x;
+    constexpr int NT = NCOL * TC;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = tid + q * NT;
+        const int row = e / NCOL, col = e % NCOL;
+        lds[col * CS + row] = col < nv ? base[(long long)row * A.row_stride + col] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const int col = tid / TC, t = tid % TC;
+    const int sign = A.mode == 1 ? 1 : -1;
+    fft_col<N, CS>(lds, col, t, sign, A.tw);
+    if (A.mode == 2) {
+        // multiply by the filter at (k_i = row, k_j = x, k_kk = kk0 + col)
+        const float two_pi_over_n = 6.283185307179586f / float(N);
+        const int fj = x < (N + 1) / 2 ? x : x - N;
+        const float kx2 = (two_pi_over_n * fj) * (two_pi_over_n * fj);
+        const float kz = two_pi_over_n * float(kk0 + col);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = t + u * TC;
+            const int fi = i < (N + 1) / 2 ? i : i - N;
+            const float ki = two_pi_over_n * float(fi);
+            const float k2 = ki * ki + kx2 + kz * kz;
+            float h;
+            if (A.filter == 0) h = A.gamma / (A.gamma + A.D * k2) * A.scale;
+            else h = (k2 > 0.0f) ? -A.scale / k2 : 0.0f;
+            float2 v = lds[col * CS + i];
+            v.x *= h;
+            v.y *= h;
+            lds[col * CS + i] = v;
+        }
+        __syncthreads();
+        fft_col<N, CS>(lds, col, t, 1, A.tw);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = tid + q * NT;
+        const int row = e / NCOL, cl = e % NCOL;
+        if (cl < nv) base[(long long)row * A.row_stride + cl] = lds[cl * CS + row];
+    }
+}
+
+template <int N>
+int launch_columns(const ColArgs& A, hipStream_t st) {
+    constexpr int NCOL = ncol_for<N>();
+    constexpr int H = N / 2 + 1;
+    constexpr int NCB = (H + NCOL - 1) / NCOL;
+    const long long nb = (long long)A.nc * N * NCB;
+    hipLaunchKernelGGL(fft_columns<N>, dim3(unsigned(nb)), dim3(NCOL * (N / 8)), 0, st, A);
+    LFM_LAUNCH_CHECK("fft_columns");
+    return LFM_OK;
+}
+
+int columns(int N, const ColArgs& A, hipStream_t st) {
+    switch (N) {
+        case 16: return launch_columns<16>(A, st);
+        case 32: return launch_columns<32>(A, st);
+        case 64: return launch_columns<64>(A, st);
+        case 128: return launch_columns<128>(A, st);
+        case 256: return launch_columns<256>(A, st);
+        case 512: return launch_columns<512>(A, st);
+        case 1024: return launch_columns<1024>(A, st);
+        default: set_error("column FFT: N=%d unsupported", N); return LFM_E_ARG;
+    }
+}
+
+bool columns_supported(int N) { return N >= 16 && N <= 1024 && (N & (N - 1)) == 0; }
+
 __global__ __launch_bounds__(256) void clip_kernel(const float* __restrict__ x,
                                                    float* __restrict__ y, size_t n) {
     for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
@@ -161,23 +373,54 @@ void destroy(ScvPlan* p) {
     if (p->work) (void)hipFree(p->work);
     if (p->spec) (void)hipFree(p->spec);
     if (p->e) (void)hipFree(p->e);
+    if (p->tw) (void)hipFree(p->tw);
     delete p;
 }
 
-int run(ScvPlan* p, const float* e, float gamma, float D, float* sa_out, hipStream_t st) {
+// Transform e -> spectrum -> filter -> out.  filter 0: Helmholtz; 1: Poisson.
+int run_filter(ScvPlan* p, const float* e, int filter, float gamma, float D, float* out,
+               hipStream_t st, int ncomp_used) {
     LFM_FFT_TRY(rocfft_execution_info_set_stream(p->info, st));
+    const int N = p->N;
+    const double n3 = double(N) * N * N;
     void* in_f[1] = {const_cast<float*>(e)};
     void* out_f[1] = {p->spec};
     LFM_FFT_TRY(rocfft_execute(p->fwd, in_f, out_f, p->info));
-    const size_t ns = size_t(p->nc) * p->N * p->N * (p->N / 2 + 1);
-    const double n3 = double(p->N) * p->N * p->N;
-    hipLaunchKernelGGL(helmholtz_filter, dim3(grid_for(ns)), dim3(256), 0, st, p->spec, p->N,
-                       p->nc, gamma, D, float(1.0 / n3));
-    LFM_LAUNCH_CHECK("helmholtz_filter");
+    if (p->method == 2) {
+        const long long H = N / 2 + 1;
+        ColArgs A{p->spec, p->tw, ncomp_used, 0, 0, 0, filter, gamma, D, float(1.0 / n3)};
+        // forward along j: slab (a, i), rows j
+        A.slab_stride = (long long)N * H; A.row_stride = H; A.mode = 0;
+        int rc = columns(N, A, st);
+        if (rc) return rc;
+        // along i: forward, filter, inverse in one pass; slab (a, j), rows i
+        A.slab_stride = H; A.row_stride = (long long)N * H; A.mode = 2;
+        rc = columns(N, A, st);
+        if (rc) return rc;
+        // inverse along j
+        A.slab_stride = (long long)N * H; A.row_stride = H; A.mode = 1;
+        rc = columns(N, A, st);
+        if (rc) return rc;
+    } else {
+        const size_t ns = size_t(p->nc) * N * N * (N / 2 + 1);
+        if (filter == 0) {
+            hipLaunchKernelGGL(helmholtz_filter, dim3(grid_for(ns)), dim3(256), 0, st, p->spec, N,
+                               p->nc, gamma, D, float(1.0 / n3));
+            LFM_LAUNCH_CHECK("helmholtz_filter");
+        } else {
+            hipLaunchKernelGGL(poisson_filter, dim3(grid_for(ns)), dim3(256), 0, st, p->spec, N,
+                               float(1.0 / n3));
+            LFM_LAUNCH_CHECK("poisson_filter");
+        }
+    }
     void* in_i[1] = {p->spec};
-    void* out_i[1] = {sa_out};
+    void* out_i[1] = {out};
     LFM_FFT_TRY(rocfft_execute(p->inv, in_i, out_i, p->info));
     return LFM_OK;
+}
+
+int run(ScvPlan* p, const float* e, float gamma, float D, float* sa_out, hipStream_t st) {
+    return run_filter(p, e, 0, gamma, D, sa_out, st, p->nc);
 }
 
 }  // namespace
@@ -185,11 +428,20 @@ int run(ScvPlan* p, const float* e, float gamma, float D, float* sa_out, hipStre
 
 using namespace lfm;
 
-extern "C" int lfm_scv_plan_create(int32_t N, int32_t ncolors, void** plan) {
+extern "C" int lfm_scv_plan_create_ex(int32_t N, int32_t ncolors, int32_t method, void** plan) {
     if (!plan) { set_error("lfm_scv_plan_create: plan is NULL"); return LFM_E_ARG; }
     *plan = nullptr;
     if (N < 8 || N > 2048 || ncolors < 1 || ncolors > 8) {
         set_error("lfm_scv_plan_create: N=%d ncolors=%d out of range", N, ncolors);
+        return LFM_E_ARG;
+    }
+    if (method == 0) method = columns_supported(N) ? 2 : 1;
+    if (method == 2 && !columns_supported(N)) {
+        set_error("lfm_scv_plan_create: column passes need a power-of-two N in [16, 1024] (N=%d)", N);
+        return LFM_E_ARG;
+    }
+    if (method != 1 && method != 2) {
+        set_error("lfm_scv_plan_create: method %d unknown", method);
         return LFM_E_ARG;
     }
     std::call_once(g_fft_once, [] { g_fft_setup = rocfft_setup(); });
@@ -200,15 +452,28 @@ extern "C" int lfm_scv_plan_create(int32_t N, int32_t ncolors, void** plan) {
     auto* p = new ScvPlan();
     p->N = N;
     p->nc = ncolors;
-    // rocFFT lengths are fastest-first: (k, j, i) for the C-order (i, j, k) array.
-    const size_t len[3] = {size_t(N), size_t(N), size_t(N)};
-    rocfft_status s = rocfft_plan_create(&p->fwd, rocfft_placement_notinplace,
-                                         rocfft_transform_type_real_forward,
-                                         rocfft_precision_single, 3, len, size_t(ncolors), nullptr);
-    if (s == rocfft_status_success)
-        s = rocfft_plan_create(&p->inv, rocfft_placement_notinplace,
-                               rocfft_transform_type_real_inverse, rocfft_precision_single, 3, len,
-                               size_t(ncolors), nullptr);
+    p->method = method;
+    rocfft_status s;
+    if (method == 1) {
+        // rocFFT lengths are fastest-first: (k, j, i) for the C-order (i, j, k) array.
+        const size_t len[3] = {size_t(N), size_t(N), size_t(N)};
+        s = rocfft_plan_create(&p->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                               rocfft_precision_single, 3, len, size_t(ncolors), nullptr);
+        if (s == rocfft_status_success)
+            s = rocfft_plan_create(&p->inv, rocfft_placement_notinplace,
+                                   rocfft_transform_type_real_inverse, rocfft_precision_single, 3,
+                                   len, size_t(ncolors), nullptr);
+    } else {
+        // 1-D real transforms along the contiguous k axis, batch ncolors * N * N
+        const size_t len[1] = {size_t(N)};
+        const size_t batch = size_t(ncolors) * N * N;
+        s = rocfft_plan_create(&p->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                               rocfft_precision_single, 1, len, batch, nullptr);
+        if (s == rocfft_status_success)
+            s = rocfft_plan_create(&p->inv, rocfft_placement_notinplace,
+                                   rocfft_transform_type_real_inverse, rocfft_precision_single, 1,
+                                   len, batch, nullptr);
+    }
     size_t wf = 0, wi = 0;
     if (s == rocfft_status_success) s = rocfft_plan_get_work_buffer_size(p->fwd, &wf);
     if (s == rocfft_status_success) s = rocfft_plan_get_work_buffer_size(p->inv, &wi);
@@ -224,8 +489,19 @@ extern "C" int lfm_scv_plan_create(int32_t N, int32_t ncolors, void** plan) {
     const size_t ns = size_t(ncolors) * N * N * (N / 2 + 1);
     if (he == hipSuccess) he = hipMalloc((void**)&p->spec, ns * sizeof(float2));
     if (he == hipSuccess) he = hipMalloc((void**)&p->e, size_t(ncolors) * N * N * N * sizeof(float));
+    if (he == hipSuccess && method == 2) {
+        // twiddles exp(-2 pi i m / N), computed in double on the host
+        float2* h = new float2[N];
+        for (int m = 0; m < N; ++m) {
+            const double ang = -2.0 * 3.14159265358979323846 * double(m) / double(N);
+            h[m] = make_float2(float(cos(ang)), float(sin(ang)));
+        }
+        he = hipMalloc((void**)&p->tw, N * sizeof(float2));
+        if (he == hipSuccess) he = hipMemcpy(p->tw, h, N * sizeof(float2), hipMemcpyHostToDevice);
+        delete[] h;
+    }
     if (he != hipSuccess) {
-        set_error("lfm_scv_plan_create: hipMalloc failed: %s", hipGetErrorString(he));
+        set_error("lfm_scv_plan_create: %s", hipGetErrorString(he));
         destroy(p);
         return LFM_E_HIP;
     }
@@ -239,6 +515,14 @@ extern "C" int lfm_scv_plan_create(int32_t N, int32_t ncolors, void** plan) {
     }
     *plan = p;
     return LFM_OK;
+}
+
+extern "C" int lfm_scv_plan_create(int32_t N, int32_t ncolors, void** plan) {
+    return lfm_scv_plan_create_ex(N, ncolors, 0, plan);
+}
+
+extern "C" int32_t lfm_scv_plan_method(void* plan) {
+    return plan ? static_cast<ScvPlan*>(plan)->method : -1;
 }
 
 extern "C" int lfm_scv_plan_destroy(void* plan) {
@@ -298,17 +582,8 @@ extern "C" int lfm_poisson_chi_f32(void* plan, const float* pr, const float* pi,
     hipLaunchKernelGGL(poisson_rhs, dim3(grid_for(n3)), dim3(256), 0, st, pr, pi, ncomp, kappa,
                        e0_sq, p->e, n3);
     LFM_LAUNCH_CHECK("poisson_rhs");
-    LFM_FFT_TRY(rocfft_execution_info_set_stream(p->info, st));
-    void* in_f[1] = {p->e};
-    void* out_f[1] = {p->spec};
-    LFM_FFT_TRY(rocfft_execute(p->fwd, in_f, out_f, p->info));
-    const size_t ns = size_t(p->N) * p->N * (p->N / 2 + 1);
-    hipLaunchKernelGGL(poisson_filter, dim3(grid_for(ns)), dim3(256), 0, st, p->spec, p->N,
-                       float(1.0 / double(n3)));
-    LFM_LAUNCH_CHECK("poisson_filter");
-    void* in_i[1] = {p->spec};
-    void* out_i[1] = {chi_out};
-    LFM_FFT_TRY(rocfft_execute(p->inv, in_i, out_i, p->info));
+    const int rc = run_filter(p, p->e, 1, 0.0f, 0.0f, chi_out, st, 1);
+    if (rc) return rc;
     hipLaunchKernelGGL(poisson_finish, dim3(grid_for(n3)), dim3(256), 0, st, chi_out, p->N, chi0,
                        boundary_fraction);
     LFM_LAUNCH_CHECK("poisson_finish");

@@ -36,6 +36,8 @@ SIGNATURES = [
     ("lfm_step_complex_f32", ctypes.c_int, _STEP_COMPLEX),
     ("lfm_step_color_f32", ctypes.c_int, _STEP_COLOR),
     ("lfm_scv_plan_create", ctypes.c_int, [_i32, _i32, ctypes.POINTER(_vp)]),
+    ("lfm_scv_plan_create_ex", ctypes.c_int, [_i32, _i32, _i32, ctypes.POINTER(_vp)]),
+    ("lfm_scv_plan_method", _i32, [_vp]),
     ("lfm_scv_plan_destroy", ctypes.c_int, [_vp]),
     ("lfm_scv_apply_psisq", ctypes.c_int, [_vp, _vp, _f32, _f32, _vp, _vp]),
     ("lfm_scv_apply", ctypes.c_int, [_vp, _vp, _vp, _f32, _f32, _vp, _vp]),

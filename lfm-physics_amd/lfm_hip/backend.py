@@ -57,13 +57,14 @@ def stream_handle() -> int:
 class ScvPlan:
     """Owner of one lfm_scv plan (batched rocFFT R2C/C2R for ncolors fields)."""
 
-    def __init__(self, N: int, ncolors: int = 3):
+    def __init__(self, N: int, ncolors: int = 3, method: int = 0):
         self._lib = _abi.load()
         h = ctypes.c_void_p()
-        _abi.check(self._lib.lfm_scv_plan_create(N, ncolors, ctypes.byref(h)),
-                   "lfm_scv_plan_create")
+        _abi.check(self._lib.lfm_scv_plan_create_ex(N, ncolors, method, ctypes.byref(h)),
+                   "lfm_scv_plan_create_ex")
         self.handle = h
         self.N, self.nc = N, ncolors
+        self.method = int(self._lib.lfm_scv_plan_method(h))
 
     def apply(self, pr, pi, gamma, D, sa_out):
         _abi.check(self._lib.lfm_scv_apply(self.handle, pr.data_ptr(), pi.data_ptr(),

@@ -422,3 +422,45 @@ def test_high_level_simulation_matches_oracle(gpu):
     np.testing.assert_allclose(sim.chi.ravel(), st.chi, atol=5e-4, rtol=1e-3)
     assert O.rel_l2(sim.chi, st.chi) < TOL64
     assert O.rel_l2(sim.psi_real, st.psi_r) < TOL64
+
+
+@pytest.mark.parametrize("N", [16, 32, 64, 128])
+def test_sa_transform_methods_agree_with_oracle(gpu, N):
+    """S_a through rocFFT 3-D plans (method 1) and through the column passes
+    (method 2) against the FP64 Helmholtz oracle; Poisson chi likewise."""
+    import ctypes
+    import torch
+    import lfm_hip as L
+    from lfm_hip.backend import ScvPlan, stream_handle
+    log(f"S_a methods N={N}")
+    rng = np.random.default_rng(N)
+    e = (rng.random((3, N, N, N)) * 2.0).astype(np.float32)
+    e[0] += L.fields.gaussian_soliton(N, (N / 3, N / 2, N / 2), 4.0, 2.0)[0] ** 2
+    ref = np.concatenate([O.helmholtz_sa(e[a].astype(np.float64), N, 0.1, 4.9) for a in range(3)])
+    et = torch.tensor(e.ravel(), device="cuda")
+    outs = {}
+    for method in (1, 2):
+        plan = ScvPlan(N, 3, method)
+        assert plan.method == method
+        sa = torch.empty(3 * N ** 3, device="cuda")
+        plan.apply_psisq(et, 0.1, 4.9, sa)
+        torch.cuda.synchronize()
+        outs[method] = np.maximum(sa.cpu().numpy(), 0.0)
+        err = O.rel_l2(outs[method], ref)
+        log(f"  method {method}: rel-L2 vs FP64 {err:.2e}")
+        assert err < 2e-6
+    assert O.rel_l2(outs[1], outs[2]) < 2e-6
+    # Poisson (GOV-04) chi through both methods
+    pr = torch.tensor(e[0].ravel(), device="cuda")
+    lib = L._abi.load()
+    ref_chi = O.poisson_chi(e[0].astype(np.float64) ** 2, N, 19.0, 1 / 63, frozen_bf=0.3)
+    for method in (1, 2):
+        plan = ScvPlan(N, 1, method)
+        chi = torch.empty(N ** 3, device="cuda")
+        rc = lib.lfm_poisson_chi_f32(plan.handle, pr.data_ptr(), None, 1, 1 / 63, 0.0, 19.0, 0.3,
+                                     chi.data_ptr(), stream_handle())
+        assert rc == 0
+        torch.cuda.synchronize()
+        err = O.rel_l2(chi.cpu().numpy() - 19.0, ref_chi.ravel() - 19.0)
+        log(f"  poisson method {method}: rel-L2 {err:.2e}")
+        assert err < 1e-5
