@@ -5,11 +5,15 @@
 //
 // Reference: lfm/core/backends/numpy_backend.py:338-358 and
 // cupy_backend.py:260-279, which rebuild the k-grid and the float64 filter on
-// every step and run one transform pair per colour.  Here one batched
-// (ncolors) single-precision rocFFT R2C / C2R plan pair is built once per
-// (N, ncolors); the filter is evaluated on the fly by one k-space kernel that
-// also folds in the 1/N^3 normalisation of the inverse transform.  The
-// clip S_a >= 0 happens where S_a is consumed (the step kernel).
+// every step and run one transform pair per colour.  Here a plan is built
+// once per (N, ncolors) and all colours go through one batched transform;
+// the filter (with the 1/N^3 normalisation of the inverse) is evaluated on
+// the fly.  Two methods:
+//   2 (default for power-of-two N in [16, 1024]): five passes written here --
+//     real lines along k to half spectra, complex lines along j, one fused
+//     forward / filter / inverse pass along i, lines along j, back to real;
+//   1 (any other N): rocFFT 3-D R2C / C2R plans and one filter kernel.
+// The clip S_a >= 0 happens where S_a is consumed (the step kernel).
 #include <rocfft/rocfft.h>
 
 #include <mutex>
@@ -21,14 +25,15 @@ namespace {
 
 struct ScvPlan {
     int N = 0, nc = 0;
-    int method = 0;          // 1: rocFFT 3-D plans; 2: 1-D rocFFT along k + column passes
+    int method = 0;          // 1: rocFFT 3-D plans; 2: row and column passes written here
+    int Hp = 0;              // row pitch of the half spectrum along kk
     rocfft_plan fwd = nullptr, inv = nullptr;
     rocfft_execution_info info = nullptr;
     void* work = nullptr;
     size_t work_bytes = 0;
-    float2* spec = nullptr;  // nc * N * N * (N/2+1)
+    float2* spec = nullptr;  // nc * N * N * Hp
     float* e = nullptr;      // nc * N^3 (|Psi_a|^2 when computed here)
-    float2* tw = nullptr;    // exp(-2 pi i m / N), m < N (column passes)
+    float2* tw = nullptr;    // exp(-2 pi i m / N), m < N (direct path)
 };
 
 std::once_flag g_fft_once;
@@ -142,15 +147,28 @@ __global__ __launch_bounds__(256) void poisson_finish(float* __restrict__ chi, i
 }
 
 // ---------------------------------------------------------------------------
-// Column passes: 1-D FFTs of length N along the j or the i axis of the half
-// spectrum [a][i][j][kk] (kk < H = N/2+1 contiguous).  A workgroup loads
-// NCOL adjacent kk columns of one (a, i) or (a, j) slab (NCOL contiguous
-// complex values per row, rows `rs` apart) into LDS, runs a Stockham
-// autosort FFT per column (radix-8 stages, one radix-2/4 stage when log2 N is
-// not a multiple of 3; every thread owns 8 values per stage) and writes the
-// columns back.  The fused mode transforms along i, multiplies by the
-// spectral filter and transforms back, so the i axis is read and written
-// once for the forward transform, the filter and the inverse transform.
+// Transform passes of the direct path (method 2): a length-N complex FFT per
+// line, every thread holding 8 values of its line.
+//
+// Stockham autosort with radix-8 stages and one radix-2/4 stage when log2 N
+// is not a multiple of 3.  Thread t of the TC = N/8 threads of a line holds
+// the values at positions t + e*TC, e = 0..7.  With that assignment every
+// stage reads exactly those positions, and the last stage (Ns * R = N) writes
+// exactly those positions, so the first stage starts from the values the
+// thread loaded from HBM, the last stage's results go straight back to HBM
+// (or into the filter and the first inverse stage), and only the stage
+// boundaries in between pass through LDS.  Twiddles come from an N-entry
+// LDS table filled from a host-computed (double precision) table.
+//
+// Passes over the half spectrum [a][i][j][kk] (kk < H = N/2+1, row pitch
+// Hp = H rounded up to 16 complex values so every row starts on a 128-byte
+// line):
+//   rows   : real lines along k, two per complex FFT (z = x_2p + i x_2p+1),
+//            split / merged with Hermitian symmetry (forward: e -> spectrum,
+//            inverse: spectrum -> S_a);
+//   columns: NCOL adjacent kk columns of one (a, i) or (a, j) slab, lines
+//            along j or along i; the i pass transforms forward, multiplies
+//            by the filter and transforms back in one read and one write.
 // ---------------------------------------------------------------------------
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
@@ -200,64 +218,84 @@ __device__ __forceinline__ void dft(float2 (&x)[R], int sign) {
     }
 }
 
-// One Stockham stage of radix R on one column `col` (length N, LDS stride
-// CS between columns) by thread t of TC = N/8 column threads.
-template <int N, int R, int CS>
-__device__ __forceinline__ void stage(float2* lds, int col, int t, int Ns, int sign,
-                                      const float2* __restrict__ tw) {
-    constexpr int B = N / R;           // butterflies in the stage
-    constexpr int TC = N / 8;          // threads per column
-    constexpr int NB = B / TC;         // butterflies per thread (8 / R)
-    float2 x[NB][R];
-    float2* c = lds + col * CS;
+// One radix-R stage (span Ns) on the 8 register values of thread t.
+// Butterfly u (of NB = 8/R) takes positions j + r*(N/R), j = t + u*TC,
+// which are the register slots u + r*NB.
+template <int N, int R>
+__device__ __forceinline__ void stage_regs(float2 (&x)[8], int t, int Ns, int sign,
+                                           const float2* __restrict__ twl) {
+    constexpr int TC = N / 8;
+    constexpr int NB = 8 / R;
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-        const int j = t + u * TC;
-        const int k = j % Ns;
+        const int k = (t + u * TC) & (Ns - 1);
+        float2 v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            float2 v = c[j + r * B];
+            float2 a = x[u + r * NB];
             if (r > 0 && Ns > 1) {
-                float2 w = tw[(r * k * (N / (Ns * R))) & (N - 1)];
+                float2 w = twl[(r * k * (N / (Ns * R))) & (N - 1)];
                 if (sign > 0) w.y = -w.y;
-                v = cmul(v, w);
+                a = cmul(a, w);
             }
-            x[u][r] = v;
+            v[r] = a;
         }
-        dft<R>(x[u], sign);
+        dft<R>(v, sign);
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[u + r * NB] = v[r];
     }
+}
+
+// Stage boundary through LDS: results of the radix-R stage of span Ns go to
+// their Stockham positions (j/Ns)*Ns*R + j%Ns + r*Ns, then every thread
+// reads back positions t + e*TC.  `at(pos)` maps a line position to its LDS
+// slot.  The leading barrier protects the previous boundary's reads.
+template <int N, int R, class At>
+__device__ __forceinline__ void exchange(float2 (&x)[8], float2* lds, int t, int Ns, At at) {
+    constexpr int TC = N / 8;
+    constexpr int NB = 8 / R;
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
         const int j = t + u * TC;
-        const int k = j % Ns;
-        const int base = (j / Ns) * Ns * R + k;
+        const int k = j & (Ns - 1);
+        const int base = (j - k) * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) c[base + r * Ns] = x[u][r];
+        for (int r = 0; r < R; ++r) lds[at(base + r * Ns)] = x[u + r * NB];
     }
     __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) x[e] = lds[at(t + e * TC)];
 }
 
-template <int N, int CS>
-__device__ __forceinline__ void fft_col(float2* lds, int col, int t, int sign,
-                                        const float2* __restrict__ tw) {
+// Full length-N transform of the register line; result in the same slots.
+template <int N, class At>
+__device__ __forceinline__ void fft_line(float2 (&x)[8], float2* lds, int t, int sign,
+                                         const float2* __restrict__ twl, At at) {
     constexpr int L = ilog2(N);
     constexpr int n8 = L / 3;
     constexpr int rem = L % 3;
     int Ns = 1;
 #pragma unroll
-    for (int st = 0; st < n8; ++st) {
-        stage<N, 8, CS>(lds, col, t, Ns, sign, tw);
+    for (int s = 0; s < n8; ++s) {
+        stage_regs<N, 8>(x, t, Ns, sign, twl);
+        if (s + 1 < n8 || rem != 0) exchange<N, 8>(x, lds, t, Ns, at);
         Ns *= 8;
     }
-    if constexpr (rem == 1) stage<N, 2, CS>(lds, col, t, Ns, sign, tw);
-    if constexpr (rem == 2) stage<N, 4, CS>(lds, col, t, Ns, sign, tw);
+    if constexpr (rem == 1) stage_regs<N, 2>(x, t, Ns, sign, twl);
+    if constexpr (rem == 2) stage_regs<N, 4>(x, t, Ns, sign, twl);
+}
+
+template <int N>
+__device__ __forceinline__ void load_twiddles(float2* twl, const float2* __restrict__ tw) {
+    for (int m = threadIdx.x; m < N; m += blockDim.x) twl[m] = tw[m];
 }
 
 struct ColArgs {
     float2* spec;
     const float2* tw;
     int nc;
+    int Hp;                  // row pitch of the half spectrum along kk
     long long slab_stride;   // complex elements between consecutive slab indices x
     long long row_stride;    // complex elements between consecutive rows (FFT axis)
     int mode;                // 0 forward, 1 inverse, 2 forward + filter + inverse (i axis)
@@ -268,60 +306,147 @@ struct ColArgs {
 template <int N>
 constexpr int ncol_for() { return N >= 1024 ? 8 : 16; }
 
+// Workgroup = NCOL adjacent kk columns x TC threads per column; lane order
+// is column-fastest so each load/store instruction moves whole 128-byte rows
+// and LDS rows ([pos][col]) are read and written contiguously.
 template <int N>
 __global__ __launch_bounds__(ncol_for<N>() * (N / 8)) void fft_columns(ColArgs A) {
     constexpr int NCOL = ncol_for<N>();
     constexpr int TC = N / 8;
     constexpr int H = N / 2 + 1;
-    constexpr int CS = N + 1;                 // LDS column stride (odd: spreads banks)
-    __shared__ float2 lds[NCOL * CS];
+    __shared__ float2 lds[N * NCOL];
+    __shared__ float2 twl[N];
     constexpr int NCB = (H + NCOL - 1) / NCOL;
     const int cb = blockIdx.x % NCB;
     const int slab = blockIdx.x / NCB;         // a * N + x
     const int a = slab / N, x = slab % N;
     const int kk0 = cb * NCOL;
-    const int nv = min(NCOL, H - kk0);
-    float2* base = A.spec + (long long)a * N * N * H + (long long)x * A.slab_stride + kk0;
-    const int tid = threadIdx.x;
-    constexpr int NT = NCOL * TC;
+    const int col = threadIdx.x % NCOL, t = threadIdx.x / NCOL;
+    const bool live = kk0 + col < H;
+    float2* base = A.spec + (long long)a * N * N * A.Hp + (long long)x * A.slab_stride + kk0 + col;
+    load_twiddles<N>(twl, A.tw);
+    float2 v[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int e = tid + q * NT;
-        const int row = e / NCOL, col = e % NCOL;
-        lds[col * CS + row] = col < nv ? base[(long long)row * A.row_stride + col] : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    const int col = tid / TC, t = tid % TC;
-    const int sign = A.mode == 1 ? 1 : -1;
-    fft_col<N, CS>(lds, col, t, sign, A.tw);
+    for (int e = 0; e < 8; ++e)
+        v[e] = live ? base[(long long)(t + e * TC) * A.row_stride] : make_float2(0.f, 0.f);
+    auto at = [col](int pos) { return pos * NCOL + col; };
+    // the first stage needs no twiddles; the first boundary's barriers
+    // publish the table before any stage reads it
+    fft_line<N>(v, lds, t, A.mode == 1 ? 1 : -1, twl, at);
     if (A.mode == 2) {
-        // multiply by the filter at (k_i = row, k_j = x, k_kk = kk0 + col)
+        // filter at (k_i = position, k_j = x, k_kk = kk0 + col)
         const float two_pi_over_n = 6.283185307179586f / float(N);
         const int fj = x < (N + 1) / 2 ? x : x - N;
-        const float kx2 = (two_pi_over_n * fj) * (two_pi_over_n * fj);
+        const float ky = two_pi_over_n * float(fj);
         const float kz = two_pi_over_n * float(kk0 + col);
+        const float k2jk = ky * ky + kz * kz;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int i = t + u * TC;
+        for (int e = 0; e < 8; ++e) {
+            const int i = t + e * TC;
             const int fi = i < (N + 1) / 2 ? i : i - N;
-            const float ki = two_pi_over_n * float(fi);
-            const float k2 = ki * ki + kx2 + kz * kz;
+            const float kx = two_pi_over_n * float(fi);
+            const float k2 = kx * kx + k2jk;
             float h;
             if (A.filter == 0) h = A.gamma / (A.gamma + A.D * k2) * A.scale;
             else h = (k2 > 0.0f) ? -A.scale / k2 : 0.0f;
-            float2 v = lds[col * CS + i];
-            v.x *= h;
-            v.y *= h;
-            lds[col * CS + i] = v;
+            v[e].x *= h;
+            v[e].y *= h;
         }
-        __syncthreads();
-        fft_col<N, CS>(lds, col, t, 1, A.tw);
+        fft_line<N>(v, lds, t, 1, twl, at);
     }
+    if (live) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int e = tid + q * NT;
-        const int row = e / NCOL, cl = e % NCOL;
-        if (cl < nv) base[(long long)row * A.row_stride + cl] = lds[cl * CS + row];
+        for (int e = 0; e < 8; ++e) base[(long long)(t + e * TC) * A.row_stride] = v[e];
+    }
+}
+
+// Lines along the contiguous k axis.  Two real lines p0 = 2q, p1 = 2q+1 form
+// z = x_p0 + i x_p1; after Z = FFT(z):
+//   X_p0[k] = (Z[k] + conj Z[N-k]) / 2,  X_p1[k] = (Z[k] - conj Z[N-k]) / (2i).
+// The inverse builds Z[k] = X_p0[k] + i X_p1[k] over all k from the half
+// spectra (X[N-k] = conj X[k]; the imaginary parts of X[0] and X[N/2] are
+// dropped, as a C2R transform does) and takes x_p0 = Re z, x_p1 = Im z.
+// LDS slot of position pos: pos + pos/8, which spreads the radix-8
+// boundaries' stride-8 writes over all banks.
+template <int N>
+constexpr int rows_threads() { return 256; }
+template <int N>
+constexpr int rows_pairs() { return rows_threads<N>() / (N / 8); }
+template <int N>
+constexpr int rows_pitch() { return N + N / 8; }
+
+template <int N>
+__global__ __launch_bounds__(rows_threads<N>()) void fft_rows_r2c(const float* __restrict__ in,
+                                                                 float2* __restrict__ spec,
+                                                                 const float2* __restrict__ tw,
+                                                                 int Hp) {
+    constexpr int TC = N / 8;
+    constexpr int PB = rows_pairs<N>();
+    constexpr int PS = rows_pitch<N>();
+    __shared__ float2 lds[PB * PS];
+    __shared__ float2 twl[N];
+    const int q = threadIdx.x / TC, t = threadIdx.x % TC;
+    const long long pair = (long long)blockIdx.x * PB + q;
+    const float* r0 = in + pair * 2 * N;
+    const float* r1 = r0 + N;
+    load_twiddles<N>(twl, tw);
+    float2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = make_float2(r0[t + e * TC], r1[t + e * TC]);
+    float2* L = lds + q * PS;
+    auto at = [](int pos) { return pos + pos / 8; };
+    fft_line<N>(v, L, t, -1, twl, at);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) L[at(t + e * TC)] = v[e];
+    __syncthreads();
+    float2* o0 = spec + pair * 2 * Hp;
+    float2* o1 = o0 + Hp;
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+        const int k = t + e * TC;
+        if (e == 4 && t != 0) break;  // k = N/2 once
+        const float2 zk = L[at(k)];
+        const float2 zm = L[at((N - k) & (N - 1))];
+        o0[k] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        o1[k] = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(rows_threads<N>()) void fft_rows_c2r(const float2* __restrict__ spec,
+                                                                 float* __restrict__ out,
+                                                                 const float2* __restrict__ tw,
+                                                                 int Hp) {
+    constexpr int TC = N / 8;
+    constexpr int PB = rows_pairs<N>();
+    constexpr int PS = rows_pitch<N>();
+    __shared__ float2 lds[PB * PS];
+    __shared__ float2 twl[N];
+    const int q = threadIdx.x / TC, t = threadIdx.x % TC;
+    const long long pair = (long long)blockIdx.x * PB + q;
+    const float2* s0 = spec + pair * 2 * Hp;
+    const float2* s1 = s0 + Hp;
+    load_twiddles<N>(twl, tw);
+    float2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int k = t + e * TC;
+        const bool lo = k <= N / 2;
+        const int m = lo ? k : N - k;
+        float2 a = s0[m], b = s1[m];
+        if (m == 0 || m == N / 2) { a.y = 0.0f; b.y = 0.0f; }
+        if (!lo) { a.y = -a.y; b.y = -b.y; }
+        v[e] = make_float2(a.x - b.y, a.y + b.x);
+    }
+    float2* L = lds + q * PS;
+    fft_line<N>(v, L, t, 1, twl, [](int pos) { return pos + pos / 8; });
+    float* r0 = out + pair * 2 * N;
+    float* r1 = r0 + N;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        r0[t + e * TC] = v[e].x;
+        r1[t + e * TC] = v[e].y;
     }
 }
 
@@ -336,20 +461,47 @@ int launch_columns(const ColArgs& A, hipStream_t st) {
     return LFM_OK;
 }
 
+// dir 0: real lines -> half spectra; 1: half spectra -> real lines
+template <int N>
+int launch_rows(int dir, const float* in, float2* spec, float* out, const float2* tw, int Hp,
+                int nc, hipStream_t st) {
+    const long long pairs = (long long)nc * N * N / 2;
+    const long long nb = pairs / rows_pairs<N>();
+    if (dir == 0)
+        hipLaunchKernelGGL(fft_rows_r2c<N>, dim3(unsigned(nb)), dim3(rows_threads<N>()), 0, st, in,
+                           spec, tw, Hp);
+    else
+        hipLaunchKernelGGL(fft_rows_c2r<N>, dim3(unsigned(nb)), dim3(rows_threads<N>()), 0, st,
+                           spec, out, tw, Hp);
+    LFM_LAUNCH_CHECK(dir == 0 ? "fft_rows_r2c" : "fft_rows_c2r");
+    return LFM_OK;
+}
+
+#define LFM_FFT_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
+
 int columns(int N, const ColArgs& A, hipStream_t st) {
     switch (N) {
-        case 16: return launch_columns<16>(A, st);
-        case 32: return launch_columns<32>(A, st);
-        case 64: return launch_columns<64>(A, st);
-        case 128: return launch_columns<128>(A, st);
-        case 256: return launch_columns<256>(A, st);
-        case 512: return launch_columns<512>(A, st);
-        case 1024: return launch_columns<1024>(A, st);
+#define LFM_CASE(n) case n: return launch_columns<n>(A, st);
+        LFM_FFT_SIZES(LFM_CASE)
+#undef LFM_CASE
         default: set_error("column FFT: N=%d unsupported", N); return LFM_E_ARG;
     }
 }
 
+int rows(int N, int dir, const float* in, float2* spec, float* out, const float2* tw, int Hp,
+         int nc, hipStream_t st) {
+    switch (N) {
+#define LFM_CASE(n) case n: return launch_rows<n>(dir, in, spec, out, tw, Hp, nc, st);
+        LFM_FFT_SIZES(LFM_CASE)
+#undef LFM_CASE
+        default: set_error("row FFT: N=%d unsupported", N); return LFM_E_ARG;
+    }
+}
+
 bool columns_supported(int N) { return N >= 16 && N <= 1024 && (N & (N - 1)) == 0; }
+
+// half-spectrum row pitch of the direct path
+int spectrum_pitch(int N) { return ((N / 2 + 1) + 15) / 16 * 16; }
 
 __global__ __launch_bounds__(256) void clip_kernel(const float* __restrict__ x,
                                                    float* __restrict__ y, size_t n) {
@@ -380,38 +532,39 @@ void destroy(ScvPlan* p) {
 // Transform e -> spectrum -> filter -> out.  filter 0: Helmholtz; 1: Poisson.
 int run_filter(ScvPlan* p, const float* e, int filter, float gamma, float D, float* out,
                hipStream_t st, int ncomp_used) {
-    LFM_FFT_TRY(rocfft_execution_info_set_stream(p->info, st));
     const int N = p->N;
     const double n3 = double(N) * N * N;
-    void* in_f[1] = {const_cast<float*>(e)};
-    void* out_f[1] = {p->spec};
-    LFM_FFT_TRY(rocfft_execute(p->fwd, in_f, out_f, p->info));
     if (p->method == 2) {
-        const long long H = N / 2 + 1;
-        ColArgs A{p->spec, p->tw, ncomp_used, 0, 0, 0, filter, gamma, D, float(1.0 / n3)};
+        int rc = rows(N, 0, e, p->spec, nullptr, p->tw, p->Hp, ncomp_used, st);
+        if (rc) return rc;
+        ColArgs A{p->spec, p->tw, ncomp_used, p->Hp, 0, 0, 0, filter, gamma, D, float(1.0 / n3)};
         // forward along j: slab (a, i), rows j
-        A.slab_stride = (long long)N * H; A.row_stride = H; A.mode = 0;
-        int rc = columns(N, A, st);
+        A.slab_stride = (long long)N * p->Hp; A.row_stride = p->Hp; A.mode = 0;
+        rc = columns(N, A, st);
         if (rc) return rc;
         // along i: forward, filter, inverse in one pass; slab (a, j), rows i
-        A.slab_stride = H; A.row_stride = (long long)N * H; A.mode = 2;
+        A.slab_stride = p->Hp; A.row_stride = (long long)N * p->Hp; A.mode = 2;
         rc = columns(N, A, st);
         if (rc) return rc;
         // inverse along j
-        A.slab_stride = (long long)N * H; A.row_stride = H; A.mode = 1;
+        A.slab_stride = (long long)N * p->Hp; A.row_stride = p->Hp; A.mode = 1;
         rc = columns(N, A, st);
         if (rc) return rc;
+        return rows(N, 1, nullptr, p->spec, out, p->tw, p->Hp, ncomp_used, st);
+    }
+    LFM_FFT_TRY(rocfft_execution_info_set_stream(p->info, st));
+    void* in_f[1] = {const_cast<float*>(e)};
+    void* out_f[1] = {p->spec};
+    LFM_FFT_TRY(rocfft_execute(p->fwd, in_f, out_f, p->info));
+    const size_t ns = size_t(p->nc) * N * N * (N / 2 + 1);
+    if (filter == 0) {
+        hipLaunchKernelGGL(helmholtz_filter, dim3(grid_for(ns)), dim3(256), 0, st, p->spec, N,
+                           p->nc, gamma, D, float(1.0 / n3));
+        LFM_LAUNCH_CHECK("helmholtz_filter");
     } else {
-        const size_t ns = size_t(p->nc) * N * N * (N / 2 + 1);
-        if (filter == 0) {
-            hipLaunchKernelGGL(helmholtz_filter, dim3(grid_for(ns)), dim3(256), 0, st, p->spec, N,
-                               p->nc, gamma, D, float(1.0 / n3));
-            LFM_LAUNCH_CHECK("helmholtz_filter");
-        } else {
-            hipLaunchKernelGGL(poisson_filter, dim3(grid_for(ns)), dim3(256), 0, st, p->spec, N,
-                               float(1.0 / n3));
-            LFM_LAUNCH_CHECK("poisson_filter");
-        }
+        hipLaunchKernelGGL(poisson_filter, dim3(grid_for(ns)), dim3(256), 0, st, p->spec, N,
+                           float(1.0 / n3));
+        LFM_LAUNCH_CHECK("poisson_filter");
     }
     void* in_i[1] = {p->spec};
     void* out_i[1] = {out};
@@ -444,49 +597,38 @@ extern "C" int lfm_scv_plan_create_ex(int32_t N, int32_t ncolors, int32_t method
         set_error("lfm_scv_plan_create: method %d unknown", method);
         return LFM_E_ARG;
     }
-    std::call_once(g_fft_once, [] { g_fft_setup = rocfft_setup(); });
-    if (g_fft_setup != rocfft_status_success) {
-        set_error("rocfft_setup failed: status %d", (int)g_fft_setup);
-        return LFM_E_FFT;
-    }
     auto* p = new ScvPlan();
     p->N = N;
     p->nc = ncolors;
     p->method = method;
-    rocfft_status s;
+    p->Hp = method == 2 ? spectrum_pitch(N) : N / 2 + 1;
     if (method == 1) {
+        std::call_once(g_fft_once, [] { g_fft_setup = rocfft_setup(); });
+        rocfft_status s = g_fft_setup;
         // rocFFT lengths are fastest-first: (k, j, i) for the C-order (i, j, k) array.
         const size_t len[3] = {size_t(N), size_t(N), size_t(N)};
-        s = rocfft_plan_create(&p->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
-                               rocfft_precision_single, 3, len, size_t(ncolors), nullptr);
+        if (s == rocfft_status_success)
+            s = rocfft_plan_create(&p->fwd, rocfft_placement_notinplace,
+                                   rocfft_transform_type_real_forward, rocfft_precision_single, 3,
+                                   len, size_t(ncolors), nullptr);
         if (s == rocfft_status_success)
             s = rocfft_plan_create(&p->inv, rocfft_placement_notinplace,
                                    rocfft_transform_type_real_inverse, rocfft_precision_single, 3,
                                    len, size_t(ncolors), nullptr);
-    } else {
-        // 1-D real transforms along the contiguous k axis, batch ncolors * N * N
-        const size_t len[1] = {size_t(N)};
-        const size_t batch = size_t(ncolors) * N * N;
-        s = rocfft_plan_create(&p->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
-                               rocfft_precision_single, 1, len, batch, nullptr);
-        if (s == rocfft_status_success)
-            s = rocfft_plan_create(&p->inv, rocfft_placement_notinplace,
-                                   rocfft_transform_type_real_inverse, rocfft_precision_single, 1,
-                                   len, batch, nullptr);
+        size_t wf = 0, wi = 0;
+        if (s == rocfft_status_success) s = rocfft_plan_get_work_buffer_size(p->fwd, &wf);
+        if (s == rocfft_status_success) s = rocfft_plan_get_work_buffer_size(p->inv, &wi);
+        if (s == rocfft_status_success) s = rocfft_execution_info_create(&p->info);
+        if (s != rocfft_status_success) {
+            set_error("lfm_scv_plan_create: rocFFT plan setup failed (status %d)", (int)s);
+            destroy(p);
+            return LFM_E_FFT;
+        }
+        p->work_bytes = wf > wi ? wf : wi;
     }
-    size_t wf = 0, wi = 0;
-    if (s == rocfft_status_success) s = rocfft_plan_get_work_buffer_size(p->fwd, &wf);
-    if (s == rocfft_status_success) s = rocfft_plan_get_work_buffer_size(p->inv, &wi);
-    if (s == rocfft_status_success) s = rocfft_execution_info_create(&p->info);
-    if (s != rocfft_status_success) {
-        set_error("lfm_scv_plan_create: rocFFT plan setup failed (status %d)", (int)s);
-        destroy(p);
-        return LFM_E_FFT;
-    }
-    p->work_bytes = wf > wi ? wf : wi;
     hipError_t he = hipSuccess;
     if (p->work_bytes) he = hipMalloc(&p->work, p->work_bytes);
-    const size_t ns = size_t(ncolors) * N * N * (N / 2 + 1);
+    const size_t ns = size_t(ncolors) * N * N * p->Hp;
     if (he == hipSuccess) he = hipMalloc((void**)&p->spec, ns * sizeof(float2));
     if (he == hipSuccess) he = hipMalloc((void**)&p->e, size_t(ncolors) * N * N * N * sizeof(float));
     if (he == hipSuccess && method == 2) {
@@ -506,7 +648,7 @@ extern "C" int lfm_scv_plan_create_ex(int32_t N, int32_t ncolors, int32_t method
         return LFM_E_HIP;
     }
     if (p->work_bytes) {
-        s = rocfft_execution_info_set_work_buffer(p->info, p->work, p->work_bytes);
+        const rocfft_status s = rocfft_execution_info_set_work_buffer(p->info, p->work, p->work_bytes);
         if (s != rocfft_status_success) {
             set_error("rocfft_execution_info_set_work_buffer failed: status %d", (int)s);
             destroy(p);

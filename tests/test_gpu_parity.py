@@ -464,3 +464,25 @@ def test_sa_transform_methods_agree_with_oracle(gpu, N):
         err = O.rel_l2(chi.cpu().numpy() - 19.0, ref_chi.ravel() - 19.0)
         log(f"  poisson method {method}: rel-L2 {err:.2e}")
         assert err < 1e-5
+
+
+@pytest.mark.parametrize("N", [256, 512])
+def test_sa_direct_path_matches_rocfft_at_large_n(gpu, N):
+    """At the production sizes the direct passes agree with rocFFT 3-D plans
+    (no FP64 oracle at this size: the small-N test pins both to it)."""
+    import torch
+    from lfm_hip.backend import ScvPlan
+    log(f"S_a direct vs rocFFT N={N}")
+    g = torch.Generator(device="cuda").manual_seed(N)
+    e = torch.rand(3 * N ** 3, device="cuda", generator=g) * 2.0
+    outs = []
+    for method in (1, 2):
+        plan = ScvPlan(N, 3, method)
+        sa = torch.empty_like(e)
+        plan.apply_psisq(e, 0.1, 4.9, sa)
+        torch.cuda.synchronize()
+        outs.append(sa.double())
+        del plan
+    err = float((outs[0] - outs[1]).norm() / outs[0].norm())
+    log(f"  rel-L2 {err:.2e}")
+    assert err < 2e-6
