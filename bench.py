@@ -267,6 +267,8 @@ def run_adjoint(args):
     res = {}
     for name, stride in (("keep_all", 1), ("sqrtT", math.ceil(math.sqrt(T)))):
         for rep in range(2):    # first pass warms the allocator; the second is timed
+            if rep:
+                del pr_t, pi_t, k, chiT, loss
             pr_t, pi_t = t(pr).requires_grad_(True), t(pi).requires_grad_(True)
             k = torch.tensor(cfg.kappa, dtype=torch.float64, device=dev, requires_grad=True)
             torch.cuda.synchronize()

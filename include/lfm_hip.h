@@ -147,27 +147,34 @@ int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_prev, const 
                        const float* mask, float* pr_out, float* pi_out, float* chi_out,
                        int32_t flags, void* stream);
 
-/* Reverse-mode (adjoint) step of the COMPLEX leapfrog, for gradients of a
+/* Reverse-mode (adjoint) sweep of the COMPLEX leapfrog, for gradients of a
  * loss of the final state.  It has no counterpart in the reference (which
  * has no gradients); it transposes the forward step of lfm_step_complex_f32
  * with the reference's buffer semantics (numpy_backend.py:132-205,
- * evolver.py:182-281).  Given the forward state of step n (pr, pi, chi,
- * chi_prev in the step kernels' chi storage, flags LFM_F_MASK /
- * LFM_F_CHI_DELTA as used forward) and the adjoints U = dL/dPsi, X = dL/dchi
- * of step n+1, it writes Ut = (1-m) U and Xt = (1-m) 1[no clamp] X of step
- * n+1 and the adjoints Un, Xn of step n; Ut2 / Xt2 are the Ut / Xt written
- * by the previous call (step n+2), NULL for the last step.  The per-workgroup
- * terms of dL/dkappa of this step land in partials (lfm_adjoint_partials_len(N)
- * doubles); when kappa_grad (device double) is not NULL their sum is also
- * added to *kappa_grad. */
+ * evolver.py:182-281).  chi is in deviation storage (LFM_F_CHI_DELTA is
+ * required; LFM_F_MASK as used forward).
+ *
+ * The sweep carries the damped adjoints Ut = (1-m) dL/dPsi and
+ * Xt = (1-m) 1[no clamp] dL/dchi of the two following steps.
+ * lfm_adjoint_complex_seed_f32 turns the adjoints U, X of the final state
+ * (whose stored chi is `chi`) into its Ut, Xt.  lfm_adjoint_complex_step_f32
+ * takes the forward state of step n (pr, pi, chi), Ut / Xt of step n+1 and
+ * Ut2 / Xt2 of step n+2 (all three NULL for the last step) and writes step
+ * n's damped Ut, Xt (damp_out != 0) or its raw dL/dPsi_n, dL/dchi_n
+ * (damp_out == 0, for n = 0); outputs must not alias Ut / Xt.  The
+ * per-workgroup terms of dL/dkappa of this step land in partials
+ * (lfm_adjoint_partials_len(N) doubles); when kappa_grad (device double) is
+ * not NULL their sum is also added to *kappa_grad. */
 int64_t lfm_adjoint_partials_len(int32_t N);
-int lfm_adjoint_complex_step_f32(const float* pr, const float* pi, const float* chi,
-                                 const float* chi_prev, const float* mask, const float* U_r,
+int lfm_adjoint_complex_seed_f32(const float* chi, const float* mask, const float* U_r,
                                  const float* U_i, const float* X, float* Ut_r, float* Ut_i,
-                                 float* Xt, const float* Ut2_r, const float* Ut2_i,
-                                 const float* Xt2, float* Un_r, float* Un_i, float* Xn,
-                                 double* partials, double* kappa_grad, const lfm_params_t* p,
-                                 void* stream);
+                                 float* Xt, const lfm_params_t* p, void* stream);
+int lfm_adjoint_complex_step_f32(const float* pr, const float* pi, const float* chi,
+                                 const float* mask, const float* Ut_r, const float* Ut_i,
+                                 const float* Xt, const float* Ut2_r, const float* Ut2_i,
+                                 const float* Xt2, float* out_r, float* out_i, float* out_x,
+                                 int32_t damp_out, double* partials, double* kappa_grad,
+                                 const lfm_params_t* p, void* stream);
 
 /* FP64 reductions (lfm/analysis/energy.py:26-135, metrics.py:29-111,
  * structure.py:26-171).  `ncomp` Psi components of N^3 each (1 REAL,

@@ -56,6 +56,9 @@ class ComplexTape:
         self.absorb = 1.0 - self.mask
         self.chi_off = float(np.float32(config.chi0))
         self.kappa = float(config.kappa)
+        self._pcache = {}
+        self._mask_ptr = self.mask.data_ptr() if self.has_mask else None
+        self._stream = None
 
     # ------------------------------------------------------------ forward
     def _flags(self, damped: bool) -> int:
@@ -70,22 +73,26 @@ class ComplexTape:
                       flags=flags)
 
     def _fwd(self, cr, ci, pr_, pi_, c, cp, damped, out_r, out_i, out_c):
-        p = self._p(self._flags(damped))
-        rc = self._lib.lfm_step_complex_f32(
-            cr.data_ptr(), pr_.data_ptr(), ci.data_ptr(), pi_.data_ptr(), c.data_ptr(),
-            cp.data_ptr(), self.mask.data_ptr() if self.has_mask else None, out_r.data_ptr(),
-            None, out_i.data_ptr(), None, out_c.data_ptr(), None, ctypes.byref(p),
-            stream_handle())
-        _abi.check(rc, "lfm_step_complex_f32")
+        """One forward step; every buffer argument is a device address (int)."""
+        key = (bool(damped), self.kappa)
+        p = self._pcache.get(key)
+        if p is None:
+            p = self._pcache[key] = ctypes.byref(self._p(self._flags(damped)))
+        rc = self._lib.lfm_step_complex_f32(cr, pr_, ci, pi_, c, cp, self._mask_ptr, out_r, None,
+                                            out_i, None, out_c, None, p, self._stream)
+        if rc:
+            _abi.check(rc, "lfm_step_complex_f32")
 
     def forward(self, pr0: torch.Tensor, pi0: torch.Tensor, kappa: float, chi_init: torch.Tensor):
         """Run T steps from Psi_0 (prev tied) and chi_init; return chi_T."""
         self.kappa = float(kappa)
+        self._stream = stream_handle()
         n3 = self.N ** 3
         f32 = dict(dtype=torch.float32, device=self.device)
         pr0 = pr0.detach().to(**f32).reshape(-1)
         pi0 = pi0.detach().to(**f32).reshape(-1)
         chi_d = chi_init.detach().to(**f32).reshape(-1) - self.chi_off
+        row = 4 * n3
         if self.K == 1:
             # every state kept: one allocation, each step writes into its slot
             R = torch.empty((self.T + 1, n3), **f32)
@@ -94,10 +101,14 @@ class ComplexTape:
             R[0].copy_(pr0)
             I[0].copy_(pi0)
             C[0].copy_(chi_d)
+            r0, i0, c0 = R.data_ptr(), I.data_ptr(), C.data_ptr()
             for n in range(self.T):
                 m = n - 1 if n else 0
-                self._fwd(R[n], I[n], R[m], I[m], C[n], C[m], n > 0, R[n + 1], I[n + 1], C[n + 1])
+                self._fwd(r0 + n * row, i0 + n * row, r0 + m * row, i0 + m * row, c0 + n * row,
+                          c0 + m * row, n > 0, r0 + (n + 1) * row, i0 + (n + 1) * row,
+                          c0 + (n + 1) * row)
             self.tape = (R, I, C)
+            self.chiT = C[self.T]
             return C[self.T] + self.chi_off
         # three-slot rotation; checkpoint (cur, prev, chi, chi_prev) every K steps
         nck = (self.T + self.K - 1) // self.K
@@ -107,94 +118,104 @@ class ComplexTape:
         S[0, 0].copy_(pr0); S[1, 0].copy_(pr0)
         S[0, 1].copy_(pi0); S[1, 1].copy_(pi0)
         S[0, 2].copy_(chi_d); S[1, 2].copy_(chi_d)
+        flat = S.view(9, n3)
+        sp = lambda slot, f: S.data_ptr() + (3 * slot + f) * row
+        gathers = {}
         cur, prev, cc, cp = 0, 1, 0, 1
         damped = False
         for n in range(self.T):
             if n % self.K == 0:
-                c = self.ck[n // self.K]
-                c[0].copy_(S[cur, 0]); c[1].copy_(S[cur, 1])
-                c[2].copy_(S[prev, 0]); c[3].copy_(S[prev, 1])
-                c[4].copy_(S[cc, 2]); c[5].copy_(S[cp, 2])
+                key = (cur, prev, cc, cp)
+                if key not in gathers:
+                    gathers[key] = torch.tensor([3 * cur, 3 * cur + 1, 3 * prev, 3 * prev + 1,
+                                                 3 * cc + 2, 3 * cp + 2], device=self.device)
+                torch.index_select(flat, 0, gathers[key], out=self.ck[n // self.K])
                 self.ck_damped[n // self.K] = damped
             nxt = 3 - cur - prev
             cn = 3 - cc - cp
-            self._fwd(S[cur, 0], S[cur, 1], S[prev, 0], S[prev, 1], S[cc, 2], S[cp, 2], damped,
-                      S[nxt, 0], S[nxt, 1], S[cn, 2])
+            self._fwd(sp(cur, 0), sp(cur, 1), sp(prev, 0), sp(prev, 1), sp(cc, 2), sp(cp, 2),
+                      damped, sp(nxt, 0), sp(nxt, 1), sp(cn, 2))
             prev, cur, cp, cc = cur, nxt, cc, cn
             damped = True
         self.seg = torch.empty((self.K + 1, 3, n3), **f32)
+        self.chiT = S[cc, 2]
         return S[cc, 2] + self.chi_off
 
     def _segment(self, s: int, e: int):
-        """States s..e-1 as (pr, pi, chi, chi_prev): from the tape, or
+        """Device addresses (pr, pi, chi) of states s..e-1: from the tape, or
         recomputed from checkpoint s into the segment buffer."""
+        row = 4 * self.N ** 3
         if self.K == 1:
             R, I, C = self.tape
-            return [(R[n], I[n], C[n], C[n - 1] if n else C[0]) for n in range(s, e)]
+            r0, i0, c0 = R.data_ptr(), I.data_ptr(), C.data_ptr()
+            return [(r0 + n * row, i0 + n * row, c0 + n * row) for n in range(s, e)]
         c = self.ck[s // self.K]
         B = self.seg
-        B[0, 0].copy_(c[0]); B[0, 1].copy_(c[1]); B[0, 2].copy_(c[4])
-        states = [(B[0, 0], B[0, 1], B[0, 2], c[5])]
-        pr_, pi_, damped = c[2], c[3], self.ck_damped[s // self.K]
+        B[0].copy_(c[[0, 1, 4]])
+        b0, k0 = B.data_ptr(), c.data_ptr()
+        bp = lambda t, f: b0 + (3 * t + f) * row
+        states = [(bp(0, 0), bp(0, 1), bp(0, 2))]
+        pr_, pi_, cp = k0 + 2 * row, k0 + 3 * row, k0 + 5 * row
+        damped = self.ck_damped[s // self.K]
         for t in range(e - s - 1):
-            cr, ci, ch, cp = states[-1]
-            self._fwd(cr, ci, pr_, pi_, ch, cp, damped, B[t + 1, 0], B[t + 1, 1], B[t + 1, 2])
-            pr_, pi_ = cr, ci
+            cr, ci, ch = states[-1]
+            self._fwd(cr, ci, pr_, pi_, ch, cp, damped, bp(t + 1, 0), bp(t + 1, 1), bp(t + 1, 2))
+            pr_, pi_, cp = cr, ci, ch
             damped = True
-            states.append((B[t + 1, 0], B[t + 1, 1], B[t + 1, 2], ch))
+            states.append((bp(t + 1, 0), bp(t + 1, 1), bp(t + 1, 2)))
         return states
 
     # ----------------------------------------------------------- backward
     def backward(self, g_chiT: torch.Tensor, g_prT=None, g_piT=None):
         """Adjoint sweep; returns (dL/dpr0, dL/dpi0, dL/dkappa, dL/dchi_init)."""
         n3 = self.N ** 3
+        row = 4 * n3
         f32 = dict(dtype=torch.float32, device=self.device)
-        A = torch.empty((11, n3), **f32)
-        Ur, Ui, X, Unr, Uni, Xn = A[0], A[1], A[2], A[3], A[4], A[5]
-        Ut = [(A[6], A[7], A[8]), (A[9], A[10], None)]
-        Xt1 = torch.empty(n3, **f32)
-        Ut[1] = (A[9], A[10], Xt1)
-        X.copy_(g_chiT.detach().to(**f32).reshape(-1))
+        # three (Ut_r, Ut_i, Xt) slots: step n+1 (stencil input), n+2, output
+        A = torch.empty((3, 3, n3), **f32)
+        G = torch.zeros((3, n3), **f32)          # raw adjoints of the final state
+        G[2].copy_(g_chiT.detach().to(**f32).reshape(-1))
         if g_prT is not None:
-            Ur.copy_(g_prT.detach().to(**f32).reshape(-1))
-        else:
-            Ur.zero_()
+            G[0].copy_(g_prT.detach().to(**f32).reshape(-1))
         if g_piT is not None:
-            Ui.copy_(g_piT.detach().to(**f32).reshape(-1))
-        else:
-            Ui.zero_()
+            G[1].copy_(g_piT.detach().to(**f32).reshape(-1))
         nb = int(self._lib.lfm_adjoint_partials_len(self.N))
         part = torch.empty((self.T, nb), dtype=torch.float64, device=self.device)
-        p = self._p(self._flags(False) & ~_abi.LFM_F_DAMP_PREV)
-        mask = self.mask.data_ptr() if self.has_mask else None
-        st = stream_handle()
-        slot, have2 = 0, False
+        p = ctypes.byref(self._p(self._flags(False) & ~_abi.LFM_F_DAMP_PREV))
+        mask = self._mask_ptr
+        st = self._stream = stream_handle()
+        ap = lambda slot, f: A.data_ptr() + (3 * slot + f) * row
+        gp = lambda f: G.data_ptr() + f * row
+        rc = self._lib.lfm_adjoint_complex_seed_f32(
+            self.chiT.data_ptr(), mask, gp(0), gp(1), gp(2), ap(0, 0), ap(0, 1), ap(0, 2), p, st)
+        _abi.check(rc, "lfm_adjoint_complex_seed_f32")
+        step = self._lib.lfm_adjoint_complex_step_f32
+        part0 = part.data_ptr()
+        c1, c2, c3, have2 = 0, 1, 2, False
         for s in reversed(range(0, self.T, self.K)):
             e = min(s + self.K, self.T)
             states = self._segment(s, e)
             for n in range(e - 1, s - 1, -1):
-                cr, ci, c, cp = states[n - s]
-                u1 = Ut[slot]
-                u2 = Ut[1 - slot]
-                rc = self._lib.lfm_adjoint_complex_step_f32(
-                    cr.data_ptr(), ci.data_ptr(), c.data_ptr(), cp.data_ptr(), mask,
-                    Ur.data_ptr(), Ui.data_ptr(), X.data_ptr(), u1[0].data_ptr(),
-                    u1[1].data_ptr(), u1[2].data_ptr(), u2[0].data_ptr() if have2 else None,
-                    u2[1].data_ptr() if have2 else None, u2[2].data_ptr() if have2 else None,
-                    Unr.data_ptr(), Uni.data_ptr(), Xn.data_ptr(), part[n].data_ptr(), None,
-                    ctypes.byref(p), st)
-                _abi.check(rc, "lfm_adjoint_complex_step_f32")
-                Ur, Unr = Unr, Ur
-                Ui, Uni = Uni, Ui
-                X, Xn = Xn, X
-                slot, have2 = 1 - slot, True
-        # the last call wrote Ut, Xt of step 1 into Ut[1 - slot]
-        u1 = Ut[1 - slot]
-        g_pr0 = Ur - u1[0]        # dL/dcur_0 + dL/dprev_0, prev_0 = Psi_0
-        g_pi0 = Ui - u1[1]
-        g_chi = X - u1[2]         # dL/dchi_0 + dL/dchi_prev_0
+                cr, ci, c = states[n - s]
+                rc = step(cr, ci, c, mask, ap(c1, 0), ap(c1, 1), ap(c1, 2),
+                          ap(c2, 0) if have2 else None, ap(c2, 1) if have2 else None,
+                          ap(c2, 2) if have2 else None, ap(c3, 0), ap(c3, 1), ap(c3, 2),
+                          1 if n > 0 else 0, part0 + 8 * nb * n, None, p, st)
+                if rc:
+                    _abi.check(rc, "lfm_adjoint_complex_step_f32")
+                c1, c2, c3, have2 = c3, c1, c2, True
+        # the last call wrote the raw U_0, X_0 into A[c1]; A[c2] holds Ut_1, Xt_1
+        U0, T1 = A[c1], A[c2]
+        g_pr0 = U0[0] - T1[0]     # dL/dcur_0 + dL/dprev_0, prev_0 = Psi_0
+        g_pi0 = U0[1] - T1[1]
+        g_chi = U0[2] - T1[2]     # dL/dchi_0 + dL/dchi_prev_0
         g_k = float(part.sum().item())
+        self.release()
         return g_pr0, g_pi0, g_k, g_chi
+
+    def release(self):
+        """Drop the tape (every kept state, checkpoints, segment buffer)."""
+        self.tape = self.ck = self.seg = self.chiT = None
 
 
 class _ChiRun(torch.autograd.Function):

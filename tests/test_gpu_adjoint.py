@@ -67,6 +67,45 @@ def test_adjoint_matches_fp64_autograd(gpu, lam):
     assert e_r <= 2e-3 and e_i <= 2e-3 and e_k <= 2e-3
 
 
+def test_adjoint_with_active_clamp(gpu):
+    """A chi region sitting just above the floor -chi0 with a negative
+    source (E0^2 = 10) is clamped on every step: the adjoint must cut the
+    chi-gradient there exactly as autograd through torch.where does."""
+    import torch
+    from lfm_hip.adjoint import evolve_chi
+    N, T = 32, 20
+    log(f"adjoint with active chi clamp N={N} T={T}")
+    import lfm_hip as L
+    cfg = L.SimulationConfig(grid_size=N, field_level=L.FieldLevel.COMPLEX, lambda_self=4 / 31,
+                             e0_sq=10.0)
+    _, pr, pi = _setup(N, 4 / 31)
+    chi = np.full((N,) * 3, 19.0, np.float32)
+    # -19 + 2^-16: exact in float32 absolute and deviation (chi - 19) storage
+    chi[6:11, 14:19, 14:19] = np.float32(-19.0 + 2.0 ** -16)
+    dev = torch.device("cuda")
+    t = lambda a: torch.tensor(a, device=dev)
+    with torch.no_grad():
+        chiT0 = evolve_chi(t(pr), t(pi), cfg.kappa, cfg, T, chi_init=t(chi)).cpu().numpy()
+    clamped = int((chiT0 == -19.0).sum())
+    log(f"  sites at the floor after {T} steps: {clamped}")
+    assert clamped >= 8
+    pr_t, pi_t = t(pr).requires_grad_(True), t(pi).requires_grad_(True)
+    k = torch.tensor(cfg.kappa, dtype=torch.float64, device=dev, requires_grad=True)
+    chiT = evolve_chi(pr_t, pi_t, k, cfg, T, chi_init=t(chi))
+    rng = np.random.default_rng(3)
+    cot64 = rng.standard_normal((N,) * 3)
+    chiT.backward(torch.tensor(cot64, dtype=torch.float32, device=dev))
+    mask = O.boundary_mask(N, cfg.boundary_fraction).reshape((N,) * 3).astype(np.float64)
+    cot64 = cot64.astype(np.float32).astype(np.float64)
+    gr, gi, gk = AR.vjp(pr, pi, chi.astype(np.float64), cfg.kappa, mask, T, cot64,
+                        lam=4 / 31, e0_sq=10.0)
+    e_r = O.rel_l2(pr_t.grad.cpu().numpy(), gr)
+    e_i = O.rel_l2(pi_t.grad.cpu().numpy(), gi)
+    e_k = abs(float(k.grad) - gk) / abs(gk)
+    log(f"  grad rel-L2 pr {e_r:.2e} pi {e_i:.2e} kappa {e_k:.2e}")
+    assert e_r <= 2e-3 and e_i <= 2e-3 and e_k <= 2e-3
+
+
 def test_checkpoint_stride_is_exact(gpu):
     import torch
     from lfm_hip.adjoint import evolve_chi
