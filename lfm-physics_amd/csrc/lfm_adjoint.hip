@@ -87,8 +87,19 @@ __global__ __launch_bounds__(AB) void adj_seed(const float* __restrict__ chi,
 __device__ __forceinline__ float ldb(const float* p, uint32_t b) {
     return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + b);
 }
+// NT: streaming (non-temporal) form for data touched once, used when the
+// step's working set is far beyond the last-level cache.
+template <bool NT>
+__device__ __forceinline__ float ldbp(const float* p, uint32_t b) {
+    const float* q = reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + b);
+    if constexpr (NT) return __builtin_nontemporal_load(q);
+    else return *q;
+}
+template <bool NT>
 __device__ __forceinline__ void stb(float* p, uint32_t b, float v) {
-    *reinterpret_cast<float*>(reinterpret_cast<char*>(p) + b) = v;
+    float* q = reinterpret_cast<float*>(reinterpret_cast<char*>(p) + b);
+    if constexpr (NT) __builtin_nontemporal_store(v, q);
+    else *q = v;
 }
 
 // 2.5-D form: each plane q of the chunk (plus one halo plane on each side) is
@@ -96,6 +107,7 @@ __device__ __forceinline__ void stb(float* p, uint32_t b, float v) {
 // that site); the 19-point Laplacians split as L(i) = A(i) + B(i-1) + B(i+1)
 // and the face sums split into in-plane parts and the +-i differences, so a
 // site i is finished when plane i+1 has been staged.
+template <bool NT>
 __global__ __launch_bounds__(AB) void adj_fused(AdjArgs A) {
     __shared__ float lds[NLF * LPL];
     const int N = A.N;
@@ -169,8 +181,8 @@ __global__ __launch_bounds__(AB) void adj_fused(AdjArgs A) {
         if (s >= 2 && s < nplanes && live) {
             const uint32_t ob = uint32_t(im) * NN4 + cb;
             v[0] = ldb(A.chi, ob);
-            v[1] = A.mask ? ldb(A.mask, ob) : 0.0f;
-            if (A.U2r) { v[2] = ldb(A.U2r, ob); v[3] = ldb(A.U2i, ob); v[4] = ldb(A.X2, ob); }
+            v[1] = A.mask ? ldbp<NT>(A.mask, ob) : 0.0f;
+            if (A.U2r) { v[2] = ldbp<NT>(A.U2r, ob); v[3] = ldbp<NT>(A.U2i, ob); v[4] = ldbp<NT>(A.X2, ob); }
         }
     };
     load_plane(sa, 0);
@@ -250,13 +262,13 @@ __global__ __launch_bounds__(AB) void adj_fused(AdjArgs A) {
                              2.0f * A.dt2 * chi * (cpr * cur + cpi * cui) - y2;
             kp += double(cxt) * (double(chi) * double(S));
             if (A.damp_out) {
-                stb(A.Or, ob, a * ur);
-                stb(A.Oi, ob, a * ui);
-                stb(A.Ox, ob, a * clamp_free(cs, a, A.chi0) * xn);
+                stb<NT>(A.Or, ob, a * ur);
+                stb<NT>(A.Oi, ob, a * ui);
+                stb<NT>(A.Ox, ob, a * clamp_free(cs, a, A.chi0) * xn);
             } else {
-                stb(A.Or, ob, ur);
-                stb(A.Oi, ob, ui);
-                stb(A.Ox, ob, xn);
+                stb<NT>(A.Or, ob, ur);
+                stb<NT>(A.Oi, ob, ui);
+                stb<NT>(A.Ox, ob, xn);
             }
         }
         // plane s's centre values belong to the site finished next iteration
@@ -290,7 +302,7 @@ int adj_capacity() {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, adj_fused, AB, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, adj_fused<true>, AB, 0) != hipSuccess ||
         cus <= 0 || per <= 0) {
         (void)hipGetLastError();
         return 1024;
@@ -387,7 +399,11 @@ extern "C" int lfm_adjoint_complex_step_f32(const float* pr, const float* pi, co
               adj_chunk(p->N), p->dt2, p->kappa, p->lambda_self, p->chi0, p->e0_sq, p->epsilon_w};
     hipStream_t st = (hipStream_t)stream;
     const unsigned nb = adj_blocks(p->N);
-    hipLaunchKernelGGL(adj_fused, dim3(nb), dim3(AB), 0, st, A);
+    // 13 float fields per site; streaming once far beyond the 256 MB last-level cache
+    if (size_t(p->N) * p->N * p->N * 52 > (size_t(512) << 20))
+        hipLaunchKernelGGL(adj_fused<true>, dim3(nb), dim3(AB), 0, st, A);
+    else
+        hipLaunchKernelGGL(adj_fused<false>, dim3(nb), dim3(AB), 0, st, A);
     LFM_LAUNCH_CHECK("adj_fused");
     if (kappa_grad) {
         hipLaunchKernelGGL(sum_into, dim3(1), dim3(256), 0, st, (const double*)partials,

@@ -182,6 +182,40 @@ __device__ __forceinline__ float2 rot(float2 a, int sign) {
     return sign < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
 }
 
+// Every pass touches each value once.  For N >= 512 (spectra of 1 GB and
+// more) the HBM accesses are streaming (non-temporal); for smaller N the
+// plain accesses let the 256 MB last-level cache hold the spectrum between
+// passes.
+typedef float f2v __attribute__((ext_vector_type(2)));
+template <int N>
+__device__ __forceinline__ float2 ld2nt(const float2* p) {
+    if constexpr (N >= 512) {
+        const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+        return make_float2(v.x, v.y);
+    } else {
+        return *p;
+    }
+}
+template <int N>
+__device__ __forceinline__ void st2nt(float2* p, float2 v) {
+    if constexpr (N >= 512) {
+        const f2v w = {v.x, v.y};
+        __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+    } else {
+        *p = v;
+    }
+}
+template <int N>
+__device__ __forceinline__ float ld1nt(const float* p) {
+    if constexpr (N >= 512) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <int N>
+__device__ __forceinline__ void st1nt(float* p, float v) {
+    if constexpr (N >= 512) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 template <int R>
 __device__ __forceinline__ void dft(float2 (&x)[R], int sign) {
     if constexpr (R == 2) {
@@ -328,7 +362,7 @@ __global__ __launch_bounds__(ncol_for<N>() * (N / 8)) void fft_columns(ColArgs A
     float2 v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e)
-        v[e] = live ? base[(long long)(t + e * TC) * A.row_stride] : make_float2(0.f, 0.f);
+        v[e] = live ? ld2nt<N>(base + (long long)(t + e * TC) * A.row_stride) : make_float2(0.f, 0.f);
     auto at = [col](int pos) { return pos * NCOL + col; };
     // the first stage needs no twiddles; the first boundary's barriers
     // publish the table before any stage reads it
@@ -356,7 +390,7 @@ __global__ __launch_bounds__(ncol_for<N>() * (N / 8)) void fft_columns(ColArgs A
     }
     if (live) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) base[(long long)(t + e * TC) * A.row_stride] = v[e];
+        for (int e = 0; e < 8; ++e) st2nt<N>(base + (long long)(t + e * TC) * A.row_stride, v[e]);
     }
 }
 
@@ -392,7 +426,8 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_r2c(const float* _
     load_twiddles<N>(twl, tw);
     float2 v[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = make_float2(r0[t + e * TC], r1[t + e * TC]);
+    for (int e = 0; e < 8; ++e)
+        v[e] = make_float2(ld1nt<N>(r0 + t + e * TC), ld1nt<N>(r1 + t + e * TC));
     float2* L = lds + q * PS;
     auto at = [](int pos) { return pos + pos / 8; };
     fft_line<N>(v, L, t, -1, twl, at);
@@ -408,8 +443,8 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_r2c(const float* _
         if (e == 4 && t != 0) break;  // k = N/2 once
         const float2 zk = L[at(k)];
         const float2 zm = L[at((N - k) & (N - 1))];
-        o0[k] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-        o1[k] = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
+        st2nt<N>(o0 + k, make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y)));
+        st2nt<N>(o1 + k, make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x)));
     }
 }
 
@@ -445,8 +480,8 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_c2r(const float2* 
     float* r1 = r0 + N;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        r0[t + e * TC] = v[e].x;
-        r1[t + e * TC] = v[e].y;
+        st1nt<N>(r0 + t + e * TC, v[e].x);
+        st1nt<N>(r1 + t + e * TC, v[e].y);
     }
 }
 

@@ -87,18 +87,34 @@ __device__ __forceinline__ float* field_out(float* pr, float* pi, int f, size_t 
 // have the same layout), so each access is one uniform 64-bit base held in
 // SGPRs plus one 32-bit lane offset (the saddr form of global loads); it
 // covers N <= 1023.  Off64 is a plain element index for any N.
-struct Off32 {
+// With NT the pointwise inputs (ldp) and every output (st) are streaming
+// (non-temporal) accesses: on lattices far larger than the last-level cache
+// they stop data touched once per step from pushing the stencil fields' halo
+// lines, which neighbouring tiles re-read, out of L2.  On lattices whose
+// state fits in the last-level cache the plain accesses keep that reuse
+// across steps, so the launcher picks NT by working-set size.
+template <bool NT>
+struct Off32T {
     uint32_t b;
-    __device__ __forceinline__ float ld(const float* p) const {
-        return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + b);
+    __device__ __forceinline__ const float* at(const float* p) const {
+        return reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + b);
+    }
+    __device__ __forceinline__ float ld(const float* p) const { return *at(p); }
+    __device__ __forceinline__ float ldp(const float* p) const {
+        if constexpr (NT) return __builtin_nontemporal_load(at(p));
+        else return *at(p);
     }
     __device__ __forceinline__ void st(float* p, float v) const {
-        *reinterpret_cast<float*>(reinterpret_cast<char*>(p) + b) = v;
+        float* q = reinterpret_cast<float*>(reinterpret_cast<char*>(p) + b);
+        if constexpr (NT) __builtin_nontemporal_store(v, q);
+        else *q = v;
     }
 };
+using Off32 = Off32T<false>;
 struct Off64 {
     size_t i;
     __device__ __forceinline__ float ld(const float* p) const { return p[i]; }
+    __device__ __forceinline__ float ldp(const float* p) const { return p[i]; }
     __device__ __forceinline__ void st(float* p, float v) const { p[i] = v; }
 };
 
@@ -118,13 +134,13 @@ __device__ __forceinline__ PointData<L> load_point(const KParams& kp, const Ptrs
     constexpr int NPF = Lv<L>::NPF;
     PointData<L> d;
 #pragma unroll
-    for (int f = 0; f < NPF; ++f) d.prev[f] = idx.ld(field<L>(P.prp, P.pip, f, P.n3));
-    d.m = (kp.flags & LFM_F_MASK) ? idx.ld(P.mask) : 0.0f;
-    d.chip = (kp.flags & LFM_F_FREEZE_CHI) ? 0.0f : idx.ld(P.chip);
+    for (int f = 0; f < NPF; ++f) d.prev[f] = idx.ldp(field<L>(P.prp, P.pip, f, P.n3));
+    d.m = (kp.flags & LFM_F_MASK) ? idx.ldp(P.mask) : 0.0f;
+    d.chip = (kp.flags & LFM_F_FREEZE_CHI) ? 0.0f : idx.ldp(P.chip);
     if constexpr (L == COLOR) {
         if (P.sa) {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) d.sa[q] = idx.ld(P.sa + q * P.n3);
+            for (int q = 0; q < 3; ++q) d.sa[q] = idx.ldp(P.sa + q * P.n3);
         }
     }
     return d;
@@ -316,7 +332,7 @@ constexpr int TK = 64;
 
 // One workgroup's work: the column tile at (k0, j0) over planes [i0, i1) of
 // one lattice (P, kp).  Shared by the single-lattice and batched kernels.
-template <int L, int BJ, int RJ>
+template <int L, int BJ, int RJ, bool NT>
 __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int k0, int j0,
                                           int i0, int i1) {
     constexpr int NPF = Lv<L>::NPF;
@@ -430,7 +446,7 @@ __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int 
             // site q is finalised in the next iteration: fetch its pointwise data now
             const uint32_t qb = uint32_t(q) * planeB;
 #pragma unroll
-            for (int rr = 0; rr < RJ; ++rr) ptn[rr] = load_point<L>(kp, P, Off32{qb + loff[rr]});
+            for (int rr = 0; rr < RJ; ++rr) ptn[rr] = load_point<L>(kp, P, Off32T<NT>{qb + loff[rr]});
         }
         const uint32_t pend_pb = uint32_t((q - 1 + N) % N) * planeB;
 #pragma unroll
@@ -485,7 +501,7 @@ __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int 
                         if (kp.ks > 0.0f) ccv = ccvyz[rr] + (qx - (1.0f / 3.0f) * sx * sx);
                     }
                 }
-                site_update<L>(kp, P, Off32{pend_pb + ioff0 + rr * rowB}, ptc[rr], cpend, lap,
+                site_update<L>(kp, P, Off32T<NT>{pend_pb + ioff0 + rr * rowB}, ptc[rr], cpend, lap,
                                chim1[rr], lap_chi, jsum, ccv);
             }
             // Advance the queue.
@@ -504,9 +520,8 @@ __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int 
     }
 }
 
-template <int L, int BJ, int RJ, int WPE>
-__global__ __launch_bounds__(64 * BJ) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
-void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
+template <int L, int BJ, int RJ, bool NT>
+__global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
     // XCD-aware block remap: blocks are dispatched round-robin over the 8
     // XCDs; this makes consecutive tiles of one i chunk share an XCD (and its
     // L2) so the halo rows neighbouring tiles both read hit in L2.
@@ -518,7 +533,7 @@ void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
     const int jt = rest % njt;
     const int ch = rest / njt;
     const int i0 = ch * CH;
-    tile_body<L, BJ, RJ>(P, kp, kt * TK, jt * BJ * RJ, i0, min(i0 + CH, kp.N));
+    tile_body<L, BJ, RJ, NT>(P, kp, kt * TK, jt * BJ * RJ, i0, min(i0 + CH, kp.N));
 }
 
 // Batched sweep: one launch advances many independent lattices.  Each
@@ -536,7 +551,7 @@ struct BatchArenas {
     float* pr_out; float* pi_out; float* chi_out;
 };
 
-template <int L, int BJ, int RJ>
+template <int L, int BJ, int RJ, bool NT>
 __global__ __launch_bounds__(64 * BJ) void step_tiled_batch(BatchArenas A,
                                                              const BatchLattice* __restrict__ lat,
                                                              const int4* __restrict__ desc,
@@ -556,7 +571,7 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled_batch(BatchArenas A,
     P.mask = (kp.flags & LFM_F_MASK) ? A.mask + o : nullptr;
     P.pr_out = A.pr_out + oc; P.chi_out = A.chi_out + o;
     P.n3 = n3;
-    tile_body<L, BJ, RJ>(P, kp, d.y, d.z, d.w, min(d.w + Lb.CH, Lb.N));
+    tile_body<L, BJ, RJ, NT>(P, kp, d.y, d.z, d.w, min(d.w + Lb.CH, Lb.N));
 }
 
 int choose_chunk(int N, int tiles) {
@@ -569,7 +584,7 @@ int choose_chunk(int N, int tiles) {
     return ch < N ? ch : N;
 }
 
-template <int L, int BJ, int RJ, int WPE>
+template <int L, int BJ, int RJ, bool NT>
 int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
     constexpr int TJ = BJ * RJ;
     const int N = kp.N;
@@ -578,10 +593,18 @@ int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
     const int CH = kp.chunk > 0 ? min(kp.chunk, N) : choose_chunk(N, nkt * njt);
     const int nch = (N + CH - 1) / CH;
     const long long nblocks = (long long)nkt * njt * nch;
-    hipLaunchKernelGGL((step_tiled<L, BJ, RJ, WPE>), dim3((unsigned)nblocks), dim3(64 * BJ), 0, st, P,
+    hipLaunchKernelGGL((step_tiled<L, BJ, RJ, NT>), dim3((unsigned)nblocks), dim3(64 * BJ), 0, st, P,
                        kp, nkt, njt, CH);
     LFM_LAUNCH_CHECK("step_tiled");
     return LFM_OK;
+}
+
+// Streaming accesses pay off once one step's traffic (algorithmic bytes per
+// site x sites) is well beyond the 256 MB last-level cache.
+template <int L>
+bool streaming(size_t sites) {
+    constexpr size_t bytes = L == REAL ? 20 : L == COMPLEX ? 40 : 112;
+    return sites * bytes > (size_t(512) << 20);
 }
 
 template <int L>
@@ -593,18 +616,22 @@ int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
         return LFM_OK;
     }
     // Tile shapes (wavefronts BJ x rows per thread RJ); 0 picks the level's default.
-    int tile = kp.tile > 0 ? kp.tile : 1;
-    switch (tile) {
-        case 1: return launch_tiled<L, 4, 1, 1>(P, kp, st);
-        case 2: return launch_tiled<L, 4, 2, 1>(P, kp, st);
-        case 3: return launch_tiled<L, 4, 4, 1>(P, kp, st);
-        case 4: return launch_tiled<L, 8, 1, 1>(P, kp, st);
-        case 5: return launch_tiled<L, 8, 2, 1>(P, kp, st);
-        case 6: return launch_tiled<L, 16, 1, 1>(P, kp, st);
-        default:
-            set_error("tile variant %d unknown (1..%d)", tile, LFM_TILE_VARIANTS);
-            return LFM_E_ARG;
+    const int tile = kp.tile > 0 ? kp.tile : 1;
+    const bool nt = streaming<L>(P.n3);
+#define LFM_TILES(NT)                                                  \
+    switch (tile) {                                                    \
+        case 1: return launch_tiled<L, 4, 1, NT>(P, kp, st);           \
+        case 2: return launch_tiled<L, 4, 2, NT>(P, kp, st);           \
+        case 3: return launch_tiled<L, 4, 4, NT>(P, kp, st);           \
+        case 4: return launch_tiled<L, 8, 1, NT>(P, kp, st);           \
+        case 5: return launch_tiled<L, 8, 2, NT>(P, kp, st);           \
+        case 6: return launch_tiled<L, 16, 1, NT>(P, kp, st);          \
+        default:                                                       \
+            set_error("tile variant %d unknown (1..%d)", tile, LFM_TILE_VARIANTS); \
+            return LFM_E_ARG;                                          \
     }
+    if (nt) { LFM_TILES(true) } else { LFM_TILES(false) }
+#undef LFM_TILES
 }
 
 constexpr int tile_tj(int tile) {
@@ -613,17 +640,20 @@ constexpr int tile_tj(int tile) {
 
 template <int L>
 int launch_batch(const BatchArenas& A, const BatchLattice* lat, const int4* desc, int ndesc,
-                 int tile, int flags_or, hipStream_t st) {
+                 int tile, int flags_or, bool nt, hipStream_t st) {
     const dim3 g(ndesc);
-    switch (tile) {
-        case 1: hipLaunchKernelGGL((step_tiled_batch<L, 4, 1>), g, dim3(256), 0, st, A, lat, desc, flags_or); break;
-        case 2: hipLaunchKernelGGL((step_tiled_batch<L, 4, 2>), g, dim3(256), 0, st, A, lat, desc, flags_or); break;
-        case 3: hipLaunchKernelGGL((step_tiled_batch<L, 4, 4>), g, dim3(256), 0, st, A, lat, desc, flags_or); break;
-        case 4: hipLaunchKernelGGL((step_tiled_batch<L, 8, 1>), g, dim3(512), 0, st, A, lat, desc, flags_or); break;
-        case 5: hipLaunchKernelGGL((step_tiled_batch<L, 8, 2>), g, dim3(512), 0, st, A, lat, desc, flags_or); break;
-        case 6: hipLaunchKernelGGL((step_tiled_batch<L, 16, 1>), g, dim3(1024), 0, st, A, lat, desc, flags_or); break;
-        default: set_error("tile variant %d unknown", tile); return LFM_E_ARG;
+#define LFM_BTILES(NT)                                                                                  \
+    switch (tile) {                                                                                     \
+        case 1: hipLaunchKernelGGL((step_tiled_batch<L, 4, 1, NT>), g, dim3(256), 0, st, A, lat, desc, flags_or); break; \
+        case 2: hipLaunchKernelGGL((step_tiled_batch<L, 4, 2, NT>), g, dim3(256), 0, st, A, lat, desc, flags_or); break; \
+        case 3: hipLaunchKernelGGL((step_tiled_batch<L, 4, 4, NT>), g, dim3(256), 0, st, A, lat, desc, flags_or); break; \
+        case 4: hipLaunchKernelGGL((step_tiled_batch<L, 8, 1, NT>), g, dim3(512), 0, st, A, lat, desc, flags_or); break; \
+        case 5: hipLaunchKernelGGL((step_tiled_batch<L, 8, 2, NT>), g, dim3(512), 0, st, A, lat, desc, flags_or); break; \
+        case 6: hipLaunchKernelGGL((step_tiled_batch<L, 16, 1, NT>), g, dim3(1024), 0, st, A, lat, desc, flags_or); break; \
+        default: set_error("tile variant %d unknown", tile); return LFM_E_ARG;                         \
     }
+    if (nt) { LFM_BTILES(true) } else { LFM_BTILES(false) }
+#undef LFM_BTILES
     LFM_LAUNCH_CHECK("step_tiled_batch");
     return LFM_OK;
 }
@@ -898,8 +928,11 @@ extern "C" int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_p
     const int fo = flags & (LFM_F_DAMP_PREV | LFM_F_FREEZE_CHI | LFM_F_CHI_DELTA);
     hipStream_t st = (hipStream_t)stream;
     switch (P->level) {
-        case 0: return launch_batch<REAL>(A, P->d_lat, P->d_desc, P->ndesc, P->tile, fo, st);
-        case 1: return launch_batch<COMPLEX>(A, P->d_lat, P->d_desc, P->ndesc, P->tile, fo, st);
-        default: return launch_batch<COLOR>(A, P->d_lat, P->d_desc, P->ndesc, P->tile, fo, st);
+        case 0: return launch_batch<REAL>(A, P->d_lat, P->d_desc, P->ndesc, P->tile, fo,
+                                          streaming<REAL>(size_t(P->total)), st);
+        case 1: return launch_batch<COMPLEX>(A, P->d_lat, P->d_desc, P->ndesc, P->tile, fo,
+                                             streaming<COMPLEX>(size_t(P->total)), st);
+        default: return launch_batch<COLOR>(A, P->d_lat, P->d_desc, P->ndesc, P->tile, fo,
+                                            streaming<COLOR>(size_t(P->total)), st);
     }
 }
