@@ -53,7 +53,6 @@ class ComplexTape:
         self.has_mask = not (mask_policy == "boundary_type"
                              and config.boundary_type == BoundaryType.PERIODIC)
         self.mask = self.backend.create_boundary_mask(self.N, config.boundary_fraction)
-        self.absorb = 1.0 - self.mask
         self.chi_off = float(np.float32(config.chi0))
         self.kappa = float(config.kappa)
         self._pcache = {}

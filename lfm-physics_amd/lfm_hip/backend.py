@@ -86,6 +86,10 @@ class ScvPlan:
             self.handle = None
 
 
+# device masks by (device, N, boundary_fraction), shared by every backend
+_MASKS: dict[tuple, torch.Tensor] = {}
+
+
 class HipBackend:
     """GPU compute backend on hand-written gfx950 kernels (see module doc)."""
 
@@ -111,7 +115,14 @@ class HipBackend:
                 "psi_B": z(), "psi_prev_B": z(), "chi_B": c(), "chi_prev_B": c()}
 
     def create_boundary_mask(self, N: int, boundary_fraction: float) -> torch.Tensor:
-        return self.from_numpy(boundary_mask_host(N, boundary_fraction))
+        """The reference's cos^2 mask on the device.  Built on the host once
+        per (N, boundary_fraction) and kept; each call returns a fresh copy,
+        so callers may modify theirs."""
+        key = (str(self.device), int(N), float(boundary_fraction))
+        m = _MASKS.get(key)
+        if m is None:
+            m = _MASKS[key] = self.from_numpy(boundary_mask_host(N, boundary_fraction))
+        return m.clone()
 
     def to_numpy(self, arr) -> np.ndarray:
         return arr.detach().to("cpu").numpy()
