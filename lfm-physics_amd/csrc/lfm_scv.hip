@@ -335,6 +335,7 @@ struct ColArgs {
     int mode;                // 0 forward, 1 inverse, 2 forward + filter + inverse (i axis)
     int filter;              // 0 Helmholtz gamma/(gamma + D k^2), 1 Poisson -1/k^2
     float gamma, D, scale;
+    int rounds;              // column groups of NCOL one workgroup takes in turn
 };
 
 template <int N>
@@ -342,7 +343,11 @@ constexpr int ncol_for() { return N >= 1024 ? 8 : 16; }
 
 // Workgroup = NCOL adjacent kk columns x TC threads per column; lane order
 // is column-fastest so each load/store instruction moves whole 128-byte rows
-// and LDS rows ([pos][col]) are read and written contiguously.
+// and LDS rows ([pos][col]) are read and written contiguously.  A workgroup
+// takes `rounds` consecutive column groups of one slab in turn (the next
+// group's loads are issued before the current group is transformed), so
+// the N rows of a slab, which for the i axis lie 8 N Hp bytes apart, are
+// opened once per `rounds` groups.
 template <int N>
 __global__ __launch_bounds__(ncol_for<N>() * (N / 8)) void fft_columns(ColArgs A) {
     constexpr int NCOL = ncol_for<N>();
@@ -350,47 +355,61 @@ __global__ __launch_bounds__(ncol_for<N>() * (N / 8)) void fft_columns(ColArgs A
     constexpr int H = N / 2 + 1;
     __shared__ float2 lds[N * NCOL];
     __shared__ float2 twl[N];
-    constexpr int NCB = (H + NCOL - 1) / NCOL;
-    const int cb = blockIdx.x % NCB;
-    const int slab = blockIdx.x / NCB;         // a * N + x
+    constexpr int NCB = (H + NCOL - 1) / NCOL;      // column groups per slab
+    const int nbs = (NCB + A.rounds - 1) / A.rounds;  // workgroups per slab
+    const int cb0 = (blockIdx.x % nbs) * A.rounds;
+    const int slab = blockIdx.x / nbs;         // a * N + x
     const int a = slab / N, x = slab % N;
-    const int kk0 = cb * NCOL;
     const int col = threadIdx.x % NCOL, t = threadIdx.x / NCOL;
-    const bool live = kk0 + col < H;
-    float2* base = A.spec + (long long)a * N * N * A.Hp + (long long)x * A.slab_stride + kk0 + col;
+    float2* sbase = A.spec + (long long)a * N * N * A.Hp + (long long)x * A.slab_stride + col;
     load_twiddles<N>(twl, A.tw);
-    float2 v[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e)
-        v[e] = live ? ld2nt<N>(base + (long long)(t + e * TC) * A.row_stride) : make_float2(0.f, 0.f);
     auto at = [col](int pos) { return pos * NCOL + col; };
-    // the first stage needs no twiddles; the first boundary's barriers
-    // publish the table before any stage reads it
-    fft_line<N>(v, lds, t, A.mode == 1 ? 1 : -1, twl, at);
-    if (A.mode == 2) {
-        // filter at (k_i = position, k_j = x, k_kk = kk0 + col)
-        const float two_pi_over_n = 6.283185307179586f / float(N);
-        const int fj = x < (N + 1) / 2 ? x : x - N;
-        const float ky = two_pi_over_n * float(fj);
-        const float kz = two_pi_over_n * float(kk0 + col);
-        const float k2jk = ky * ky + kz * kz;
+    const float two_pi_over_n = 6.283185307179586f / float(N);
+    const int fj = x < (N + 1) / 2 ? x : x - N;
+    const float ky = two_pi_over_n * float(fj);
+    float2 v[8], vn[8];
+    auto load = [&](float2 (&dst)[8], int cb) {
+        const bool live = cb * NCOL + col < H;
+        const float2* b = sbase + cb * NCOL;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int i = t + e * TC;
-            const int fi = i < (N + 1) / 2 ? i : i - N;
-            const float kx = two_pi_over_n * float(fi);
-            const float k2 = kx * kx + k2jk;
-            float h;
-            if (A.filter == 0) h = A.gamma / (A.gamma + A.D * k2) * A.scale;
-            else h = (k2 > 0.0f) ? -A.scale / k2 : 0.0f;
-            v[e].x *= h;
-            v[e].y *= h;
+        for (int e = 0; e < 8; ++e)
+            dst[e] = live ? ld2nt<N>(b + (long long)(t + e * TC) * A.row_stride) : make_float2(0.f, 0.f);
+    };
+    load(v, cb0);
+    for (int r = 0; r < A.rounds; ++r) {
+        const int cb = cb0 + r;
+        const bool more = r + 1 < A.rounds && cb + 1 < NCB;
+        if (more) load(vn, cb + 1);
+        const int kk = cb * NCOL + col;
+        // the first stage needs no twiddles; the first boundary's barriers
+        // publish the table before any stage reads it
+        fft_line<N>(v, lds, t, A.mode == 1 ? 1 : -1, twl, at);
+        if (A.mode == 2) {
+            // filter at (k_i = position, k_j = x, k_kk = kk)
+            const float kz = two_pi_over_n * float(kk);
+            const float k2jk = ky * ky + kz * kz;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int i = t + e * TC;
+                const int fi = i < (N + 1) / 2 ? i : i - N;
+                const float kx = two_pi_over_n * float(fi);
+                const float k2 = kx * kx + k2jk;
+                float h;
+                if (A.filter == 0) h = A.gamma / (A.gamma + A.D * k2) * A.scale;
+                else h = (k2 > 0.0f) ? -A.scale / k2 : 0.0f;
+                v[e].x *= h;
+                v[e].y *= h;
+            }
+            fft_line<N>(v, lds, t, 1, twl, at);
         }
-        fft_line<N>(v, lds, t, 1, twl, at);
-    }
-    if (live) {
+        if (kk < H) {
+            float2* b = sbase + cb * NCOL;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) st2nt<N>(base + (long long)(t + e * TC) * A.row_stride, v[e]);
+            for (int e = 0; e < 8; ++e) st2nt<N>(b + (long long)(t + e * TC) * A.row_stride, v[e]);
+        }
+        if (!more) break;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = vn[e];
     }
 }
 
@@ -490,7 +509,7 @@ int launch_columns(const ColArgs& A, hipStream_t st) {
     constexpr int NCOL = ncol_for<N>();
     constexpr int H = N / 2 + 1;
     constexpr int NCB = (H + NCOL - 1) / NCOL;
-    const long long nb = (long long)A.nc * N * NCB;
+    const long long nb = (long long)A.nc * N * ((NCB + A.rounds - 1) / A.rounds);
     hipLaunchKernelGGL(fft_columns<N>, dim3(unsigned(nb)), dim3(NCOL * (N / 8)), 0, st, A);
     LFM_LAUNCH_CHECK("fft_columns");
     return LFM_OK;
@@ -564,6 +583,9 @@ void destroy(ScvPlan* p) {
     delete p;
 }
 
+// Column groups per workgroup for the j passes and the i pass.
+constexpr int kRoundsJ = 2, kRoundsI = 4;
+
 // Transform e -> spectrum -> filter -> out.  filter 0: Helmholtz; 1: Poisson.
 int run_filter(ScvPlan* p, const float* e, int filter, float gamma, float D, float* out,
                hipStream_t st, int ncomp_used) {
@@ -572,17 +594,20 @@ int run_filter(ScvPlan* p, const float* e, int filter, float gamma, float D, flo
     if (p->method == 2) {
         int rc = rows(N, 0, e, p->spec, nullptr, p->tw, p->Hp, ncomp_used, st);
         if (rc) return rc;
-        ColArgs A{p->spec, p->tw, ncomp_used, p->Hp, 0, 0, 0, filter, gamma, D, float(1.0 / n3)};
+        ColArgs A{p->spec, p->tw, ncomp_used, p->Hp, 0, 0, 0, filter, gamma, D, float(1.0 / n3), 1};
         // forward along j: slab (a, i), rows j
         A.slab_stride = (long long)N * p->Hp; A.row_stride = p->Hp; A.mode = 0;
+        A.rounds = kRoundsJ;
         rc = columns(N, A, st);
         if (rc) return rc;
         // along i: forward, filter, inverse in one pass; slab (a, j), rows i
         A.slab_stride = p->Hp; A.row_stride = (long long)N * p->Hp; A.mode = 2;
+        A.rounds = kRoundsI;
         rc = columns(N, A, st);
         if (rc) return rc;
         // inverse along j
         A.slab_stride = (long long)N * p->Hp; A.row_stride = p->Hp; A.mode = 1;
+        A.rounds = kRoundsJ;
         rc = columns(N, A, st);
         if (rc) return rc;
         return rows(N, 1, nullptr, p->spec, out, p->tw, p->Hp, ncomp_used, st);
