@@ -42,7 +42,8 @@ extern "C" {
 #define LFM_F_MASK 0x01       /* mask != NULL: cos^2 absorbing blend (m in [0,1]) */
 #define LFM_F_DAMP_PREV 0x02  /* Psi prev buffers hold undamped values: the kernel
                                  multiplies them by (1-m) when it reads them
-                                 (three-buffer rotation; see DESIGN) */
+                                 (three-buffer rotation: the old current buffer
+                                 becomes prev without being rewritten) */
 #define LFM_F_FREEZE_CHI 0x04 /* GOV-01 only: chi is read, never written */
 #define LFM_F_DIRECT 0x08     /* force the untiled one-site-per-thread kernel */
 #define LFM_F_CHI_DELTA 0x10  /* every chi buffer (in and out) holds chi - chi0: the
@@ -99,13 +100,15 @@ int lfm_step_color_f32(const float* pr, const float* pr_prev, const float* pi,
 
 /* S_a Helmholtz transform (numpy_backend.py:338-358, cupy_backend.py:260-279):
  * sa_out[a] = irfftn(gamma/(gamma + D k^2) * rfftn(e[a])), k_d = 2 pi fftfreq(N),
- * for a < ncolors, written unclipped.  A plan owns rocFFT plans and the
- * spectrum/work buffers for one (N, ncolors); it is not thread-safe. */
+ * for a < ncolors, written unclipped.  A plan owns the transform set-up and
+ * the spectrum buffers for one (N, ncolors); it is not thread-safe. */
 int lfm_scv_plan_create(int32_t N, int32_t ncolors, void** plan);
 /* method 0 = automatic; 1 = rocFFT 3-D R2C/C2R plans + a k-space filter
- * kernel; 2 = rocFFT 1-D R2C/C2R along k plus hand-written column FFT passes
- * along j and i, the i pass fusing forward transform, filter and inverse
- * transform (power-of-two N in [16, 1024]; automatic picks it there). */
+ * kernel (any N); 2 = five passes written here: real lines along k to half
+ * spectra, complex lines along j, one pass along i fusing the forward
+ * transform, the filter and the inverse transform, lines along j, half
+ * spectra back to real lines (power-of-two N in [16, 1024]; automatic picks
+ * it there). */
 int lfm_scv_plan_create_ex(int32_t N, int32_t ncolors, int32_t method, void** plan);
 int32_t lfm_scv_plan_method(void* plan);
 int lfm_scv_plan_destroy(void* plan);
