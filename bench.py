@@ -41,13 +41,43 @@ HBM_PEAK_GBS = 8000.0
 #   COMPLEX, no mask: reads Pr, Pi, Pr_p, Pi_p, chi, chi_p; writes Pr', Pi', chi' -> 36 B
 #   REAL, mask: reads E, E_p, chi, chi_p, mask; writes E', chi'         -> 28 B
 WORKLOADS = {
-    "color512": dict(level=2, N=512, preset="full_physics", mask=True, b_alg=112,
+    "color512": dict(level=2, N=512, preset="full_physics", mask=True, b_alg=112, streams=(18, 10),
                      desc="COLOR full_physics 512^3 fp32, mask on, S_a FFT every step"),
-    "complex512": dict(level=1, N=512, preset="gravity_em", mask=False, b_alg=36,
+    "complex512": dict(level=1, N=512, preset="gravity_em", mask=False, b_alg=36, streams=(6, 3),
                        desc="COMPLEX gravity_em 512^3 fp32, periodic (mask-free kernel)"),
-    "real64": dict(level=0, N=64, preset="gravity_only", mask=True, b_alg=28,
+    "real64": dict(level=0, N=64, preset="gravity_only", mask=True, b_alg=28, streams=(5, 2),
                    desc="REAL gravity_only 64^3 fp32, mask on"),
 }
+
+
+def stream_ceiling(nin, nout, n, reps=10):
+    """HBM rate of a pure streaming kernel (lfm_calib_stream_f32) with the step
+    kernel's count of planar float32 read and write streams, n floats each:
+    what that access pattern reaches on this device with no stencil, halo or
+    arithmetic in the way.  Returns GB/s (best of plain and streaming stores)."""
+    import ctypes
+    import torch
+    from lfm_hip import _abi
+    lib = _abi.load()
+    bufs = [torch.zeros(n, device="cuda") for _ in range(nin + nout)]
+    ins = (ctypes.c_void_p * nin)(*[b.data_ptr() for b in bufs[:nin]])
+    outs = (ctypes.c_void_p * nout)(*[b.data_ptr() for b in bufs[nin:]])
+    st = torch.cuda.current_stream().cuda_stream
+    best = 0.0
+    for nt in (0, 1):
+        for _ in range(2):
+            _abi.check(lib.lfm_calib_stream_f32(ins, nin, outs, nout, n, nt, st), "lfm_calib_stream_f32")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            lib.lfm_calib_stream_f32(ins, nin, outs, nout, n, nt, st)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        best = max(best, 4.0 * (nin + nout) * n / (ms * 1e-3) / 1e9)
+    del bufs
+    torch.cuda.empty_cache()
+    return best
 
 
 def make_config(w, N):
@@ -311,6 +341,8 @@ def main():
     ap.add_argument("--direct", action="store_true", help="use the untiled kernel")
     ap.add_argument("--tile", default="auto", help="tile shape 1..6, 0 default, or auto (tune)")
     ap.add_argument("--chunk", type=int, default=0, help="i planes per workgroup (0 auto)")
+    ap.add_argument("--sa-transform", default="auto", choices=["auto", "fused", "separate"],
+                    help="COLOR S_a: k-axis passes fused into the step kernel, or five separate passes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the leapfrog path here runs on a single MI355X "
@@ -349,7 +381,8 @@ def main():
     N = args.size or w["N"]
     cfg = make_config(w, N)
     policy = "reference" if w["mask"] else "boundary_type"
-    ev = L.Evolver(cfg, backend="hip", mask_policy=policy)
+    ev = L.Evolver(cfg, backend="hip", mask_policy=policy,
+                   sa_transform=args.sa_transform if w["level"] == 2 else "auto")
     pr, pi = gpu_ics(w, N)
     ev.set_psi_real(pr)
     if pi is not None:
@@ -387,8 +420,12 @@ def main():
     achieved = w["b_alg"] * n3 / (kms * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    kname = (f"step_color_kfft<{N}>" if getattr(ev, "_fused", False)
+             else f"step_tiled<{['REAL', 'COMPLEX', 'COLOR'][w['level']]}>")
     if os.path.exists(tfile) and args.size is None and not args.direct:
-        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        rec = json.load(open(tfile))
+        if rec.get("kernel") == kname:
+            traffic = rec.get("hbm_bytes_per_launch")
     out = {
         "metric": METRIC, "value": n3 / (ms * 1e-3) / 1e9, "unit": "GLUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -398,7 +435,7 @@ def main():
                    "preset": w["preset"], "mask": w["mask"], "kernel": "direct" if args.direct else "tiled"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"step_tiled<{['REAL', 'COMPLEX', 'COLOR'][w['level']]}>",
+                     "kernel": kname,
                      "kernel_ms": kms, "bytes_per_site": w["b_alg"]},
         "finite": ok,
         "tile": {"shape": ev.tile, "chunk": ev.chunk,
@@ -413,6 +450,11 @@ def main():
         except Exception:
             pass
         out["cpu_baseline"] = cb
+    if args.size is None and not args.direct:
+        # the same access pattern with nothing but streaming: the achievable rate
+        ceil = stream_ceiling(*w["streams"], n3)
+        out["roofline"]["stream_ceiling"] = ceil
+        out["roofline"]["frac_of_stream_ceiling"] = achieved / ceil
     print(json.dumps(out), flush=True)
 
 

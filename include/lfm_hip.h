@@ -98,6 +98,29 @@ int lfm_step_color_f32(const float* pr, const float* pr_prev, const float* pi,
                        const float* sa, float* sa_clip_out, float* psisq_out,
                        const lfm_params_t* p, void* stream);
 
+/* COLOR step with S_a (kappa_tube > 0) and the k-axis passes of the S_a
+ * transform fused into the step kernel: the same update as
+ * lfm_step_color_f32 (numpy_backend.py:207-382, cupy_backend.py:224-317,
+ * S_a as numpy_backend.py:338-358) in rotation mode (no prev outputs).
+ * `scv_plan` is a direct-method plan (method 2, 3 colours, the step's N,
+ * N in {64, 128, 256, 512}).  On entry its spectrum must hold the k-axis
+ * spectra of S_a for (pr, pi): left by lfm_scv_prepare_f32, or by
+ * lfm_scv_columns_f32 after the previous fused step.  On return it holds the
+ * k-axis half spectra of |Psi_out|^2; lfm_scv_columns_f32 turns them into the
+ * next step's input.  A sequence is: prepare once, then per step
+ * lfm_step_color_fused_f32 followed by lfm_scv_columns_f32. */
+int lfm_step_color_fused_f32(const float* pr, const float* pr_prev, const float* pi,
+                             const float* pi_prev, const float* chi, const float* chi_prev,
+                             const float* mask, float* pr_out, float* pi_out, float* chi_out,
+                             void* scv_plan, const lfm_params_t* p, void* stream);
+/* Direct-method plan only.  prepare: spectrum := k-axis spectra of S_a for
+ * (pr, pi) (|Psi_a|^2, the k pass and the three column passes).  columns:
+ * the three column passes (j forward; i forward, Helmholtz filter with the
+ * 1/N^3 normalisation, inverse; j inverse) in place. */
+int lfm_scv_prepare_f32(void* plan, const float* pr, const float* pi, float gamma, float D,
+                        void* stream);
+int lfm_scv_columns_f32(void* plan, float gamma, float D, void* stream);
+
 /* S_a Helmholtz transform (numpy_backend.py:338-358, cupy_backend.py:260-279):
  * sa_out[a] = irfftn(gamma/(gamma + D k^2) * rfftn(e[a])), k_d = 2 pi fftfreq(N),
  * for a < ncolors, written unclipped.  A plan owns the transform set-up and
@@ -198,6 +221,14 @@ int lfm_energy_f64(const float* pr, const float* pr_prev, const float* pi,
 int lfm_chi_stats_f64(const float* chi, double chi_offset, int32_t N, double boundary_fraction,
                       double well_threshold, double void_threshold, double* partials,
                       double* out, void* stream);
+
+/* Bandwidth calibration (no counterpart in the reference; a measurement aid
+ * for benchmarks): out[w][t] = (w+1) * sum_r in[r][t] for t < n, nin read and
+ * nout write streams of n floats (nin <= 32, 1 <= nout <= 32), streaming
+ * (non-temporal) accesses when `streaming` != 0.  The host arrays `in` and
+ * `out` hold device pointers. */
+int lfm_calib_stream_f32(const float* const* in, int32_t nin, float* const* out, int32_t nout,
+                         int64_t n, int32_t streaming, void* stream);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,7 @@
 #include <mutex>
 
 #include "lfm_internal.h"
+#include "lfm_fft.h"
 
 namespace lfm {
 namespace {
@@ -170,161 +171,6 @@ __global__ __launch_bounds__(256) void poisson_finish(float* __restrict__ chi, i
 //            along j or along i; the i pass transforms forward, multiplies
 //            by the filter and transforms back in one read and one write.
 // ---------------------------------------------------------------------------
-constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
-
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-// multiply by -i (forward, sign = -1) or +i (inverse, sign = +1)
-__device__ __forceinline__ float2 rot(float2 a, int sign) {
-    return sign < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
-}
-
-// Every pass touches each value once.  For N >= 512 (spectra of 1 GB and
-// more) the HBM accesses are streaming (non-temporal); for smaller N the
-// plain accesses let the 256 MB last-level cache hold the spectrum between
-// passes.
-typedef float f2v __attribute__((ext_vector_type(2)));
-template <int N>
-__device__ __forceinline__ float2 ld2nt(const float2* p) {
-    if constexpr (N >= 512) {
-        const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
-        return make_float2(v.x, v.y);
-    } else {
-        return *p;
-    }
-}
-template <int N>
-__device__ __forceinline__ void st2nt(float2* p, float2 v) {
-    if constexpr (N >= 512) {
-        const f2v w = {v.x, v.y};
-        __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
-    } else {
-        *p = v;
-    }
-}
-template <int N>
-__device__ __forceinline__ float ld1nt(const float* p) {
-    if constexpr (N >= 512) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-template <int N>
-__device__ __forceinline__ void st1nt(float* p, float v) {
-    if constexpr (N >= 512) __builtin_nontemporal_store(v, p);
-    else *p = v;
-}
-
-template <int R>
-__device__ __forceinline__ void dft(float2 (&x)[R], int sign) {
-    if constexpr (R == 2) {
-        const float2 a = x[0], b = x[1];
-        x[0] = cadd(a, b);
-        x[1] = csub(a, b);
-    } else if constexpr (R == 4) {
-        const float2 t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]);
-        const float2 t2 = cadd(x[1], x[3]), t3 = rot(csub(x[1], x[3]), sign);
-        x[0] = cadd(t0, t2);
-        x[2] = csub(t0, t2);
-        x[1] = cadd(t1, t3);
-        x[3] = csub(t1, t3);
-    } else {
-        const float c = 0.70710678118654752f;
-        const float2 t0 = cadd(x[0], x[4]), t1 = csub(x[0], x[4]);
-        const float2 t2 = cadd(x[2], x[6]), t3 = rot(csub(x[2], x[6]), sign);
-        const float2 t4 = cadd(x[1], x[5]), t5 = csub(x[1], x[5]);
-        const float2 t6 = cadd(x[3], x[7]), t7 = rot(csub(x[3], x[7]), sign);
-        const float2 e0 = cadd(t0, t2), e2 = csub(t0, t2), e1 = cadd(t1, t3), e3 = csub(t1, t3);
-        const float2 o0 = cadd(t4, t6), o2 = csub(t4, t6), o1 = cadd(t5, t7), o3 = csub(t5, t7);
-        // W8^1 = (c, sign c), W8^2 = sign i, W8^3 = (-c, sign c)
-        const float2 w1o1 = make_float2(c * (o1.x - sign * o1.y), c * (o1.y + sign * o1.x));
-        const float2 w2o2 = rot(o2, sign);
-        const float2 w3o3 = make_float2(-c * (o3.x + sign * o3.y), c * (sign * o3.x - o3.y));
-        x[0] = cadd(e0, o0);
-        x[4] = csub(e0, o0);
-        x[1] = cadd(e1, w1o1);
-        x[5] = csub(e1, w1o1);
-        x[2] = cadd(e2, w2o2);
-        x[6] = csub(e2, w2o2);
-        x[3] = cadd(e3, w3o3);
-        x[7] = csub(e3, w3o3);
-    }
-}
-
-// One radix-R stage (span Ns) on the 8 register values of thread t.
-// Butterfly u (of NB = 8/R) takes positions j + r*(N/R), j = t + u*TC,
-// which are the register slots u + r*NB.
-template <int N, int R>
-__device__ __forceinline__ void stage_regs(float2 (&x)[8], int t, int Ns, int sign,
-                                           const float2* __restrict__ twl) {
-    constexpr int TC = N / 8;
-    constexpr int NB = 8 / R;
-#pragma unroll
-    for (int u = 0; u < NB; ++u) {
-        const int k = (t + u * TC) & (Ns - 1);
-        float2 v[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            float2 a = x[u + r * NB];
-            if (r > 0 && Ns > 1) {
-                float2 w = twl[(r * k * (N / (Ns * R))) & (N - 1)];
-                if (sign > 0) w.y = -w.y;
-                a = cmul(a, w);
-            }
-            v[r] = a;
-        }
-        dft<R>(v, sign);
-#pragma unroll
-        for (int r = 0; r < R; ++r) x[u + r * NB] = v[r];
-    }
-}
-
-// Stage boundary through LDS: results of the radix-R stage of span Ns go to
-// their Stockham positions (j/Ns)*Ns*R + j%Ns + r*Ns, then every thread
-// reads back positions t + e*TC.  `at(pos)` maps a line position to its LDS
-// slot.  The leading barrier protects the previous boundary's reads.
-template <int N, int R, class At>
-__device__ __forceinline__ void exchange(float2 (&x)[8], float2* lds, int t, int Ns, At at) {
-    constexpr int TC = N / 8;
-    constexpr int NB = 8 / R;
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < NB; ++u) {
-        const int j = t + u * TC;
-        const int k = j & (Ns - 1);
-        const int base = (j - k) * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) lds[at(base + r * Ns)] = x[u + r * NB];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < 8; ++e) x[e] = lds[at(t + e * TC)];
-}
-
-// Full length-N transform of the register line; result in the same slots.
-template <int N, class At>
-__device__ __forceinline__ void fft_line(float2 (&x)[8], float2* lds, int t, int sign,
-                                         const float2* __restrict__ twl, At at) {
-    constexpr int L = ilog2(N);
-    constexpr int n8 = L / 3;
-    constexpr int rem = L % 3;
-    int Ns = 1;
-#pragma unroll
-    for (int s = 0; s < n8; ++s) {
-        stage_regs<N, 8>(x, t, Ns, sign, twl);
-        if (s + 1 < n8 || rem != 0) exchange<N, 8>(x, lds, t, Ns, at);
-        Ns *= 8;
-    }
-    if constexpr (rem == 1) stage_regs<N, 2>(x, t, Ns, sign, twl);
-    if constexpr (rem == 2) stage_regs<N, 4>(x, t, Ns, sign, twl);
-}
-
-template <int N>
-__device__ __forceinline__ void load_twiddles(float2* twl, const float2* __restrict__ tw) {
-    for (int m = threadIdx.x; m < N; m += blockDim.x) twl[m] = tw[m];
-}
-
 struct ColArgs {
     float2* spec;
     const float2* tw;
@@ -586,6 +432,27 @@ void destroy(ScvPlan* p) {
 // Column groups per workgroup for the j passes and the i pass.
 constexpr int kRoundsJ = 2, kRoundsI = 4;
 
+// The three column passes of the direct method on the spectrum in place.
+int run_columns(ScvPlan* p, int filter, float gamma, float D, hipStream_t st, int ncomp_used) {
+    const int N = p->N;
+    const double n3 = double(N) * N * N;
+    ColArgs A{p->spec, p->tw, ncomp_used, p->Hp, 0, 0, 0, filter, gamma, D, float(1.0 / n3), 1};
+    // forward along j: slab (a, i), rows j
+    A.slab_stride = (long long)N * p->Hp; A.row_stride = p->Hp; A.mode = 0;
+    A.rounds = kRoundsJ;
+    int rc = columns(N, A, st);
+    if (rc) return rc;
+    // along i: forward, filter, inverse in one pass; slab (a, j), rows i
+    A.slab_stride = p->Hp; A.row_stride = (long long)N * p->Hp; A.mode = 2;
+    A.rounds = kRoundsI;
+    rc = columns(N, A, st);
+    if (rc) return rc;
+    // inverse along j
+    A.slab_stride = (long long)N * p->Hp; A.row_stride = p->Hp; A.mode = 1;
+    A.rounds = kRoundsJ;
+    return columns(N, A, st);
+}
+
 // Transform e -> spectrum -> filter -> out.  filter 0: Helmholtz; 1: Poisson.
 int run_filter(ScvPlan* p, const float* e, int filter, float gamma, float D, float* out,
                hipStream_t st, int ncomp_used) {
@@ -594,21 +461,7 @@ int run_filter(ScvPlan* p, const float* e, int filter, float gamma, float D, flo
     if (p->method == 2) {
         int rc = rows(N, 0, e, p->spec, nullptr, p->tw, p->Hp, ncomp_used, st);
         if (rc) return rc;
-        ColArgs A{p->spec, p->tw, ncomp_used, p->Hp, 0, 0, 0, filter, gamma, D, float(1.0 / n3), 1};
-        // forward along j: slab (a, i), rows j
-        A.slab_stride = (long long)N * p->Hp; A.row_stride = p->Hp; A.mode = 0;
-        A.rounds = kRoundsJ;
-        rc = columns(N, A, st);
-        if (rc) return rc;
-        // along i: forward, filter, inverse in one pass; slab (a, j), rows i
-        A.slab_stride = p->Hp; A.row_stride = (long long)N * p->Hp; A.mode = 2;
-        A.rounds = kRoundsI;
-        rc = columns(N, A, st);
-        if (rc) return rc;
-        // inverse along j
-        A.slab_stride = (long long)N * p->Hp; A.row_stride = p->Hp; A.mode = 1;
-        A.rounds = kRoundsJ;
-        rc = columns(N, A, st);
+        rc = run_columns(p, filter, gamma, D, st, ncomp_used);
         if (rc) return rc;
         return rows(N, 1, nullptr, p->spec, out, p->tw, p->Hp, ncomp_used, st);
     }
@@ -637,6 +490,32 @@ int run(ScvPlan* p, const float* e, float gamma, float D, float* sa_out, hipStre
 }
 
 }  // namespace
+
+bool scv_view(void* plan, ScvView* v) {
+    auto* p = static_cast<ScvPlan*>(plan);
+    if (!p) return false;
+    *v = ScvView{p->spec, p->tw, p->N, p->nc, p->Hp, p->method};
+    return true;
+}
+
+int scv_prepare(void* plan, const float* pr, const float* pi, float gamma, float D,
+                hipStream_t st) {
+    auto* p = static_cast<ScvPlan*>(plan);
+    if (!p || p->method != 2) { set_error("scv_prepare: needs a direct-method plan"); return LFM_E_ARG; }
+    const size_t n = size_t(p->nc) * p->N * p->N * p->N;
+    hipLaunchKernelGGL(psisq_kernel, dim3(grid_for(n)), dim3(256), 0, st, pr, pi, p->e, n);
+    LFM_LAUNCH_CHECK("psisq_kernel");
+    int rc = rows(p->N, 0, p->e, p->spec, nullptr, p->tw, p->Hp, p->nc, st);
+    if (rc) return rc;
+    return run_columns(p, 0, gamma, D, st, p->nc);
+}
+
+int scv_columns(void* plan, float gamma, float D, hipStream_t st) {
+    auto* p = static_cast<ScvPlan*>(plan);
+    if (!p || p->method != 2) { set_error("scv_columns: needs a direct-method plan"); return LFM_E_ARG; }
+    return run_columns(p, 0, gamma, D, st, p->nc);
+}
+
 }  // namespace lfm
 
 using namespace lfm;
@@ -756,6 +635,27 @@ extern "C" int lfm_scv_apply(void* plan, const float* pr, const float* pi, float
     hipLaunchKernelGGL(psisq_kernel, dim3(grid_for(n)), dim3(256), 0, st, pr, pi, p->e, n);
     LFM_LAUNCH_CHECK("psisq_kernel");
     return run(p, p->e, gamma, D, sa_out, st);
+}
+
+extern "C" int lfm_scv_prepare_f32(void* plan, const float* pr, const float* pi, float gamma,
+                                   float D, void* stream) {
+    auto* p = static_cast<ScvPlan*>(plan);
+    if (!p || !pr || !pi) { set_error("lfm_scv_prepare_f32: NULL argument"); return LFM_E_ARG; }
+    if (!(gamma > 0.0f) || D < 0.0f) {
+        set_error("lfm_scv_prepare_f32: need gamma > 0 and D >= 0 (got %g, %g)", gamma, D);
+        return LFM_E_ARG;
+    }
+    return scv_prepare(plan, pr, pi, gamma, D, (hipStream_t)stream);
+}
+
+extern "C" int lfm_scv_columns_f32(void* plan, float gamma, float D, void* stream) {
+    auto* p = static_cast<ScvPlan*>(plan);
+    if (!p) { set_error("lfm_scv_columns_f32: NULL plan"); return LFM_E_ARG; }
+    if (!(gamma > 0.0f) || D < 0.0f) {
+        set_error("lfm_scv_columns_f32: need gamma > 0 and D >= 0 (got %g, %g)", gamma, D);
+        return LFM_E_ARG;
+    }
+    return scv_columns(plan, gamma, D, (hipStream_t)stream);
 }
 
 extern "C" int lfm_clip_nonneg_f32(const float* x, float* y, int64_t n, void* stream) {

@@ -22,6 +22,7 @@
 //                               - CV + chi_self - kappa_s CCV - kappa_t SCV)
 //   chi' = max(chi', -chi0); chi' = m chi0 + (1-m) chi'; Psi' *= (1-m).
 #include "lfm_internal.h"
+#include "lfm_fft.h"
 
 #include <algorithm>
 #include <vector>
@@ -129,7 +130,7 @@ struct PointData {
     float sa[3];
 };
 
-template <int L, class O>
+template <int L, class O, bool SA = true>
 __device__ __forceinline__ PointData<L> load_point(const KParams& kp, const Ptrs& P, O idx) {
     constexpr int NPF = Lv<L>::NPF;
     PointData<L> d;
@@ -137,7 +138,7 @@ __device__ __forceinline__ PointData<L> load_point(const KParams& kp, const Ptrs
     for (int f = 0; f < NPF; ++f) d.prev[f] = idx.ldp(field<L>(P.prp, P.pip, f, P.n3));
     d.m = (kp.flags & LFM_F_MASK) ? idx.ldp(P.mask) : 0.0f;
     d.chip = (kp.flags & LFM_F_FREEZE_CHI) ? 0.0f : idx.ldp(P.chip);
-    if constexpr (L == COLOR) {
+    if constexpr (L == COLOR && SA) {
         if (P.sa) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) d.sa[q] = idx.ldp(P.sa + q * P.n3);
@@ -156,7 +157,8 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
                                             const PointData<L>& pd,
                                             const float (&c)[Lv<L>::NPF],
                                             const float (&lap)[Lv<L>::NPF], float chi_st,
-                                            float lap_chi, float jsum, float ccv) {
+                                            float lap_chi, float jsum, float ccv,
+                                            float* ret3 = nullptr) {
     constexpr int NPF = Lv<L>::NPF;
     constexpr int C = Lv<L>::NC;
     const size_t n3 = P.n3;
@@ -193,6 +195,10 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
         if (P.psisq) {
 #pragma unroll
             for (int q = 0; q < C; ++q) idx.st(P.psisq + q * n3, nw[q] * nw[q] + nw[q + C] * nw[q + C]);
+        }
+        if (ret3) {
+#pragma unroll
+            for (int q = 0; q < C; ++q) ret3[q] = nw[q] * nw[q] + nw[q + C] * nw[q + C];
         }
     }
     if (kp.flags & LFM_F_FREEZE_CHI) return;
@@ -634,6 +640,331 @@ int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
 #undef LFM_TILES
 }
 
+// ---------------------------------------------------------------------------
+// COLOR step with the k-axis passes of the S_a transform folded in.
+//
+// The direct S_a transform is five passes: real k-lines to half spectra,
+// lines along j, the fused i pass (forward, filter, inverse), lines along j,
+// half spectra back to real k-lines.  Here a workgroup owns one full k-row j
+// of every plane of a chunk, so the first and last of those passes run inside
+// the step: the workgroup reads the k-axis spectra of S_a for its row (what
+// the three column passes left in the plan's spectrum), turns them into S_a
+// with two inverse line FFTs (colours 0 and 1 packed as z = x_0 + i x_1,
+// colour 2 alone), updates the sites, and turns |Psi'_a|^2 of the new row
+// into its k-axis half spectra with two forward line FFTs, written in place
+// over the rows it has read.  Each spectrum row belongs to one workgroup and
+// is read two planes before it is rewritten.  Per site this replaces the S_a
+// read and the |Psi'|^2 write of the plain step (12 B each) by spectrum reads
+// and writes of the same size and removes two whole transform passes.
+//
+// The stencil part is the plain kernel's plane march with a full-row tile:
+// thread k owns column k; the staged plane is the rows j-1, j, j+1
+// (periodic); the k halo is the row's own wrap-around.  Four of the
+// N/8-thread line groups run the two inverse and the two forward FFTs of a
+// plane side by side; every thread takes part in the barriers.
+// ---------------------------------------------------------------------------
+template <int N>
+struct KfftLds {
+    static constexpr int LW = N + 2;
+    static constexpr int PS = N + N / 8;      // line FFT buffer (slot p + p/8)
+    static constexpr int H = N / 2 + 1;
+    float st[3][LW][8];                       // stencil fields of rows j-1, j, j+1
+    float fz[3][N];                           // |Psi'_a|^2 of the finished row
+    float2 fx[4][PS];                         // line FFT exchange buffers
+    float sab[2][3][N];                       // S_a of the pending plane (parity of q)
+    float2 specin[3][H];                      // k-axis spectra of S_a, next plane
+    float2 twl[N];
+};
+
+template <int N, bool NT>
+__global__ __launch_bounds__(N) void step_color_kfft(Ptrs P, KParams kp, float2* __restrict__ spec,
+                                                     const float2* __restrict__ tw, int Hp,
+                                                     int CH) {
+    constexpr int L = COLOR;
+    constexpr int NPF = Lv<L>::NPF;            // 6
+    constexpr int C = Lv<L>::NC;               // 3
+    constexpr int NF = NPF + 1;                // + chi
+    constexpr int TC = N / 8;
+    constexpr int H = N / 2 + 1;
+    static_assert(4 * TC <= N, "four line groups per workgroup");
+    __shared__ KfftLds<N> S;
+
+    const uint32_t rowB = uint32_t(N) * 4u;
+    const uint32_t planeB = rowB * uint32_t(N);
+    const size_t n3 = P.n3;
+    // XCD grouping as in step_tiled: consecutive rows share an L2
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+    const int j = lb % N;
+    const int i0 = (lb / N) * CH;
+    const int i1 = min(i0 + CH, N);
+    const int k = threadIdx.x;
+
+    S.twl[k] = tw[k];
+
+    const float* src[NF];
+#pragma unroll
+    for (int f = 0; f < NPF; ++f) src[f] = field<L>(P.pr, P.pi, f, n3);
+    src[NPF] = P.chi;
+    uint32_t roff[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) roff[r] = uint32_t((j - 1 + r + N) % N) * rowB + uint32_t(k) * 4u;
+
+    float pre[NF][3];
+    auto prefetch = [&](int q) {
+        const uint32_t pb = uint32_t((q + N) % N) * planeB;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) pre[f][r] = Off32{pb + roff[r]}.ld(src[f]);
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int f = 0; f < NF; ++f) S.st[r][k + 1][f] = pre[f][r];
+        if (k == N - 1) {   // column -1 is column N-1 of the same row
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int f = 0; f < NF; ++f) S.st[r][0][f] = pre[f][r];
+        }
+        if (k == 0) {       // column N is column 0
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int f = 0; f < NF; ++f) S.st[r][N + 1][f] = pre[f][r];
+        }
+    };
+    // k-axis spectra of S_a of plane q, all colours
+    constexpr int NSPEC = 3 * H;
+    constexpr int NSU = (NSPEC + N - 1) / N;
+    float2 spre[NSU];
+    auto spec_row = [&](int c, int q) {
+        return spec + ((size_t(c) * N + q) * N + j) * size_t(Hp);
+    };
+    auto spec_prefetch = [&](int q) {
+#pragma unroll
+        for (int u = 0; u < NSU; ++u) {
+            const int id = k + u * N;
+            if (id < NSPEC) spre[u] = ld2nt<N>(spec_row(id / H, q) + id % H);
+        }
+    };
+    auto spec_stage = [&]() {
+#pragma unroll
+        for (int u = 0; u < NSU; ++u) {
+            const int id = k + u * N;
+            if (id < NSPEC) S.specin[id / H][id % H] = spre[u];
+        }
+    };
+
+    // line groups: 0 inverse of colours 0+1, 1 inverse of colour 2,
+    // 2 forward of colours 0+1, 3 forward of colour 2
+    const int grp = k / TC, tq = k % TC;
+    const bool fjob = grp < 4;
+    const bool inv = grp < 2;
+    const bool pair = (grp & 1) == 0;
+    float2* xb = S.fx[fjob ? grp : 0];
+    auto slot = [](int pos) { return pos + pos / 8; };
+    // One FFT phase: S_a of plane qi (when inv_ok) into sab; the k-axis half
+    // spectra of the |Psi'|^2 row of plane qf (when fwd_ok) to the spectrum.
+    auto fft_phase = [&](int qi, bool inv_ok, int qf, bool fwd_ok) {
+        float2 v[8];
+        if (inv) {
+            const int c0 = pair ? 0 : 2;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int kk = tq + e * TC;
+                const bool lo = kk <= N / 2;
+                const int m = lo ? kk : N - kk;
+                float2 a = S.specin[c0][m];
+                float2 bb = pair ? S.specin[1][m] : make_float2(0.0f, 0.0f);
+                if (m == 0 || m == N / 2) { a.y = 0.0f; bb.y = 0.0f; }
+                if (!lo) { a.y = -a.y; bb.y = -bb.y; }
+                v[e] = make_float2(a.x - bb.y, a.y + bb.x);
+            }
+        } else if (fjob) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int kk = tq + e * TC;
+                v[e] = pair ? make_float2(S.fz[0][kk], S.fz[1][kk]) : make_float2(S.fz[2][kk], 0.0f);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = make_float2(0.0f, 0.0f);
+        }
+        fft_line_m<N>(v, xb, tq, inv ? 1 : -1, S.twl, slot, fjob);
+        if (inv && inv_ok) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int kk = tq + e * TC;
+                if (pair) {
+                    S.sab[qi & 1][0][kk] = v[e].x;
+                    S.sab[qi & 1][1][kk] = v[e].y;
+                } else {
+                    S.sab[qi & 1][2][kk] = v[e].x;
+                }
+            }
+        }
+        // forward: split the packed pair with Hermitian symmetry
+        __syncthreads();
+        if (fjob && !inv) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xb[slot(tq + e * TC)] = v[e];
+        }
+        __syncthreads();
+        if (fjob && !inv && fwd_ok) {
+            float2* o0 = spec_row(pair ? 0 : 2, qf);
+            float2* o1 = spec_row(1, qf);
+#pragma unroll
+            for (int e = 0; e < 5; ++e) {
+                const int kk = tq + e * TC;
+                if (e == 4 && tq != 0) break;   // kk = N/2 once
+                const float2 zk = xb[slot(kk)];
+                const float2 zm = xb[slot((N - kk) & (N - 1))];
+                st2nt<N>(o0 + kk, make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y)));
+                if (pair) st2nt<N>(o1 + kk, make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x)));
+            }
+        }
+    };
+
+    // register queue (as in tile_body, one row)
+    float acc[NF], bprev[NF], cm2[NPF], cm1[NPF], chim1 = 0.0f, jyz = 0.0f, ccvyz = 0.0f;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) { acc[f] = 0.0f; bprev[f] = 0.0f; }
+#pragma unroll
+    for (int f = 0; f < NPF; ++f) { cm2[f] = 0.0f; cm1[f] = 0.0f; }
+    const uint32_t ioff = uint32_t(j) * rowB + uint32_t(k) * 4u;
+    const int cnt = (i1 - i0) + 2;   // planes i0-1 .. i1
+    prefetch(i0 - 1);
+    spec_prefetch(i0);
+    for (int s = 0; s < cnt; ++s) {
+        const int q = i0 - 1 + s;
+        const bool site_q = q >= i0 && q < i1;
+        PointData<L> pd;
+        if (s >= 2) {
+            // pointwise inputs of the site finished this iteration (plane q-1);
+            // the staging and the FFT phase below cover their latency
+            pd = load_point<L, Off32T<NT>, false>(kp, P, Off32T<NT>{uint32_t(q - 1) * planeB + ioff});
+        }
+        __syncthreads();
+        stage();
+        if (site_q) spec_stage();
+        __syncthreads();
+        if (s + 1 < cnt) prefetch(q + 1);
+        if (q + 1 >= i0 && q + 1 < i1) spec_prefetch(q + 1);
+        // S_a of plane q; k-axis spectra of |Psi'|^2 of plane q-2 (finished last iteration)
+        fft_phase(q, site_q, q - 2, s >= 3);
+        const int x = k + 1;
+        float A[NF], B[NF], cq[NF], dy[NPF], dz[NPF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const float ctr = S.st[1][x][f];
+            const float nn = S.st[0][x][f], ss = S.st[2][x][f];
+            const float ww = S.st[1][x - 1][f], ee = S.st[1][x + 1][f];
+            const float faces = (nn + ss) + (ww + ee);
+            const float diag = (S.st[0][x - 1][f] + S.st[0][x + 1][f]) +
+                               (S.st[2][x - 1][f] + S.st[2][x + 1][f]);
+            A[f] = WF * faces + WE * diag - 4.0f * ctr;
+            B[f] = WF * ctr + WE * faces;
+            cq[f] = ctr;
+            if (f < NPF) { dy[f] = ss - nn; dz[f] = ee - ww; }
+        }
+        float sy = 0.0f, sz = 0.0f, qy = 0.0f, qz = 0.0f;
+#pragma unroll
+        for (int a2 = 0; a2 < C; ++a2) {
+            const float jy = cq[a2] * dy[a2 + C] - cq[a2 + C] * dy[a2];
+            const float jz = cq[a2] * dz[a2 + C] - cq[a2 + C] * dz[a2];
+            sy += jy; sz += jz; qy += jy * jy; qz += jz * jz;
+        }
+        const float jyz_new = sy + sz;
+        const float ccv_new = (qy - (1.0f / 3.0f) * sy * sy) + (qz - (1.0f / 3.0f) * sz * sz);
+        if (s >= 2) {
+            // finish the site at plane q-1 (always a site plane of this chunk)
+            float lap[NPF], cpend[NPF];
+#pragma unroll
+            for (int f = 0; f < NPF; ++f) { lap[f] = acc[f] + B[f]; cpend[f] = cm1[f]; }
+            const float lap_chi = acc[NPF] + B[NPF];
+            float sx = 0.0f, qx = 0.0f;
+#pragma unroll
+            for (int a2 = 0; a2 < C; ++a2) {
+                const float jx = cm1[a2] * (cq[a2 + C] - cm2[a2 + C]) -
+                                 cm1[a2 + C] * (cq[a2] - cm2[a2]);
+                sx += jx; qx += jx * jx;
+            }
+            const float jsum = sx + jyz;
+            const float ccv = kp.ks > 0.0f ? ccvyz + (qx - (1.0f / 3.0f) * sx * sx) : 0.0f;
+#pragma unroll
+            for (int c = 0; c < C; ++c) pd.sa[c] = S.sab[(q - 1) & 1][c][k];
+            float e2[C];
+            site_update<L>(kp, P, Off32T<NT>{uint32_t(q - 1) * planeB + ioff}, pd, cpend, lap,
+                           chim1, lap_chi, jsum, ccv, e2);
+#pragma unroll
+            for (int c = 0; c < C; ++c) S.fz[c][k] = e2[c];
+        }
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            acc[f] = A[f] + bprev[f];
+            bprev[f] = B[f];
+        }
+#pragma unroll
+        for (int f = 0; f < NPF; ++f) { cm2[f] = cm1[f]; cm1[f] = cq[f]; }
+        chim1 = cq[NPF];
+        jyz = jyz_new;
+        ccvyz = ccv_new;
+    }
+    // the last finished plane's |Psi'|^2 row
+    __syncthreads();
+    fft_phase(0, false, i1 - 1, true);
+}
+
+// Workgroups of step_color_kfft<N> one device holds at once; 0 if unknown.
+template <int N, bool NT>
+int kfft_capacity() {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_color_kfft<N, NT>, N, 0) !=
+            hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return cus * per;
+}
+
+template <int N, bool NT>
+int launch_kfft(const Ptrs& P, const KParams& kp, const ScvView& v, hipStream_t st) {
+    // one round of resident workgroups: as many i chunks as fit beside the
+    // N rows (at least one)
+    static const int cap = kfft_capacity<N, NT>();
+    int CH = kp.chunk;
+    if (CH <= 0) {
+        int nch = cap > 0 ? cap / N : 1;
+        if (nch < 1) nch = 1;
+        CH = (N + nch - 1) / nch;
+    }
+    CH = min(CH, N);
+    const int nch = (N + CH - 1) / CH;
+    const unsigned nblocks = unsigned(N * nch);
+    hipLaunchKernelGGL((step_color_kfft<N, NT>), dim3(nblocks), dim3(N), 0, st, P, kp, v.spec, v.tw,
+                       v.Hp, CH);
+    LFM_LAUNCH_CHECK("step_color_kfft");
+    return LFM_OK;
+}
+
+bool kfft_supported(int N) { return N == 64 || N == 128 || N == 256 || N == 512; }
+
+int launch_kfft_any(const Ptrs& P, const KParams& kp, const ScvView& v, hipStream_t st) {
+    const bool nt = streaming<COLOR>(P.n3);
+    switch (kp.N) {
+        case 64: return nt ? launch_kfft<64, true>(P, kp, v, st) : launch_kfft<64, false>(P, kp, v, st);
+        case 128: return nt ? launch_kfft<128, true>(P, kp, v, st) : launch_kfft<128, false>(P, kp, v, st);
+        case 256: return nt ? launch_kfft<256, true>(P, kp, v, st) : launch_kfft<256, false>(P, kp, v, st);
+        case 512: return nt ? launch_kfft<512, true>(P, kp, v, st) : launch_kfft<512, false>(P, kp, v, st);
+        default: set_error("fused COLOR step: N=%d unsupported", kp.N); return LFM_E_ARG;
+    }
+}
+
 constexpr int tile_tj(int tile) {
     return tile == 1 ? 4 : tile == 2 ? 8 : tile == 3 ? 16 : tile == 4 ? 8 : tile == 5 ? 16 : 16;
 }
@@ -812,6 +1143,54 @@ extern "C" int lfm_step_color_f32(const float* pr, const float* pr_prev, const f
     P.psisq = psisq_out;
     P.n3 = n3;
     return launch<COLOR>(P, to_k(p), (hipStream_t)stream);
+}
+
+extern "C" int lfm_step_color_fused_f32(const float* pr, const float* pr_prev, const float* pi,
+                                        const float* pi_prev, const float* chi,
+                                        const float* chi_prev, const float* mask, float* pr_out,
+                                        float* pi_out, float* chi_out, void* scv_plan,
+                                        const lfm_params_t* p, void* stream) {
+    if (!pr || !pr_prev || !pi || !pi_prev || !chi || !chi_prev || !pr_out || !pi_out ||
+        !chi_out || !p || !scv_plan) {
+        set_error("lfm_step_color_fused_f32: required pointer is NULL");
+        return LFM_E_ARG;
+    }
+    if ((p->flags & LFM_F_MASK) && !mask) {
+        set_error("lfm_step_color_fused_f32: LFM_F_MASK set but mask is NULL");
+        return LFM_E_ARG;
+    }
+    if (!(p->kappa_tube > 0.0f)) {
+        set_error("lfm_step_color_fused_f32: needs kappa_tube > 0 (use lfm_step_color_f32)");
+        return LFM_E_ARG;
+    }
+    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI)) {
+        set_error("lfm_step_color_fused_f32: LFM_F_DIRECT / LFM_F_FREEZE_CHI not supported");
+        return LFM_E_ARG;
+    }
+    ScvView v{};
+    scv_view(scv_plan, &v);
+    if (v.method != 2 || v.nc != 3 || v.N != p->N || !kfft_supported(p->N)) {
+        set_error("lfm_step_color_fused_f32: needs a direct-method S_a plan with 3 colours of the "
+                  "same N, N in {64, 128, 256, 512} (plan N=%d colours=%d method=%d, step N=%d)",
+                  v.N, v.nc, v.method, p->N);
+        return LFM_E_ARG;
+    }
+    const size_t n3 = size_t(p->N) * p->N * p->N;
+    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask};
+    const size_t nin[] = {3 * n3, 3 * n3, 3 * n3, 3 * n3, n3, n3, n3};
+    float* outs[] = {pr_out, pi_out, chi_out};
+    const size_t nout[] = {3 * n3, 3 * n3, n3};
+    int rc = validate(p, ins, nin, 7, outs, nout, 3, "lfm_step_color_fused_f32");
+    if (rc) return rc;
+    Ptrs P{};
+    P.pr = pr; P.pi = pi; P.prp = pr_prev; P.pip = pi_prev; P.chi = chi; P.chip = chi_prev;
+    P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
+    // S_a comes from the plan's spectrum inside the kernel; a non-NULL marker
+    // switches the SCV term on (it is never dereferenced)
+    P.sa = reinterpret_cast<const float*>(v.spec);
+    P.pr_out = pr_out; P.pi_out = pi_out; P.chi_out = chi_out;
+    P.n3 = n3;
+    return launch_kfft_any(P, to_k(p), v, (hipStream_t)stream);
 }
 
 extern "C" int lfm_batch_plan_create(int32_t level, int32_t nlat, const lfm_params_t* params,

@@ -35,6 +35,9 @@ SIGNATURES = [
     ("lfm_step_real_f32", ctypes.c_int, _STEP_REAL),
     ("lfm_step_complex_f32", ctypes.c_int, _STEP_COMPLEX),
     ("lfm_step_color_f32", ctypes.c_int, _STEP_COLOR),
+    ("lfm_step_color_fused_f32", ctypes.c_int, [_vp] * 11 + [_vp, _vp]),
+    ("lfm_scv_prepare_f32", ctypes.c_int, [_vp, _vp, _vp, _f32, _f32, _vp]),
+    ("lfm_scv_columns_f32", ctypes.c_int, [_vp, _f32, _f32, _vp]),
     ("lfm_scv_plan_create", ctypes.c_int, [_i32, _i32, ctypes.POINTER(_vp)]),
     ("lfm_scv_plan_create_ex", ctypes.c_int, [_i32, _i32, _i32, ctypes.POINTER(_vp)]),
     ("lfm_scv_plan_method", _i32, [_vp]),
@@ -49,6 +52,7 @@ SIGNATURES = [
     ("lfm_batch_workgroups", _i32, [_vp]),
     ("lfm_batch_step_f32", ctypes.c_int, [_vp] * 11 + [_i32, _vp]),
     ("lfm_adjoint_partials_len", _i64, [_i32]),
+    ("lfm_calib_stream_f32", ctypes.c_int, [_vp, _i32, _vp, _i32, ctypes.c_int64, _i32, _vp]),
     ("lfm_adjoint_complex_seed_f32", ctypes.c_int, [_vp] * 8 + [_vp, _vp]),
     ("lfm_adjoint_complex_step_f32", ctypes.c_int, [_vp] * 13 + [_i32] + [_vp] * 4),
     ("lfm_reduce_partials_len", _i64, [_i32]),

@@ -17,7 +17,8 @@ INCLUDE = os.path.join(REPO, "include")
 LIB_NAME = "liblfm_hip.so"
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-SOURCES = ["lfm_abi.hip", "lfm_step.hip", "lfm_scv.hip", "lfm_reduce.hip", "lfm_adjoint.hip"]
+SOURCES = ["lfm_abi.hip", "lfm_step.hip", "lfm_scv.hip", "lfm_reduce.hip", "lfm_adjoint.hip",
+           "lfm_calib.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -64,7 +64,7 @@ class Evolver:
     """
 
     def __init__(self, config: SimulationConfig, backend="auto", mask_policy="reference",
-                 chi_storage="delta"):
+                 chi_storage="delta", sa_transform="auto"):
         self.config = config
         self.backend = get_backend(backend)
         self._lib = _abi.load()
@@ -103,11 +103,27 @@ class Evolver:
         self.boundary_mask = self.backend.create_boundary_mask(self.N, config.boundary_fraction)
         self._absorb = 1.0 - self.boundary_mask
         self.sa_enabled = bool(config.sa_enabled and lvl == FieldLevel.COLOR)
+        if sa_transform not in ("auto", "fused", "separate"):
+            raise ValueError(f"sa_transform must be 'auto', 'fused' or 'separate', got {sa_transform!r}")
+        self._fused = False
         if self.sa_enabled:
             self._sa = torch.zeros(3 * self.total, dtype=f32, device=dev)
             self._psisq = torch.empty(3 * self.total, dtype=f32, device=dev)
             self._psisq_valid = False
             self._scv = ScvPlan(self.N, 3)
+            # "fused": the k-axis transform passes folded into the step kernel
+            # (power-of-two N up to 512).  Its full-row workgroups hold one
+            # workgroup per CU at N = 512, so it is latency-bound there; measured
+            # per step (MI355X): N=64 0.047 vs 0.051 ms, N=128 0.138 vs 0.116,
+            # N=256 0.88 vs 0.78, N=512 8.4 vs 6.5 ms for fused vs separate.
+            # "auto" therefore fuses only at N = 64.
+            can_fuse = self._scv.method == 2 and self.N in (64, 128, 256, 512)
+            if sa_transform == "fused" and not can_fuse:
+                raise ValueError(f"sa_transform='fused' needs N in (64, 128, 256, 512), got {self.N}")
+            self._fused = can_fuse and (sa_transform == "fused"
+                                        or (sa_transform == "auto" and self.N == 64))
+            self._spec_valid = False    # plan spectrum = k-axis spectra of S_a for current Psi
+            self._sa_stale = False      # self._sa not updated by fused steps
         else:
             self._sa = None
         self._params = {}
@@ -141,15 +157,20 @@ class Evolver:
                                                   self._cp, self._oc, self._op))
         results = {}
         old = (self.tile, self.chunk, self.kernel_events)
-        for tile in range(1, _abi.LFM_TILE_VARIANTS + 1):
-            for ch in chunks:
-                self.tile, self.chunk = tile, ch
-                self.kernel_events = []
-                for _ in range(reps + 1):
-                    self._step()
-                torch.cuda.synchronize(self.device)
-                ts = [a.elapsed_time(b) for a, b in self.kernel_events[1:]]
-                results[(tile, ch)] = sorted(ts)[len(ts) // 2]
+        if self._fused:
+            # the fused COLOR kernel has one tile shape (a full k-row); tune its chunk
+            N = self.N
+            combos = [(0, ch) for ch in dict.fromkeys((0, N, N // 2, N // 4, N // 8)) if ch == 0 or ch >= 4]
+        else:
+            combos = [(t, ch) for t in range(1, _abi.LFM_TILE_VARIANTS + 1) for ch in chunks]
+        for tile, ch in combos:
+            self.tile, self.chunk = tile, ch
+            self.kernel_events = []
+            for _ in range(reps + 1):
+                self._step()
+            torch.cuda.synchronize(self.device)
+            ts = [a.elapsed_time(b) for a, b in self.kernel_events[1:]]
+            results[(tile, ch)] = sorted(ts)[len(ts) // 2]
         best = min(results, key=results.get)
         self.tile, self.chunk = best
         self.kernel_events = old[2]
@@ -163,6 +184,8 @@ class Evolver:
         (self._cur, self._prev, self._damped, self._cc, self._cp, self._oc, self._op) = saved[3]
         if self.sa_enabled:
             self._psisq_valid = False
+            self._spec_valid = False
+            self._sa_stale = True    # S_a of the restored state's last step: recomputed on read
         return results
 
     def _step(self, freeze_chi: bool = False, direct: bool = False) -> None:
@@ -196,9 +219,30 @@ class Evolver:
                                           pi[cur].data_ptr(), pi[prev].data_ptr(), chi_in,
                                           chip_in, mask, pr[nxt].data_ptr(), None,
                                           pi[nxt].data_ptr(), None, chi_out, None, p, st)
+        elif self._fused and not (freeze_chi or direct):
+            cfg = self.config
+            if not self._spec_valid:
+                _abi.check(lib.lfm_scv_prepare_f32(self._scv.handle, pr[cur].data_ptr(),
+                                                   pi[cur].data_ptr(), cfg.sa_gamma, cfg.sa_d,
+                                                   st), "lfm_scv_prepare_f32")
+            ev0 = self._mark()
+            rc = lib.lfm_step_color_fused_f32(pr[cur].data_ptr(), pr[prev].data_ptr(),
+                                              pi[cur].data_ptr(), pi[prev].data_ptr(), chi_in,
+                                              chip_in, mask, pr[nxt].data_ptr(),
+                                              pi[nxt].data_ptr(), chi_out, self._scv.handle, p, st)
+            self._mark(ev0)
+            _abi.check(rc, "lfm_step_color_fused_f32")
+            _abi.check(lib.lfm_scv_columns_f32(self._scv.handle, cfg.sa_gamma, cfg.sa_d, st),
+                       "lfm_scv_columns_f32")
+            ev0 = None
+            self._spec_valid = True
+            self._sa_stale = True
+            self._psisq_valid = False
         else:
             sa = psisq = None
             if self.sa_enabled:
+                self._spec_valid = False
+                self._sa_stale = False
                 if self._psisq_valid:
                     self._scv.apply_psisq(self._psisq, self.config.sa_gamma, self.config.sa_d,
                                           self._sa)
@@ -213,7 +257,8 @@ class Evolver:
                                         chi_out, None, sa, None, psisq, p, st)
             if self.sa_enabled:
                 self._psisq_valid = True
-        self._mark(ev0)
+        if ev0 is not None:
+            self._mark(ev0)
         _abi.check(rc, "step")
         self._prev, self._cur = cur, nxt
         self._damped = True
@@ -245,9 +290,13 @@ class Evolver:
 
     # -------------------------------------------------------- device views
     def native_psi_real(self) -> torch.Tensor:
+        """Live device view of the current Psi real parts.  Writes through it
+        are allowed, so the cached |Psi|^2 / S_a spectra are dropped."""
+        self._touch_psi()
         return self._pr[self._cur]
 
     def native_psi_imag(self) -> torch.Tensor | None:
+        self._touch_psi()
         return self._pi[self._cur] if self._has_imag else None
 
     def native_chi(self) -> torch.Tensor:
@@ -308,8 +357,16 @@ class Evolver:
         return e2
 
     def get_sa_fields(self) -> np.ndarray | None:
+        """S_a used by the last step: the transform of |Psi|^2 of that step's
+        input Psi (numpy_backend.py:341), clipped at 0."""
         if self._sa is None:
             return None
+        if self._sa_stale:
+            # fused steps keep S_a only in k space: recompute it from the last
+            # step's input Psi, which the rotation keeps (undamped) in the prev slot
+            self._scv.apply(self._pr[self._prev], self._pi[self._prev], self.config.sa_gamma,
+                            self.config.sa_d, self._sa)
+            self._sa_stale = False
         return torch.clamp_min(self._sa, 0.0).cpu().numpy().reshape(3, self.N, self.N, self.N)
 
     # ------------------------------------------------------------- setters
@@ -335,6 +392,7 @@ class Evolver:
     def _touch_psi(self) -> None:
         if self.sa_enabled:
             self._psisq_valid = False
+            self._spec_valid = False
 
     def set_psi_real(self, arr) -> None:
         d = self._dev(arr)
@@ -407,6 +465,7 @@ class Evolver:
             raise ValueError("SA fields not allocated -- set kappa_tube > 0 in SimulationConfig"
                              " before creating the Evolver.")
         self._sa.copy_(self._dev(arr))
+        self._sa_stale = False
 
     # --------------------------------------------------------- reductions
     def _red_bufs(self):

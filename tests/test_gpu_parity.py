@@ -486,3 +486,34 @@ def test_sa_direct_path_matches_rocfft_at_large_n(gpu, N):
     err = float((outs[0] - outs[1]).norm() / outs[0].norm())
     log(f"  rel-L2 {err:.2e}")
     assert err < 2e-6
+
+
+@pytest.mark.parametrize("N", [64, 128])
+def test_fused_color_step_matches_separate(gpu, N):
+    """COLOR full_physics with the k-axis S_a passes fused into the step kernel
+    against the same run with five separate transform passes: same physics,
+    FFT rounding apart.  Also the S_a read-back and a mid-run setter."""
+    import torch
+    import lfm_hip as L
+    log(f"fused vs separate COLOR N={N}")
+    cfg = L.full_physics(N)
+    pr, pi = L.fields.place_solitons(N, [(N / 3, N / 2, N / 2), (2 * N / 3, N / 2, N / 3),
+                                         (N / 2, N / 3, 2 * N / 3)], 3.0, 2.5,
+                                     phases=[0.3, 1.9, 4.0], colors=[0, 1, 2])
+    evs = {}
+    for mode in ("fused", "separate"):
+        ev = L.Evolver(cfg, backend="hip", sa_transform=mode)
+        assert ev._fused == (mode == "fused")
+        ev.set_psi_real(pr)
+        ev.set_psi_imag(pi)
+        ev.evolve(12)
+        ev.set_psi_real_current(ev.get_psi_real() * 1.01)   # invalidates the cached spectra
+        ev.evolve(13)
+        torch.cuda.synchronize()
+        evs[mode] = ev
+    a, b = evs["fused"], evs["separate"]
+    for name in ("get_psi_real", "get_psi_imag", "get_chi", "get_sa_fields"):
+        x, y = getattr(a, name)(), getattr(b, name)()
+        err = O.rel_l2(x - (19.0 if name == "get_chi" else 0.0), y - (19.0 if name == "get_chi" else 0.0))
+        log(f"  {name}: rel-L2 {err:.2e}")
+        assert err < 2e-6, name
