@@ -43,9 +43,13 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     "color512": dict(level=2, N=512, preset="full_physics", mask=True, b_alg=112, streams=(18, 10),
                      desc="COLOR full_physics 512^3 fp32, mask on, S_a FFT every step"),
+    # b_alg2 / streams2: per step of the two-step kernel (one launch reads the
+    # fields once and writes both steps' Psi and chi)
     "complex512": dict(level=1, N=512, preset="gravity_em", mask=False, b_alg=36, streams=(6, 3),
+                       b_alg2=24, streams2=(6, 6),
                        desc="COMPLEX gravity_em 512^3 fp32, periodic (mask-free kernel)"),
     "real64": dict(level=0, N=64, preset="gravity_only", mask=True, b_alg=28, streams=(5, 2),
+                   b_alg2=18, streams2=(5, 4),
                    desc="REAL gravity_only 64^3 fp32, mask on"),
 }
 
@@ -395,33 +399,42 @@ def main():
         tuning = ev.autotune(chunks=(args.chunk,))
     elif not args.direct:
         ev.tile, ev.chunk = int(args.tile), args.chunk
-    step = (lambda: ev._step(direct=True)) if args.direct else ev._step
-    for _ in range(args.warmup):
+    # REAL / COMPLEX advance two steps per launch (as Evolver.evolve does)
+    pairs = getattr(ev, "two_step", False) and not args.direct and N <= 1023
+    if args.direct:
+        step = lambda: ev._step(direct=True)
+    else:
+        step = ev._step2 if pairs else ev._step
+    per = 2 if pairs else 1
+    for _ in range(max(1, args.warmup // per)):
         step()
     torch.cuda.synchronize()
+    ncalls = max(1, args.steps // per)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(ncalls):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
+    ms = (t1 - t0) * 1e3 / (ncalls * per)
 
     # kernel-only timing: events on the launch stream around the stencil kernel
-    ev.kernel_events = []
-    nk = max(10, min(args.steps, 50))
+    ev.kernel_events, ev.kernel_event_steps = [], []
+    nk = max(10, min(args.steps, 50)) // per
     for _ in range(nk):
         step()
     torch.cuda.synchronize()
-    kms = sum(a.elapsed_time(b) for a, b in ev.kernel_events) / len(ev.kernel_events)
+    kms = (sum(a.elapsed_time(b) for a, b in ev.kernel_events) / sum(ev.kernel_event_steps))
     ev.kernel_events = None
     ok = bool(torch.isfinite(ev.native_chi()).all().item())
 
     n3 = N ** 3
-    achieved = w["b_alg"] * n3 / (kms * 1e-3) / 1e9
+    b_alg = w["b_alg2"] if pairs else w["b_alg"]
+    achieved = b_alg * n3 / (kms * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    lvl = ['REAL', 'COMPLEX', 'COLOR'][w['level']]
     kname = (f"step_color_kfft<{N}>" if getattr(ev, "_fused", False)
-             else f"step_tiled<{['REAL', 'COMPLEX', 'COLOR'][w['level']]}>")
+             else f"step2_tiled<{lvl}>" if pairs else f"step_tiled<{lvl}>")
     if os.path.exists(tfile) and args.size is None and not args.direct:
         rec = json.load(open(tfile))
         if rec.get("kernel") == kname:
@@ -436,7 +449,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname,
-                     "kernel_ms": kms, "bytes_per_site": w["b_alg"]},
+                     "kernel_ms": kms, "bytes_per_site": b_alg,
+                     "steps_per_launch": 2 if pairs else 1},
         "finite": ok,
         "tile": {"shape": ev.tile, "chunk": ev.chunk,
                  "tuned_ms": {f"{t}/{c}": round(v, 4) for (t, c), v in tuning.items()} if tuning else None},
@@ -452,7 +466,7 @@ def main():
         out["cpu_baseline"] = cb
     if args.size is None and not args.direct:
         # the same access pattern with nothing but streaming: the achievable rate
-        ceil = stream_ceiling(*w["streams"], n3)
+        ceil = stream_ceiling(*(w["streams2"] if pairs else w["streams"]), n3)
         out["roofline"]["stream_ceiling"] = ceil
         out["roofline"]["frac_of_stream_ceiling"] = achieved / ceil
     print(json.dumps(out), flush=True)

@@ -91,6 +91,23 @@ int lfm_step_complex_f32(const float* pr, const float* pr_prev, const float* pi,
  * sa_clip_out: NULL, or receives max(sa, 0) (the reference's sa_fields_out).
  * psisq_out:   NULL, or receives |Psi_a'|^2 of the NEW fields (3*N^3), the
  *              input of the next step's S_a transform. */
+/* Two leapfrog steps per call (REAL / COMPLEX), rotation mode: from
+ * (Psi_n = pr/pi, Psi_{n-1} = pr_prev/pi_prev, chi_n, chi_{n-1}) it writes
+ * step n+1 into *_out1 and step n+2 into *_out2, exactly as two calls of
+ * lfm_step_real_f32 / lfm_step_complex_f32 would (flags as for the first of
+ * them; the second reads Psi_n damped).  The next pair then takes
+ * (out2, out1) as (current, prev) with LFM_F_DAMP_PREV.  Outputs must not
+ * alias any input (four Psi and four chi buffers in a rotation).
+ * LFM_F_FREEZE_CHI and LFM_F_DIRECT are not supported; N <= 1023. */
+int lfm_step2_real_f32(const float* E, const float* E_prev, const float* chi,
+                       const float* chi_prev, const float* mask, float* E_out1,
+                       float* chi_out1, float* E_out2, float* chi_out2,
+                       const lfm_params_t* p, void* stream);
+int lfm_step2_complex_f32(const float* pr, const float* pr_prev, const float* pi,
+                          const float* pi_prev, const float* chi, const float* chi_prev,
+                          const float* mask, float* pr_out1, float* pi_out1, float* chi_out1,
+                          float* pr_out2, float* pi_out2, float* chi_out2,
+                          const lfm_params_t* p, void* stream);
 int lfm_step_color_f32(const float* pr, const float* pr_prev, const float* pi,
                        const float* pi_prev, const float* chi, const float* chi_prev,
                        const float* mask, float* pr_out, float* pr_prev_out, float* pi_out,

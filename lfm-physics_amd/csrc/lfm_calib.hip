@@ -49,6 +49,9 @@ void launch_stream(const StreamArgs& A, hipStream_t st) {
         case 502: hipLaunchKernelGGL((stream_kernel<5, 2, NT>), g, b, 0, st, A); break;
         case 1003: hipLaunchKernelGGL((stream_kernel<10, 3, NT>), g, b, 0, st, A); break;
         case 101: hipLaunchKernelGGL((stream_kernel<1, 1, NT>), g, b, 0, st, A); break;
+        case 606: hipLaunchKernelGGL((stream_kernel<6, 6, NT>), g, b, 0, st, A); break;
+        case 706: hipLaunchKernelGGL((stream_kernel<7, 6, NT>), g, b, 0, st, A); break;
+        case 504: hipLaunchKernelGGL((stream_kernel<5, 4, NT>), g, b, 0, st, A); break;
         default: hipLaunchKernelGGL((stream_kernel<0, 0, NT>), g, b, 0, st, A); break;
     }
 }

@@ -25,6 +25,7 @@
 #include "lfm_fft.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace lfm {
@@ -32,6 +33,22 @@ namespace {
 
 constexpr float WF = 1.0f / 3.0f;  // face weight   (lfm/constants.py:135)
 constexpr float WE = 1.0f / 6.0f;  // edge weight   (lfm/constants.py:138)
+
+// Stencil and current arithmetic with every rounding spelled out (fmaf and
+// non-contractible products), so kernels that evaluate the same site in
+// different code (one step or two per launch) round identically.
+// In-plane partial A = WF (4 faces) + WE (4 edges) - 4 f; off-plane partial
+// B = WF f + WE (4 faces); lap(i) = (A(i) + B(i-1)) + B(i+1).
+__device__ __forceinline__ float lap_part_a(float faces, float diag, float ctr) {
+    return fmaf(WF, faces, fmaf(WE, diag, __fmul_rn(-4.0f, ctr)));
+}
+__device__ __forceinline__ float lap_part_b(float ctr, float faces) {
+    return fmaf(WF, ctr, __fmul_rn(WE, faces));
+}
+// a b - c d (one current component)
+__device__ __forceinline__ float cross2(float a, float b, float c, float d) {
+    return fmaf(a, b, -__fmul_rn(c, d));
+}
 
 enum Level { REAL = 0, COMPLEX = 1, COLOR = 2 };
 
@@ -147,11 +164,113 @@ __device__ __forceinline__ PointData<L> load_point(const KParams& kp, const Ptrs
     return d;
 }
 
-// Per-site update, shared by both kernels.
+// Per-site arithmetic of one leapfrog step, shared by every kernel.
 //  c[f]     Psi component values at the site      lap[f]  their Laplacians
 //  jsum     sum over colours and directions of j_{a,d}   (COMPLEX/COLOR)
 //  ccv      sum_d (sum_a j_{a,d}^2 - 1/3 (sum_a j_{a,d})^2)  (COLOR)
 //  chi_st, lap_chi  chi as stored and the Laplacian of the stored field
+//  scv_on   the S_a term is on (COLOR); pd.sa holds the raw S_a
+// Out: nw (new Psi), cn (new chi as stored; chi_st under LFM_F_FREEZE_CHI),
+// sclip (S_a clipped at 0, COLOR with scv_on).
+template <int L>
+__device__ __forceinline__ void site_compute(const KParams& kp, const PointData<L>& pd,
+                                             bool scv_on, const float (&c)[Lv<L>::NPF],
+                                             const float (&lap)[Lv<L>::NPF], float chi_st,
+                                             float lap_chi, float jsum, float ccv,
+                                             float (&nw)[Lv<L>::NPF], float& cn,
+                                             float (&sclip)[3]) {
+    // Every rounding is explicit (fmaf, __fmul_rn, __fadd_rn, __fsub_rn), so
+    // the result does not depend on how the compiler contracts the code in a
+    // given kernel: all kernels produce bit-identical updates for identical
+    // inputs.
+    constexpr int NPF = Lv<L>::NPF;
+    constexpr int C = Lv<L>::NC;
+    // chi_st is chi as stored: chi itself, or chi - chi0 with LFM_F_CHI_DELTA.
+    const bool delta = (kp.flags & LFM_F_CHI_DELTA) != 0;
+    const float chi_c = delta ? __fadd_rn(kp.chi0, chi_st) : chi_st;
+    const bool has_mask = (kp.flags & LFM_F_MASK) != 0;
+    const float m = has_mask ? pd.m : 0.0f;
+    const float a = __fsub_rn(1.0f, m);
+    const float chi_sq = __fmul_rn(chi_c, chi_c);
+
+    float avg[2] = {0.0f, 0.0f};
+    if constexpr (L == COLOR) {
+        if (kp.ecc > 0.0f) {
+            avg[0] = __fmul_rn(__fadd_rn(__fadd_rn(c[0], c[1]), c[2]), 1.0f / 3.0f);
+            avg[1] = __fmul_rn(__fadd_rn(__fadd_rn(c[3], c[4]), c[5]), 1.0f / 3.0f);
+        }
+    }
+    const bool damp = has_mask && (kp.flags & LFM_F_DAMP_PREV);
+#pragma unroll
+    for (int f = 0; f < NPF; ++f) {
+        float pv = pd.prev[f];
+        if (damp) pv = __fmul_rn(pv, a);
+        // 2 c - pv + dt2 (lap - chi^2 c)
+        float v = fmaf(kp.dt2, fmaf(-chi_sq, c[f], lap[f]), __fsub_rn(__fmul_rn(2.0f, c[f]), pv));
+        if constexpr (L == COLOR) {
+            if (kp.ecc > 0.0f)
+                v = fmaf(-__fmul_rn(__fmul_rn(kp.dt2, kp.ecc), chi_sq), __fsub_rn(c[f], avg[f / C]), v);
+        }
+        nw[f] = __fmul_rn(v, a);
+    }
+    cn = chi_st;
+    if (kp.flags & LFM_F_FREEZE_CHI) return;
+
+    const float kc0 = __fdiv_rn(kp.kappa, kp.chi0);
+    float psi_sq;
+    float src;
+    float cv = 0.0f, scv = 0.0f;
+    if constexpr (L == REAL) {
+        psi_sq = __fmul_rn(c[0], c[0]);
+        src = __fmul_rn(__fmul_rn(kc0, chi_c), __fsub_rn(psi_sq, kp.e0sq));
+    } else {
+        float e[C > 0 ? C : 1];
+        psi_sq = 0.0f;
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            e[q] = fmaf(c[q], c[q], __fmul_rn(c[q + C], c[q + C]));
+            psi_sq = __fadd_rn(psi_sq, e[q]);
+        }
+        const float j_total = __fmul_rn(0.5f, jsum);
+        src = __fmul_rn(__fmul_rn(kc0, chi_c), __fsub_rn(fmaf(kp.epsw, j_total, psi_sq), kp.e0sq));
+        if constexpr (L == COLOR) {
+            if (kp.kc > 0.0f) {
+                const float total_sq = __fmul_rn(psi_sq, psi_sq);
+                if (total_sq > 1e-30f) {
+                    const float sum_sq = fmaf(e[0], e[0], fmaf(e[1], e[1], __fmul_rn(e[2], e[2])));
+                    const float f_c = __fsub_rn(__fdiv_rn(sum_sq, total_sq), 1.0f / 3.0f);
+                    cv = __fmul_rn(__fmul_rn(__fmul_rn(__fdiv_rn(kp.kc, kp.chi0), chi_c), f_c), psi_sq);
+                }
+            }
+            if (scv_on) {
+                float s_sum = 0.0f, s_sq = 0.0f;
+#pragma unroll
+                for (int q = 0; q < C; ++q) {
+                    float sv = pd.sa[q];
+                    sv = sv > 0.0f ? sv : 0.0f;
+                    sclip[q] = sv;
+                    s_sum = __fadd_rn(s_sum, sv);
+                    s_sq = fmaf(sv, sv, s_sq);
+                }
+                scv = fmaf(-1.0f / 3.0f, __fmul_rn(s_sum, s_sum), s_sq);
+            }
+        }
+    }
+    // chi_self = -4 lam chi (chi^2 - chi0^2); with deviation storage the
+    // factor chi^2 - chi0^2 is formed as d (2 chi0 + d), free of cancellation.
+    const float hat = delta ? __fmul_rn(chi_st, fmaf(2.0f, kp.chi0, chi_st))
+                            : fmaf(-kp.chi0, kp.chi0, chi_sq);
+    const float chi_self = __fmul_rn(__fmul_rn(-4.0f * kp.lam, chi_c), hat);
+    float accel = __fadd_rn(__fsub_rn(__fsub_rn(lap_chi, src), cv), chi_self);
+    if constexpr (L == COLOR) accel = fmaf(-kp.kt, scv, fmaf(-kp.ks, ccv, accel));
+    // leapfrog in storage units (a constant offset drops out of 2c - c_prev and of lap)
+    float cnv = fmaf(kp.dt2, accel, __fsub_rn(__fmul_rn(2.0f, chi_st), pd.chip));
+    const float floor_v = delta ? -2.0f * kp.chi0 : -kp.chi0;
+    if (cnv < floor_v) cnv = floor_v;   // BH excision clamp chi >= -chi0 (NaN passes through)
+    cn = delta ? __fmul_rn(a, cnv) : fmaf(m, kp.chi0, __fmul_rn(a, cnv));   // frozen boundary: chi -> chi0
+}
+
+// Per-site update with its stores (site_compute, then the outputs of P).
 template <int L, class O>
 __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O idx,
                                             const PointData<L>& pd,
@@ -162,32 +281,11 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
     constexpr int NPF = Lv<L>::NPF;
     constexpr int C = Lv<L>::NC;
     const size_t n3 = P.n3;
-    // chi_st is chi as stored: chi itself, or chi - chi0 with LFM_F_CHI_DELTA.
-    const bool delta = (kp.flags & LFM_F_CHI_DELTA) != 0;
-    const float chi_c = delta ? kp.chi0 + chi_st : chi_st;
-    const bool has_mask = (kp.flags & LFM_F_MASK) != 0;
-    const float m = has_mask ? pd.m : 0.0f;
-    const float a = 1.0f - m;
-    const float chi_sq = chi_c * chi_c;
-
-    float avg[2] = {0.0f, 0.0f};
-    if constexpr (L == COLOR) {
-        if (kp.ecc > 0.0f) {
-            avg[0] = (c[0] + c[1] + c[2]) * (1.0f / 3.0f);
-            avg[1] = (c[3] + c[4] + c[5]) * (1.0f / 3.0f);
-        }
-    }
-    const bool damp = has_mask && (kp.flags & LFM_F_DAMP_PREV);
-    float nw[NPF];
+    float nw[NPF], cn, sclip[3];
+    site_compute<L>(kp, pd, P.sa != nullptr, c, lap, chi_st, lap_chi, jsum, ccv, nw, cn, sclip);
+    const float a = 1.0f - ((kp.flags & LFM_F_MASK) ? pd.m : 0.0f);
 #pragma unroll
     for (int f = 0; f < NPF; ++f) {
-        float pv = pd.prev[f];
-        if (damp) pv = pv * a;
-        float v = 2.0f * c[f] - pv + kp.dt2 * (lap[f] - chi_sq * c[f]);
-        if constexpr (L == COLOR) {
-            if (kp.ecc > 0.0f) v -= kp.dt2 * kp.ecc * chi_sq * (c[f] - avg[f / C]);
-        }
-        nw[f] = v * a;
         idx.st(field_out<L>(P.pr_out, P.pi_out, f, n3), nw[f]);
         if (P.prp_out) idx.st(field_out<L>(P.prp_out, P.pip_out, f, n3), c[f] * a);
     }
@@ -202,57 +300,12 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
         }
     }
     if (kp.flags & LFM_F_FREEZE_CHI) return;
-
-    float psi_sq;
-    float src;
-    float cv = 0.0f, scv = 0.0f;
-    if constexpr (L == REAL) {
-        psi_sq = c[0] * c[0];
-        src = (kp.kappa / kp.chi0) * chi_c * (psi_sq - kp.e0sq);
-    } else {
-        float e[C > 0 ? C : 1];
-        psi_sq = 0.0f;
+    if constexpr (L == COLOR) {
+        if (P.sa && P.sa_clip) {
 #pragma unroll
-        for (int q = 0; q < C; ++q) {
-            e[q] = c[q] * c[q] + c[q + C] * c[q + C];
-            psi_sq += e[q];
-        }
-        const float j_total = 0.5f * jsum;
-        src = (kp.kappa / kp.chi0) * chi_c * (psi_sq + kp.epsw * j_total - kp.e0sq);
-        if constexpr (L == COLOR) {
-            if (kp.kc > 0.0f) {
-                const float total_sq = psi_sq * psi_sq;
-                if (total_sq > 1e-30f) {
-                    const float sum_sq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
-                    const float f_c = sum_sq / total_sq - (1.0f / 3.0f);
-                    cv = (kp.kc / kp.chi0) * chi_c * f_c * psi_sq;
-                }
-            }
-            if (P.sa) {
-                float s_sum = 0.0f, s_sq = 0.0f;
-#pragma unroll
-                for (int q = 0; q < C; ++q) {
-                    float s = pd.sa[q];
-                    s = s > 0.0f ? s : 0.0f;
-                    if (P.sa_clip) idx.st(P.sa_clip + q * n3, s);
-                    s_sum += s;
-                    s_sq += s * s;
-                }
-                scv = s_sq - (1.0f / 3.0f) * s_sum * s_sum;
-            }
+            for (int q = 0; q < C; ++q) idx.st(P.sa_clip + q * n3, sclip[q]);
         }
     }
-    // chi_self = -4 lam chi (chi^2 - chi0^2); with deviation storage the
-    // factor chi^2 - chi0^2 is formed as d (2 chi0 + d), free of cancellation.
-    const float hat = delta ? chi_st * (2.0f * kp.chi0 + chi_st) : chi_sq - kp.chi0 * kp.chi0;
-    const float chi_self = -4.0f * kp.lam * chi_c * hat;
-    float accel = lap_chi - src - cv + chi_self;
-    if constexpr (L == COLOR) accel = accel - kp.ks * ccv - kp.kt * scv;
-    // leapfrog in storage units (a constant offset drops out of 2c - c_prev and of lap)
-    float cn = 2.0f * chi_st - pd.chip + kp.dt2 * accel;
-    const float floor_v = delta ? -2.0f * kp.chi0 : -kp.chi0;
-    if (cn < floor_v) cn = floor_v;   // BH excision clamp chi >= -chi0 (NaN passes through)
-    cn = delta ? a * cn : m * kp.chi0 + a * cn;   // frozen boundary: chi -> chi0
     idx.st(P.chi_out, cn);
     if (P.chip_out) idx.st(P.chip_out, chi_st);
 }
@@ -468,8 +521,8 @@ __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int 
                 const float faces = (nn + ss) + (ww + ee);
                 const float diag = (lds[r - 1][x - 1][f] + lds[r - 1][x + 1][f]) +
                                    (lds[r + 1][x - 1][f] + lds[r + 1][x + 1][f]);
-                A[f] = WF * faces + WE * diag - 4.0f * ctr;
-                B[f] = WF * ctr + WE * faces;
+                A[f] = lap_part_a(faces, diag, ctr);
+                B[f] = lap_part_b(ctr, faces);
                 cq[f] = ctr;
                 if (f < NPF) { dy[f] = ss - nn; dz[f] = ee - ww; }
             }
@@ -479,8 +532,8 @@ __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int 
                 float sy = 0.0f, sz = 0.0f, qy = 0.0f, qz = 0.0f;
 #pragma unroll
                 for (int a2 = 0; a2 < C; ++a2) {
-                    const float jy = cq[a2] * dy[a2 + C] - cq[a2 + C] * dy[a2];
-                    const float jz = cq[a2] * dz[a2 + C] - cq[a2 + C] * dz[a2];
+                    const float jy = cross2(cq[a2], dy[a2 + C], cq[a2 + C], dy[a2]);
+                    const float jz = cross2(cq[a2], dz[a2 + C], cq[a2 + C], dz[a2]);
                     sy += jy; sz += jz; qy += jy * jy; qz += jz * jz;
                 }
                 jyz_new = sy + sz;
@@ -498,8 +551,8 @@ __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int 
                     float sx = 0.0f, qx = 0.0f;
 #pragma unroll
                     for (int a2 = 0; a2 < C; ++a2) {
-                        const float jx = cm1[rr][a2] * (cq[a2 + C] - cm2[rr][a2 + C]) -
-                                         cm1[rr][a2 + C] * (cq[a2] - cm2[rr][a2]);
+                        const float jx = cross2(cm1[rr][a2], cq[a2 + C] - cm2[rr][a2 + C],
+                                                cm1[rr][a2 + C], cq[a2] - cm2[rr][a2]);
                         sx += jx; qx += jx * jx;
                     }
                     jsum = sx + jyz[rr];
@@ -865,16 +918,16 @@ __global__ __launch_bounds__(N) void step_color_kfft(Ptrs P, KParams kp, float2*
             const float faces = (nn + ss) + (ww + ee);
             const float diag = (S.st[0][x - 1][f] + S.st[0][x + 1][f]) +
                                (S.st[2][x - 1][f] + S.st[2][x + 1][f]);
-            A[f] = WF * faces + WE * diag - 4.0f * ctr;
-            B[f] = WF * ctr + WE * faces;
+            A[f] = lap_part_a(faces, diag, ctr);
+            B[f] = lap_part_b(ctr, faces);
             cq[f] = ctr;
             if (f < NPF) { dy[f] = ss - nn; dz[f] = ee - ww; }
         }
         float sy = 0.0f, sz = 0.0f, qy = 0.0f, qz = 0.0f;
 #pragma unroll
         for (int a2 = 0; a2 < C; ++a2) {
-            const float jy = cq[a2] * dy[a2 + C] - cq[a2 + C] * dy[a2];
-            const float jz = cq[a2] * dz[a2 + C] - cq[a2 + C] * dz[a2];
+            const float jy = cross2(cq[a2], dy[a2 + C], cq[a2 + C], dy[a2]);
+            const float jz = cross2(cq[a2], dz[a2 + C], cq[a2 + C], dz[a2]);
             sy += jy; sz += jz; qy += jy * jy; qz += jz * jz;
         }
         const float jyz_new = sy + sz;
@@ -888,8 +941,8 @@ __global__ __launch_bounds__(N) void step_color_kfft(Ptrs P, KParams kp, float2*
             float sx = 0.0f, qx = 0.0f;
 #pragma unroll
             for (int a2 = 0; a2 < C; ++a2) {
-                const float jx = cm1[a2] * (cq[a2 + C] - cm2[a2 + C]) -
-                                 cm1[a2 + C] * (cq[a2] - cm2[a2]);
+                const float jx = cross2(cm1[a2], cq[a2 + C] - cm2[a2 + C], cm1[a2 + C],
+                                        cq[a2] - cm2[a2]);
                 sx += jx; qx += jx * jx;
             }
             const float jsum = sx + jyz;
@@ -963,6 +1016,295 @@ int launch_kfft_any(const Ptrs& P, const KParams& kp, const ScvView& v, hipStrea
         case 512: return nt ? launch_kfft<512, true>(P, kp, v, st) : launch_kfft<512, false>(P, kp, v, st);
         default: set_error("fused COLOR step: N=%d unsupported", kp.N); return LFM_E_ARG;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Two leapfrog steps per launch (REAL and COMPLEX).
+//
+// A workgroup owns a 64 (k) x T2J (j) tile and marches along i.  For each
+// input plane it stages Psi_n and chi_n on the tile plus a two-cell ring,
+// computes step n -> n+1 on the tile plus a one-cell ring (the ring values
+// are what step n+1 -> n+2 needs as neighbours), keeps those in a second
+// three-plane window, and computes step n+1 -> n+2 on the tile one plane
+// behind.  Step n+1 reads Psi_n as its (damped) prev and chi_n as its chi
+// prev straight from the first window.  Per site the two steps read
+// Psi_n, Psi_{n-1}, chi_n, chi_{n-1} and the mask once and write Psi and chi
+// of both steps (the rotation keeps step n+1 as the next prev): 52 B for two
+// COMPLEX steps with mask instead of 2 x 40 B.
+//
+// Both steps use site_compute, and the Laplacian and current are summed in
+// the same order as in tile_body, so the results equal two single steps.
+// ---------------------------------------------------------------------------
+constexpr int T2K = 64, T2J = 8, T2T = 256;   // tile and workgroup size
+
+struct Ptrs2 {
+    Ptrs P;            // inputs, and step n+1 outputs in pr_out / pi_out / chi_out
+    float* pr_out2;    // step n+2
+    float* pi_out2;
+    float* chi_out2;
+};
+
+template <int L>
+struct Win2 {
+    static constexpr int NPF = Lv<L>::NPF;
+    static constexpr int NF = NPF + 1;                 // + chi
+    static constexpr int R0 = T2J + 4, C0 = T2K + 4;  // two-cell ring
+    static constexpr int R1 = T2J + 2, C1 = T2K + 2;  // one-cell ring
+    // window 0: Psi_n and chi_n (float4 for COMPLEX, float2 for REAL);
+    // window 1: Psi_{n+1}, chi_{n+1} and the mask (float4)
+    using V0 = std::conditional_t<NF == 3, float4, float2>;
+    V0 w0[3][R0][C0];
+    float4 w1[3][R1][C1];
+};
+
+__device__ __forceinline__ float fld(const float4& v, int f) {
+    return f == 0 ? v.x : f == 1 ? v.y : f == 2 ? v.z : v.w;
+}
+__device__ __forceinline__ float fld(const float2& v, int f) { return f == 0 ? v.x : v.y; }
+
+// In-plane parts of one plane at (r, c) of a window slot, for fields f < NF:
+// the partials A and B (lap_part_a / lap_part_b), the centre and the j / k
+// differences, summed exactly as tile_body sums them.
+template <int NF, class V, int RR, int CC>
+__device__ __forceinline__ void plane_parts(const V (&W)[RR][CC], int r, int c, float (&A)[NF],
+                                            float (&B)[NF], float (&ctr)[NF], float (&dj)[NF],
+                                            float (&dk)[NF]) {
+    const V nw = W[r - 1][c - 1], nc = W[r - 1][c], ne = W[r - 1][c + 1];
+    const V cw = W[r][c - 1], cc = W[r][c], ce = W[r][c + 1];
+    const V sw = W[r + 1][c - 1], sc = W[r + 1][c], se = W[r + 1][c + 1];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const float x = fld(cc, f);
+        const float nn = fld(nc, f), ss = fld(sc, f), ww = fld(cw, f), ee = fld(ce, f);
+        const float faces = (nn + ss) + (ww + ee);
+        const float diag = (fld(nw, f) + fld(ne, f)) + (fld(sw, f) + fld(se, f));
+        A[f] = lap_part_a(faces, diag, x);
+        B[f] = lap_part_b(x, faces);
+        ctr[f] = x;
+        dj[f] = ss - nn;
+        dk[f] = ee - ww;
+    }
+}
+
+// The plane-march queue of one site column (as in tile_body): fed one plane
+// at a time, it finishes the site one plane behind.
+template <int L>
+struct Queue2 {
+    static constexpr int NPF = Lv<L>::NPF, NF = NPF + 1;
+    float acc[NF], bprev[NF], cm1[NPF], cm2[NPF], chim1, jyz;
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) { acc[f] = 0.0f; bprev[f] = 0.0f; }
+#pragma unroll
+        for (int f = 0; f < NPF; ++f) { cm1[f] = 0.0f; cm2[f] = 0.0f; }
+        chim1 = 0.0f;
+        jyz = 0.0f;
+    }
+    // Feed plane q's parts; when `finish`, complete the site at plane q-1
+    // with the pointwise inputs pd into nw, cn.
+    __device__ __forceinline__ void feed(const KParams& kp, const float (&A)[NF], const float (&B)[NF],
+                                         const float (&ctr)[NF], const float (&dj)[NF],
+                                         const float (&dk)[NF], bool finish, const PointData<L>& pd,
+                                         float (&nw)[NPF], float& cn) {
+        if (finish) {
+            float lap[NPF];
+#pragma unroll
+            for (int f = 0; f < NPF; ++f) lap[f] = acc[f] + B[f];
+            const float lap_chi = acc[NPF] + B[NPF];
+            float jsum = 0.0f;
+            if constexpr (L == COMPLEX) {
+                const float sx = 0.0f + cross2(cm1[0], ctr[1] - cm2[1], cm1[1], ctr[0] - cm2[0]);
+                jsum = sx + jyz;
+            }
+            float sclip[3];
+            site_compute<L>(kp, pd, false, cm1, lap, chim1, lap_chi, jsum, 0.0f, nw, cn, sclip);
+        }
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            acc[f] = A[f] + bprev[f];
+            bprev[f] = B[f];
+        }
+#pragma unroll
+        for (int f = 0; f < NPF; ++f) { cm2[f] = cm1[f]; cm1[f] = ctr[f]; }
+        chim1 = ctr[NPF];
+        if constexpr (L == COMPLEX) {
+            const float sy = 0.0f + cross2(ctr[0], dj[1], ctr[1], dj[0]);
+            const float sz = 0.0f + cross2(ctr[0], dk[1], ctr[1], dk[0]);
+            jyz = sy + sz;
+        }
+    }
+};
+
+template <int L, bool NT>
+__device__ __forceinline__ void tile2_body(const Ptrs2& P2, const KParams& kp, int k0, int j0,
+                                           int i0, int i1) {
+    using Wn = Win2<L>;
+    using V0 = typename Wn::V0;
+    constexpr int NPF = Lv<L>::NPF;
+    constexpr int NF = Wn::NF;
+    constexpr int P0 = Wn::R0 * Wn::C0;           // staged positions of window 0
+    constexpr int P1 = Wn::R1 * Wn::C1;           // step n+1 positions
+    constexpr int U0 = (P0 + T2T - 1) / T2T;
+    constexpr int U1 = (P1 + T2T - 1) / T2T;
+    constexpr int RB = T2J / (T2T / 64);          // step n+2 rows per thread (2)
+    __shared__ Wn S;
+    const Ptrs& P = P2.P;
+    const int N = kp.N;
+    const size_t n3 = P.n3;
+    const uint32_t rowB = uint32_t(N) * 4u;
+    const uint32_t planeB = rowB * uint32_t(N);
+    const int t = threadIdx.x;
+    const int nk = min(T2K, N - k0), nj = min(T2J, N - j0);
+    auto wrap = [N](int x) { x %= N; return x < 0 ? x + N : x; };
+
+    const float* src[NF];
+#pragma unroll
+    for (int f = 0; f < NPF; ++f) src[f] = field<L>(P.pr, P.pi, f, n3);
+    src[NPF] = P.chi;
+    // window-0 staging positions (two-cell ring around the tile)
+    uint32_t o0[U0];
+#pragma unroll
+    for (int u = 0; u < U0; ++u) {
+        const int p = min(t + u * T2T, P0 - 1);
+        o0[u] = uint32_t(wrap(j0 - 2 + p / Wn::C0)) * rowB + uint32_t(wrap(k0 - 2 + p % Wn::C0)) * 4u;
+    }
+    // step n+1 positions (one-cell ring), fixed per thread across planes
+    uint32_t o1[U1];
+    int r1s[U1], c1s[U1];
+#pragma unroll
+    for (int u = 0; u < U1; ++u) {
+        const int p = min(t + u * T2T, P1 - 1);
+        r1s[u] = p / Wn::C1;
+        c1s[u] = p % Wn::C1;
+        o1[u] = uint32_t(wrap(j0 - 1 + r1s[u])) * rowB + uint32_t(wrap(k0 - 1 + c1s[u])) * 4u;
+    }
+    V0 pre[U0];
+    auto prefetch = [&](int q) {
+        const uint32_t pb = uint32_t(wrap(q)) * planeB;
+#pragma unroll
+        for (int u = 0; u < U0; ++u) {
+            float v[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) v[f] = Off32{pb + o0[u]}.ld(src[f]);
+            if constexpr (NF == 3) pre[u] = make_float4(v[0], v[1], v[2], 0.0f);
+            else pre[u] = make_float2(v[0], v[1]);
+        }
+    };
+    // pointwise inputs of step n -> n+1 at the step positions of plane q
+    PointData<L> pa[U1], pn[U1];
+    auto prefetch_pt = [&](int q) {
+        const uint32_t pb = uint32_t(wrap(q)) * planeB;
+#pragma unroll
+        for (int u = 0; u < U1; ++u) pn[u] = load_point<L>(kp, P, Off32T<NT>{pb + o1[u]});
+    };
+    const bool has_mask = (kp.flags & LFM_F_MASK) != 0;
+    KParams kpB = kp;                      // step n+1: its prev is Psi_n, read damped
+    if (has_mask) kpB.flags |= LFM_F_DAMP_PREV;
+    const int tx = t & 63, ty = t >> 6;
+    Queue2<L> qa[U1], qb[RB];
+#pragma unroll
+    for (int u = 0; u < U1; ++u) qa[u].clear();
+#pragma unroll
+    for (int rr = 0; rr < RB; ++rr) qb[rr].clear();
+
+    const int cnt = (i1 - i0) + 4;         // input planes i0-2 .. i1+1
+    prefetch(i0 - 2);
+    for (int s = 0; s < cnt; ++s) {
+        const int q = i0 - 2 + s;
+        const int sq = s % 3;              // window-0 slot of plane q (window 1 alike)
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U0; ++u) {
+            const int p = t + u * T2T;
+            if (p < P0) S.w0[sq][p / Wn::C0][p % Wn::C0] = pre[u];
+        }
+        __syncthreads();
+        if (s + 1 < cnt) prefetch(q + 1);
+        if (s >= 2) {
+            // pointwise inputs of plane q-1, loaded last iteration
+#pragma unroll
+            for (int u = 0; u < U1; ++u) pa[u] = pn[u];
+        }
+        // planes i0-1 .. i1 are centres of step n -> n+1; plane q is one next iteration
+        if (s >= 1) prefetch_pt(q);
+        // step n -> n+1: feed plane q, finish plane q-1 (written to window 1)
+        const int s1 = (s + 2) % 3;        // window-1 slot of plane q-1
+        const bool own = q - 1 >= i0 && q - 1 < i1;
+#pragma unroll
+        for (int u = 0; u < U1; ++u) {
+            const int p = t + u * T2T;
+            float A[NF], B[NF], ctr[NF], dj[NF], dk[NF], nw[NPF], cn = 0.0f;
+            plane_parts<NF>(S.w0[sq], r1s[u] + 1, c1s[u] + 1, A, B, ctr, dj, dk);
+            qa[u].feed(kp, A, B, ctr, dj, dk, s >= 2, pa[u], nw, cn);
+            if (s >= 2 && p < P1) {
+                const float m = has_mask ? pa[u].m : 0.0f;
+                if constexpr (NPF == 2) S.w1[s1][r1s[u]][c1s[u]] = make_float4(nw[0], nw[1], cn, m);
+                else S.w1[s1][r1s[u]][c1s[u]] = make_float4(nw[0], cn, m, 0.0f);
+                if (own && r1s[u] >= 1 && r1s[u] <= nj && c1s[u] >= 1 && c1s[u] <= nk) {
+                    const Off32T<NT> o{uint32_t(q - 1) * planeB + o1[u]};
+#pragma unroll
+                    for (int f = 0; f < NPF; ++f) o.st(field_out<L>(P.pr_out, P.pi_out, f, n3), nw[f]);
+                    o.st(P.chi_out, cn);
+                }
+            }
+        }
+        __syncthreads();
+        if (s >= 2) {
+            // step n+1 -> n+2: feed window-1 plane q-1, finish plane q-2
+            const int sb = (s + 1) % 3;    // window-0 and window-1 slot of plane q-2
+            const bool fin = s >= 4;
+#pragma unroll
+            for (int rr = 0; rr < RB; ++rr) {
+                const int r = ty * RB + rr;
+                float A[NF], B[NF], ctr[NF], dj[NF], dk[NF], nw[NPF], cn = 0.0f;
+                plane_parts<NF>(S.w1[s1], r + 1, tx + 1, A, B, ctr, dj, dk);
+                PointData<L> pd;
+                const V0 z = S.w0[sb][r + 2][tx + 2];
+#pragma unroll
+                for (int f = 0; f < NPF; ++f) pd.prev[f] = fld(z, f);
+                pd.chip = fld(z, NPF);
+                pd.m = fld(S.w1[sb][r + 1][tx + 1], NPF + 1);
+                qb[rr].feed(kpB, A, B, ctr, dj, dk, fin, pd, nw, cn);
+                if (fin && r < nj && tx < nk) {
+                    const Off32T<NT> o{uint32_t(q - 2) * planeB + uint32_t(j0 + r) * rowB +
+                                       uint32_t(k0 + tx) * 4u};
+#pragma unroll
+                    for (int f = 0; f < NPF; ++f) o.st(field_out<L>(P2.pr_out2, P2.pi_out2, f, n3), nw[f]);
+                    o.st(P2.chi_out2, cn);
+                }
+            }
+        }
+    }
+}
+
+template <int L, bool NT>
+__global__ __launch_bounds__(T2T) void step2_tiled(Ptrs2 P2, KParams kp, int nkt, int njt, int CH) {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int lb = (nb & 7) == 0 ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+    const int kt = lb % nkt;
+    const int rest = lb / nkt;
+    const int jt = rest % njt;
+    const int i0 = (rest / njt) * CH;
+    tile2_body<L, NT>(P2, kp, kt * T2K, jt * T2J, i0, min(i0 + CH, kp.N));
+}
+
+template <int L, bool NT>
+int launch_step2_t(const Ptrs2& P2, const KParams& kp, hipStream_t st) {
+    const int N = kp.N;
+    const int nkt = (N + T2K - 1) / T2K, njt = (N + T2J - 1) / T2J;
+    const int CH = kp.chunk > 0 ? min(kp.chunk, N) : choose_chunk(N, nkt * njt);
+    const int nch = (N + CH - 1) / CH;
+    const long long nblocks = (long long)nkt * njt * nch;
+    hipLaunchKernelGGL((step2_tiled<L, NT>), dim3((unsigned)nblocks), dim3(T2T), 0, st, P2, kp, nkt,
+                       njt, CH);
+    LFM_LAUNCH_CHECK("step2_tiled");
+    return LFM_OK;
+}
+
+template <int L>
+int launch_step2(const Ptrs2& P2, const KParams& kp, hipStream_t st) {
+    return streaming<L>(P2.P.n3) ? launch_step2_t<L, true>(P2, kp, st)
+                                 : launch_step2_t<L, false>(P2, kp, st);
 }
 
 constexpr int tile_tj(int tile) {
@@ -1096,6 +1438,81 @@ extern "C" int lfm_step_complex_f32(const float* pr, const float* pr_prev, const
     P.chi_out = chi_out; P.chip_out = chi_prev_out;
     P.n3 = n3;
     return launch<COMPLEX>(P, to_k(p), (hipStream_t)stream);
+}
+
+static int step2_check(const lfm_params_t* p, const char* fn) {
+    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI)) {
+        set_error("%s: LFM_F_DIRECT / LFM_F_FREEZE_CHI not supported", fn);
+        return LFM_E_ARG;
+    }
+    if (p->N > 1023) {
+        set_error("%s: N=%d above 1023", fn, p->N);
+        return LFM_E_ARG;
+    }
+    return LFM_OK;
+}
+
+extern "C" int lfm_step2_real_f32(const float* E, const float* E_prev, const float* chi,
+                                  const float* chi_prev, const float* mask, float* E_out1,
+                                  float* chi_out1, float* E_out2, float* chi_out2,
+                                  const lfm_params_t* p, void* stream) {
+    if (!E || !E_prev || !chi || !chi_prev || !E_out1 || !chi_out1 || !E_out2 || !chi_out2 || !p) {
+        set_error("lfm_step2_real_f32: required pointer is NULL");
+        return LFM_E_ARG;
+    }
+    if ((p->flags & LFM_F_MASK) && !mask) {
+        set_error("lfm_step2_real_f32: LFM_F_MASK set but mask is NULL");
+        return LFM_E_ARG;
+    }
+    if (int rc = step2_check(p, "lfm_step2_real_f32")) return rc;
+    const size_t n3 = size_t(p->N) * p->N * p->N;
+    const float* ins[] = {E, E_prev, chi, chi_prev, mask};
+    const size_t nin[] = {n3, n3, n3, n3, n3};
+    float* outs[] = {E_out1, chi_out1, E_out2, chi_out2};
+    const size_t nout[] = {n3, n3, n3, n3};
+    int rc = validate(p, ins, nin, 5, outs, nout, 4, "lfm_step2_real_f32");
+    if (rc) return rc;
+    Ptrs2 P2{};
+    Ptrs& P = P2.P;
+    P.pr = E; P.prp = E_prev; P.chi = chi; P.chip = chi_prev;
+    P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
+    P.pr_out = E_out1; P.chi_out = chi_out1;
+    P.n3 = n3;
+    P2.pr_out2 = E_out2; P2.chi_out2 = chi_out2;
+    return launch_step2<REAL>(P2, to_k(p), (hipStream_t)stream);
+}
+
+extern "C" int lfm_step2_complex_f32(const float* pr, const float* pr_prev, const float* pi,
+                                     const float* pi_prev, const float* chi,
+                                     const float* chi_prev, const float* mask, float* pr_out1,
+                                     float* pi_out1, float* chi_out1, float* pr_out2,
+                                     float* pi_out2, float* chi_out2, const lfm_params_t* p,
+                                     void* stream) {
+    if (!pr || !pr_prev || !pi || !pi_prev || !chi || !chi_prev || !pr_out1 || !pi_out1 ||
+        !chi_out1 || !pr_out2 || !pi_out2 || !chi_out2 || !p) {
+        set_error("lfm_step2_complex_f32: required pointer is NULL");
+        return LFM_E_ARG;
+    }
+    if ((p->flags & LFM_F_MASK) && !mask) {
+        set_error("lfm_step2_complex_f32: LFM_F_MASK set but mask is NULL");
+        return LFM_E_ARG;
+    }
+    if (int rc = step2_check(p, "lfm_step2_complex_f32")) return rc;
+    const size_t n3 = size_t(p->N) * p->N * p->N;
+    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask};
+    const size_t nin[] = {n3, n3, n3, n3, n3, n3, n3};
+    float* outs[] = {pr_out1, pi_out1, chi_out1, pr_out2, pi_out2, chi_out2};
+    const size_t nout[] = {n3, n3, n3, n3, n3, n3};
+    int rc = validate(p, ins, nin, 7, outs, nout, 6, "lfm_step2_complex_f32");
+    if (rc) return rc;
+    Ptrs2 P2{};
+    Ptrs& P = P2.P;
+    P.pr = pr; P.pi = pi; P.prp = pr_prev; P.pip = pi_prev; P.chi = chi; P.chip = chi_prev;
+    P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
+    P.pr_out = pr_out1; P.pi_out = pi_out1; P.chi_out = chi_out1;
+    P.n3 = n3;
+    P2.pr_out2 = pr_out2; P2.pi_out2 = pi_out2; P2.chi_out2 = chi_out2;
+    return launch_step2<COMPLEX>(P2, to_k(p), (hipStream_t)stream);
 }
 
 extern "C" int lfm_step_color_f32(const float* pr, const float* pr_prev, const float* pi,

@@ -64,7 +64,7 @@ class Evolver:
     """
 
     def __init__(self, config: SimulationConfig, backend="auto", mask_policy="reference",
-                 chi_storage="delta", sa_transform="auto"):
+                 chi_storage="delta", sa_transform="auto", two_step=False):
         self.config = config
         self.backend = get_backend(backend)
         self._lib = _abi.load()
@@ -84,14 +84,21 @@ class Evolver:
         self.device = dev
         n = self._n_psi * self.total
         f32 = torch.float32
-        self._pr = [torch.zeros(n, dtype=f32, device=dev) for _ in range(3)]
-        self._pi = [torch.zeros(n, dtype=f32, device=dev) for _ in range(3)] if self._has_imag else None
+        # two_step: REAL / COMPLEX advance two steps per launch (lfm_step2_*,
+        # bit-identical to single steps), which needs a fourth rotation slot:
+        # both outputs must differ from both inputs.  Off by default: measured
+        # on MI355X it reads 1.5x fewer bytes but is not yet faster (COMPLEX
+        # 512^3 0.98 vs 0.94 ms per step, REAL 64^3 10.8 vs 8.6 us).
+        self.two_step = bool(two_step) and lvl != FieldLevel.COLOR
+        ns = 4 if self.two_step else 3
+        self._pr = [torch.zeros(n, dtype=f32, device=dev) for _ in range(ns)]
+        self._pi = [torch.zeros(n, dtype=f32, device=dev) for _ in range(ns)] if self._has_imag else None
         if chi_storage not in ("delta", "absolute"):
             raise ValueError(f"chi_storage must be 'delta' or 'absolute', got {chi_storage!r}")
         self.chi_storage = chi_storage
         self._chi_off = float(np.float32(config.chi0)) if chi_storage == "delta" else 0.0
         self._chi = [torch.full((self.total,), float(config.chi0) - self._chi_off, dtype=f32,
-                                device=dev) for _ in range(3)]
+                                device=dev) for _ in range(ns)]
         self._cur, self._prev = 0, 1          # Psi slots
         self._damped = False                   # Psi prev needs (1-m) on read
         self._cc, self._cp, self._oc, self._op = 0, 1, 0, 1   # chi references
@@ -131,9 +138,11 @@ class Evolver:
         self.chunk = 0     # i planes per workgroup (0 = automatic)
         self._partials = None
         self._red_out = None
-        # Optional list: when set, each step appends a (start, end) event pair
-        # recorded on the launch stream around the stencil kernel alone.
+        # Optional list: when set, each launch appends a (start, end) event pair
+        # recorded on the launch stream around the stencil kernel alone, and
+        # kernel_event_steps the number of steps that launch advanced.
         self.kernel_events: list | None = None
+        self.kernel_event_steps: list = []
 
     # ------------------------------------------------------------------ steps
     def _p(self, flags):
@@ -157,19 +166,24 @@ class Evolver:
                                                   self._cp, self._oc, self._op))
         results = {}
         old = (self.tile, self.chunk, self.kernel_events)
+        N = self.N
+        pairs = self.two_step and N <= 1023
         if self._fused:
             # the fused COLOR kernel has one tile shape (a full k-row); tune its chunk
-            N = self.N
             combos = [(0, ch) for ch in dict.fromkeys((0, N, N // 2, N // 4, N // 8)) if ch == 0 or ch >= 4]
+        elif pairs:
+            # the two-step kernel has one tile shape; tune its chunk
+            combos = [(0, ch) for ch in dict.fromkeys((0, 8, 16, 32, 64)) if ch <= N]
         else:
             combos = [(t, ch) for t in range(1, _abi.LFM_TILE_VARIANTS + 1) for ch in chunks]
         for tile, ch in combos:
             self.tile, self.chunk = tile, ch
-            self.kernel_events = []
+            self.kernel_events, self.kernel_event_steps = [], []
             for _ in range(reps + 1):
-                self._step()
+                self._step2() if pairs else self._step()
             torch.cuda.synchronize(self.device)
-            ts = [a.elapsed_time(b) for a, b in self.kernel_events[1:]]
+            ts = [a.elapsed_time(b) / n for (a, b), n in
+                  zip(self.kernel_events[1:], self.kernel_event_steps[1:])]
             results[(tile, ch)] = sorted(ts)[len(ts) // 2]
         best = min(results, key=results.get)
         self.tile, self.chunk = best
@@ -188,10 +202,13 @@ class Evolver:
             self._sa_stale = True    # S_a of the restored state's last step: recomputed on read
         return results
 
+    def _free(self, pool, *used):
+        return next(q for q in range(len(pool)) if q not in used)
+
     def _step(self, freeze_chi: bool = False, direct: bool = False) -> None:
         lib = self._lib
         cur, prev = self._cur, self._prev
-        nxt = 3 - cur - prev
+        nxt = self._free(self._pr, cur, prev)
         flags = _abi.LFM_F_CHI_DELTA if self._chi_off else 0
         if self.has_mask:
             flags |= _abi.LFM_F_MASK
@@ -204,7 +221,7 @@ class Evolver:
         p = ctypes.byref(self._p(flags))
         st = stream_handle()
         cc, cp = self._cc, self._cp
-        cn = next(s for s in range(3) if s != cc and s != cp)
+        cn = self._free(self._chi, cc, cp)
         chi_in, chip_in = self._chi[cc].data_ptr(), self._chi[cp].data_ptr()
         chi_out = self._chi[cn].data_ptr()
         mask = self.boundary_mask.data_ptr() if self.has_mask else None
@@ -268,7 +285,46 @@ class Evolver:
         else:
             self._cc, self._cp, self._oc, self._op = cn, cc, cc, cp
 
-    def _mark(self, start=None):
+    def _step2(self) -> None:
+        """Two steps in one launch (REAL / COMPLEX, chi evolving)."""
+        lib = self._lib
+        cur, prev = self._cur, self._prev
+        n1 = self._free(self._pr, cur, prev)
+        n2 = self._free(self._pr, cur, prev, n1)
+        flags = _abi.LFM_F_CHI_DELTA if self._chi_off else 0
+        if self.has_mask:
+            flags |= _abi.LFM_F_MASK
+            if self._damped:
+                flags |= _abi.LFM_F_DAMP_PREV
+        p = ctypes.byref(self._p(flags))
+        st = stream_handle()
+        cc, cp = self._cc, self._cp
+        c1 = self._free(self._chi, cc, cp)
+        c2 = self._free(self._chi, cc, cp, c1)
+        ch = self._chi
+        mask = self.boundary_mask.data_ptr() if self.has_mask else None
+        pr, pi = self._pr, self._pi
+        ev0 = self._mark()
+        if self.level == 0:
+            rc = lib.lfm_step2_real_f32(pr[cur].data_ptr(), pr[prev].data_ptr(), ch[cc].data_ptr(),
+                                        ch[cp].data_ptr(), mask, pr[n1].data_ptr(),
+                                        ch[c1].data_ptr(), pr[n2].data_ptr(), ch[c2].data_ptr(),
+                                        p, st)
+        else:
+            rc = lib.lfm_step2_complex_f32(pr[cur].data_ptr(), pr[prev].data_ptr(),
+                                           pi[cur].data_ptr(), pi[prev].data_ptr(),
+                                           ch[cc].data_ptr(), ch[cp].data_ptr(), mask,
+                                           pr[n1].data_ptr(), pi[n1].data_ptr(), ch[c1].data_ptr(),
+                                           pr[n2].data_ptr(), pi[n2].data_ptr(), ch[c2].data_ptr(),
+                                           p, st)
+        if ev0 is not None:
+            self._mark(ev0, steps=2)
+        _abi.check(rc, "step2")
+        self._prev, self._cur = n1, n2
+        self._damped = True
+        self._cc, self._cp, self._oc, self._op = c2, c1, c1, cc
+
+    def _mark(self, start=None, steps=1):
         """Record a timing event on the current stream (kernel_events only)."""
         if self.kernel_events is None:
             return None
@@ -276,15 +332,27 @@ class Evolver:
         e.record()
         if start is not None:
             self.kernel_events.append((start, e))
+            self.kernel_event_steps.append(steps)
         return e
 
     def evolve(self, steps: int, callback=None, freeze_chi: bool = False) -> None:
         """Run `steps` leapfrog steps; callback(evolver, step) every
-        config.report_interval steps (lfm/core/evolver.py:137-180)."""
+        config.report_interval steps (lfm/core/evolver.py:137-180).  REAL and
+        COMPLEX run pairs of steps per launch when no callback falls between
+        them; the result is the same as single steps."""
         report = self.config.report_interval
-        for _ in range(int(steps)):
-            self._step(freeze_chi=freeze_chi)
-            self.step += 1
+        left = int(steps)
+        pair_ok = self.two_step and not freeze_chi and self.N <= 1023
+        while left > 0:
+            to_report = report - self.step % report if (callback is not None and report > 0) else left
+            if pair_ok and left >= 2 and to_report >= 2:
+                self._step2()
+                self.step += 2
+                left -= 2
+            else:
+                self._step(freeze_chi=freeze_chi)
+                self.step += 1
+                left -= 1
             if callback is not None and report > 0 and self.step % report == 0:
                 callback(self, self.step)
 
@@ -439,13 +507,13 @@ class Evolver:
 
     def set_chi_current(self, arr) -> None:
         """Both sets' current chi := arr; prev buffers unchanged."""
-        s = next(q for q in range(3) if q != self._cp and q != self._op)
+        s = self._free(self._chi, self._cp, self._op)
         self._chi[s].copy_(self._dev_chi(arr))
         self._cc = self._oc = s
 
     def set_chi_prev(self, arr) -> None:
         """Both sets' prev chi := arr; current buffers unchanged."""
-        s = next(q for q in range(3) if q != self._cc and q != self._oc)
+        s = self._free(self._chi, self._cc, self._oc)
         self._chi[s].copy_(self._dev_chi(arr))
         self._cp = self._op = s
 

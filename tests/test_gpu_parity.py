@@ -517,3 +517,39 @@ def test_fused_color_step_matches_separate(gpu, N):
         err = O.rel_l2(x - (19.0 if name == "get_chi" else 0.0), y - (19.0 if name == "get_chi" else 0.0))
         log(f"  {name}: rel-L2 {err:.2e}")
         assert err < 2e-6, name
+
+
+@pytest.mark.parametrize("level,N,periodic", [("REAL", 16, False), ("REAL", 40, True),
+                                              ("COMPLEX", 20, False), ("COMPLEX", 64, False),
+                                              ("COMPLEX", 72, True), ("COMPLEX", 130, False)])
+def test_two_step_kernel_equals_single_steps(gpu, level, N, periodic):
+    """Two steps per launch (lfm_step2_*) against single steps, through the
+    Evolver: an odd step count (pairs plus one single), a mid-run setter and a
+    change of chunk; ragged tiles at N = 20, 72, 130.  Bit-identical."""
+    import lfm_hip as L
+    log(f"two-step vs single {level} N={N} periodic={periodic}")
+    lvl = getattr(L.FieldLevel, level)
+    kw = dict(boundary_type=L.BoundaryType.PERIODIC) if periodic else {}
+    cfg = L.SimulationConfig(grid_size=N, field_level=lvl, lambda_self=4 / 31, **kw)
+    r, i = L.fields.gaussian_soliton(N, (N / 2.3, N / 2, N / 1.8), 4.0, 2.5, phase=0.7)
+    res = {}
+    for two in (True, False):
+        ev = L.Evolver(cfg, backend="hip", two_step=two,
+                       mask_policy="boundary_type" if periodic else "reference")
+        assert ev.two_step == two
+        ev.set_psi_real(r)
+        if lvl != L.FieldLevel.REAL:
+            ev.set_psi_imag(i)
+        ev.evolve(7)
+        ev.set_psi_real_current(ev.get_psi_real() * 0.99)
+        ev.chunk = 8
+        ev.evolve(10)
+        cur, prev = ev.get_chi_storage()
+        res[two] = (ev.get_psi_real(), ev.get_psi_imag(), cur, prev, ev.get_psi_real_prev())
+    # same arithmetic, same rounding: bit-identical fields (chi in its stored form)
+    for a, b, name in zip(res[True], res[False], ("psi_r", "psi_i", "chi", "chi_prev", "psi_r_prev")):
+        if a is None:
+            continue
+        d = float(np.abs(a.astype(np.float64) - b).max())
+        log(f"  {name}: max|diff| {d:.3e}")
+        np.testing.assert_array_equal(a, b, err_msg=name)
