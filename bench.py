@@ -345,6 +345,8 @@ def main():
     ap.add_argument("--direct", action="store_true", help="use the untiled kernel")
     ap.add_argument("--tile", default="auto", help="tile shape 1..6, 0 default, or auto (tune)")
     ap.add_argument("--chunk", type=int, default=0, help="i planes per workgroup (0 auto)")
+    ap.add_argument("--two-step", action="store_true",
+                    help="REAL / COMPLEX: two steps per launch (lfm_step2_*)")
     ap.add_argument("--sa-transform", default="auto", choices=["auto", "fused", "separate"],
                     help="COLOR S_a: k-axis passes fused into the step kernel, or five separate passes")
     args = ap.parse_args()
@@ -386,7 +388,8 @@ def main():
     cfg = make_config(w, N)
     policy = "reference" if w["mask"] else "boundary_type"
     ev = L.Evolver(cfg, backend="hip", mask_policy=policy,
-                   sa_transform=args.sa_transform if w["level"] == 2 else "auto")
+                   sa_transform=args.sa_transform if w["level"] == 2 else "auto",
+                   two_step=args.two_step)
     pr, pi = gpu_ics(w, N)
     ev.set_psi_real(pr)
     if pi is not None:

@@ -1050,34 +1050,48 @@ struct Win2 {
     static constexpr int NF = NPF + 1;                 // + chi
     static constexpr int R0 = T2J + 4, C0 = T2K + 4;  // two-cell ring
     static constexpr int R1 = T2J + 2, C1 = T2K + 2;  // one-cell ring
-    // window 0: Psi_n and chi_n (float4 for COMPLEX, float2 for REAL);
-    // window 1: Psi_{n+1}, chi_{n+1} and the mask (float4)
-    using V0 = std::conditional_t<NF == 3, float4, float2>;
-    V0 w0[3][R0][C0];
-    float4 w1[3][R1][C1];
+    // Psi (float2 for COMPLEX, float for REAL) and chi in separate arrays:
+    // lanes read consecutive 8- or 4-byte slots, free of bank conflicts.
+    using VP = std::conditional_t<NPF == 2, float2, float>;
+    VP p0[3][R0][C0];       // window 0: Psi_n
+    float c0[3][R0][C0];    //           chi_n
+    VP p1[3][R1][C1];       // window 1: Psi_{n+1}
+    float c1[3][R1][C1];    //           chi_{n+1}
+    float m1[3][R1][C1];    //           mask
 };
 
-__device__ __forceinline__ float fld(const float4& v, int f) {
-    return f == 0 ? v.x : f == 1 ? v.y : f == 2 ? v.z : v.w;
-}
 __device__ __forceinline__ float fld(const float2& v, int f) { return f == 0 ? v.x : v.y; }
+__device__ __forceinline__ float fld(float v, int) { return v; }
+__device__ __forceinline__ void put(float2& d, const float (&v)[2]) { d = make_float2(v[0], v[1]); }
+__device__ __forceinline__ void put(float& d, const float (&v)[1]) { d = v[0]; }
 
-// In-plane parts of one plane at (r, c) of a window slot, for fields f < NF:
-// the partials A and B (lap_part_a / lap_part_b), the centre and the j / k
-// differences, summed exactly as tile_body sums them.
-template <int NF, class V, int RR, int CC>
-__device__ __forceinline__ void plane_parts(const V (&W)[RR][CC], int r, int c, float (&A)[NF],
-                                            float (&B)[NF], float (&ctr)[NF], float (&dj)[NF],
-                                            float (&dk)[NF]) {
-    const V nw = W[r - 1][c - 1], nc = W[r - 1][c], ne = W[r - 1][c + 1];
-    const V cw = W[r][c - 1], cc = W[r][c], ce = W[r][c + 1];
-    const V sw = W[r + 1][c - 1], sc = W[r + 1][c], se = W[r + 1][c + 1];
+// In-plane parts of one plane at (r, c) of a window slot (Psi array P, chi
+// array X): the partials A and B (lap_part_a / lap_part_b), the centre and
+// the j / k differences of every field (Psi components, then chi), summed
+// exactly as tile_body sums them.
+template <int NF, class VP, int RR, int CC>
+__device__ __forceinline__ void plane_parts(const VP (&P)[RR][CC], const float (&X)[RR][CC], int r,
+                                            int c, float (&A)[NF], float (&B)[NF], float (&ctr)[NF],
+                                            float (&dj)[NF], float (&dk)[NF]) {
+    const VP pnw = P[r - 1][c - 1], pnc = P[r - 1][c], pne = P[r - 1][c + 1];
+    const VP pcw = P[r][c - 1], pcc = P[r][c], pce = P[r][c + 1];
+    const VP psw = P[r + 1][c - 1], psc = P[r + 1][c], pse = P[r + 1][c + 1];
+    float v[NF][9];   // nw nc ne cw cc ce sw sc se
+#pragma unroll
+    for (int f = 0; f < NF - 1; ++f) {
+        v[f][0] = fld(pnw, f); v[f][1] = fld(pnc, f); v[f][2] = fld(pne, f);
+        v[f][3] = fld(pcw, f); v[f][4] = fld(pcc, f); v[f][5] = fld(pce, f);
+        v[f][6] = fld(psw, f); v[f][7] = fld(psc, f); v[f][8] = fld(pse, f);
+    }
+    v[NF - 1][0] = X[r - 1][c - 1]; v[NF - 1][1] = X[r - 1][c]; v[NF - 1][2] = X[r - 1][c + 1];
+    v[NF - 1][3] = X[r][c - 1];     v[NF - 1][4] = X[r][c];     v[NF - 1][5] = X[r][c + 1];
+    v[NF - 1][6] = X[r + 1][c - 1]; v[NF - 1][7] = X[r + 1][c]; v[NF - 1][8] = X[r + 1][c + 1];
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-        const float x = fld(cc, f);
-        const float nn = fld(nc, f), ss = fld(sc, f), ww = fld(cw, f), ee = fld(ce, f);
+        const float x = v[f][4];
+        const float nn = v[f][1], ss = v[f][7], ww = v[f][3], ee = v[f][5];
         const float faces = (nn + ss) + (ww + ee);
-        const float diag = (fld(nw, f) + fld(ne, f)) + (fld(sw, f) + fld(se, f));
+        const float diag = (v[f][0] + v[f][2]) + (v[f][6] + v[f][8]);
         A[f] = lap_part_a(faces, diag, x);
         B[f] = lap_part_b(x, faces);
         ctr[f] = x;
@@ -1139,7 +1153,7 @@ template <int L, bool NT>
 __device__ __forceinline__ void tile2_body(const Ptrs2& P2, const KParams& kp, int k0, int j0,
                                            int i0, int i1) {
     using Wn = Win2<L>;
-    using V0 = typename Wn::V0;
+    using VP = typename Wn::VP;
     constexpr int NPF = Lv<L>::NPF;
     constexpr int NF = Wn::NF;
     constexpr int P0 = Wn::R0 * Wn::C0;           // staged positions of window 0
@@ -1178,17 +1192,13 @@ __device__ __forceinline__ void tile2_body(const Ptrs2& P2, const KParams& kp, i
         c1s[u] = p % Wn::C1;
         o1[u] = uint32_t(wrap(j0 - 1 + r1s[u])) * rowB + uint32_t(wrap(k0 - 1 + c1s[u])) * 4u;
     }
-    V0 pre[U0];
+    float pre[U0][NF];
     auto prefetch = [&](int q) {
         const uint32_t pb = uint32_t(wrap(q)) * planeB;
 #pragma unroll
-        for (int u = 0; u < U0; ++u) {
-            float v[NF];
+        for (int u = 0; u < U0; ++u)
 #pragma unroll
-            for (int f = 0; f < NF; ++f) v[f] = Off32{pb + o0[u]}.ld(src[f]);
-            if constexpr (NF == 3) pre[u] = make_float4(v[0], v[1], v[2], 0.0f);
-            else pre[u] = make_float2(v[0], v[1]);
-        }
+            for (int f = 0; f < NF; ++f) pre[u][f] = Off32{pb + o0[u]}.ld(src[f]);
     };
     // pointwise inputs of step n -> n+1 at the step positions of plane q
     PointData<L> pa[U1], pn[U1];
@@ -1216,7 +1226,13 @@ __device__ __forceinline__ void tile2_body(const Ptrs2& P2, const KParams& kp, i
 #pragma unroll
         for (int u = 0; u < U0; ++u) {
             const int p = t + u * T2T;
-            if (p < P0) S.w0[sq][p / Wn::C0][p % Wn::C0] = pre[u];
+            if (p < P0) {
+                float pv[NPF];
+#pragma unroll
+                for (int f = 0; f < NPF; ++f) pv[f] = pre[u][f];
+                put(S.p0[sq][p / Wn::C0][p % Wn::C0], pv);
+                S.c0[sq][p / Wn::C0][p % Wn::C0] = pre[u][NPF];
+            }
         }
         __syncthreads();
         if (s + 1 < cnt) prefetch(q + 1);
@@ -1234,12 +1250,12 @@ __device__ __forceinline__ void tile2_body(const Ptrs2& P2, const KParams& kp, i
         for (int u = 0; u < U1; ++u) {
             const int p = t + u * T2T;
             float A[NF], B[NF], ctr[NF], dj[NF], dk[NF], nw[NPF], cn = 0.0f;
-            plane_parts<NF>(S.w0[sq], r1s[u] + 1, c1s[u] + 1, A, B, ctr, dj, dk);
+            plane_parts<NF>(S.p0[sq], S.c0[sq], r1s[u] + 1, c1s[u] + 1, A, B, ctr, dj, dk);
             qa[u].feed(kp, A, B, ctr, dj, dk, s >= 2, pa[u], nw, cn);
             if (s >= 2 && p < P1) {
-                const float m = has_mask ? pa[u].m : 0.0f;
-                if constexpr (NPF == 2) S.w1[s1][r1s[u]][c1s[u]] = make_float4(nw[0], nw[1], cn, m);
-                else S.w1[s1][r1s[u]][c1s[u]] = make_float4(nw[0], cn, m, 0.0f);
+                put(S.p1[s1][r1s[u]][c1s[u]], nw);
+                S.c1[s1][r1s[u]][c1s[u]] = cn;
+                S.m1[s1][r1s[u]][c1s[u]] = has_mask ? pa[u].m : 0.0f;
                 if (own && r1s[u] >= 1 && r1s[u] <= nj && c1s[u] >= 1 && c1s[u] <= nk) {
                     const Off32T<NT> o{uint32_t(q - 1) * planeB + o1[u]};
 #pragma unroll
@@ -1257,13 +1273,13 @@ __device__ __forceinline__ void tile2_body(const Ptrs2& P2, const KParams& kp, i
             for (int rr = 0; rr < RB; ++rr) {
                 const int r = ty * RB + rr;
                 float A[NF], B[NF], ctr[NF], dj[NF], dk[NF], nw[NPF], cn = 0.0f;
-                plane_parts<NF>(S.w1[s1], r + 1, tx + 1, A, B, ctr, dj, dk);
+                plane_parts<NF>(S.p1[s1], S.c1[s1], r + 1, tx + 1, A, B, ctr, dj, dk);
                 PointData<L> pd;
-                const V0 z = S.w0[sb][r + 2][tx + 2];
+                const VP z = S.p0[sb][r + 2][tx + 2];
 #pragma unroll
                 for (int f = 0; f < NPF; ++f) pd.prev[f] = fld(z, f);
-                pd.chip = fld(z, NPF);
-                pd.m = fld(S.w1[sb][r + 1][tx + 1], NPF + 1);
+                pd.chip = S.c0[sb][r + 2][tx + 2];
+                pd.m = S.m1[sb][r + 1][tx + 1];
                 qb[rr].feed(kpB, A, B, ctr, dj, dk, fin, pd, nw, cn);
                 if (fin && r < nj && tx < nk) {
                     const Off32T<NT> o{uint32_t(q - 2) * planeB + uint32_t(j0 + r) * rowB +
