@@ -58,7 +58,8 @@ def stream_ceiling(nin, nout, n, reps=10):
     """HBM rate of a pure streaming kernel (lfm_calib_stream_f32) with the step
     kernel's count of planar float32 read and write streams, n floats each:
     what that access pattern reaches on this device with no stencil, halo or
-    arithmetic in the way.  Returns GB/s (best of plain and streaming stores)."""
+    arithmetic in the way.  Returns GB/s, the best over plain / non-temporal
+    accesses, 4- / 8-byte lanes and 4096 / 65536 workgroups."""
     import ctypes
     import torch
     from lfm_hip import _abi
@@ -68,13 +69,15 @@ def stream_ceiling(nin, nout, n, reps=10):
     outs = (ctypes.c_void_p * nout)(*[b.data_ptr() for b in bufs[nin:]])
     st = torch.cuda.current_stream().cuda_stream
     best = 0.0
-    for nt in (0, 1):
+    variants = [(f, g) for f in (0, 1, 2, 3) for g in (4096, 65536) if n % 2 == 0 or f < 2]
+    for flags, grid in variants:
         for _ in range(2):
-            _abi.check(lib.lfm_calib_stream_f32(ins, nin, outs, nout, n, nt, st), "lfm_calib_stream_f32")
+            _abi.check(lib.lfm_calib_stream_f32(ins, nin, outs, nout, n, flags, grid, st),
+                       "lfm_calib_stream_f32")
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
-            lib.lfm_calib_stream_f32(ins, nin, outs, nout, n, nt, st)
+            lib.lfm_calib_stream_f32(ins, nin, outs, nout, n, flags, grid, st)
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
@@ -399,7 +402,7 @@ def main():
 
     tuning = None
     if args.tile == "auto" and not args.direct:
-        tuning = ev.autotune(chunks=(args.chunk,))
+        tuning = ev.autotune(chunks=(0, 8, 16, 32, 64) if args.chunk == 0 else (args.chunk,))
     elif not args.direct:
         ev.tile, ev.chunk = int(args.tile), args.chunk
     # REAL / COMPLEX advance two steps per launch (as Evolver.evolve does)

@@ -241,11 +241,15 @@ int lfm_chi_stats_f64(const float* chi, double chi_offset, int32_t N, double bou
 
 /* Bandwidth calibration (no counterpart in the reference; a measurement aid
  * for benchmarks): out[w][t] = (w+1) * sum_r in[r][t] for t < n, nin read and
- * nout write streams of n floats (nin <= 32, 1 <= nout <= 32), streaming
- * (non-temporal) accesses when `streaming` != 0.  The host arrays `in` and
- * `out` hold device pointers. */
+ * nout write streams of n floats (nin <= 32, 1 <= nout <= 32).  flags:
+ * LFM_CALIB_STREAMING for non-temporal accesses, LFM_CALIB_VEC2 for two
+ * floats per lane (8-byte accesses; n even, pointers 8-byte aligned).  grid:
+ * workgroups of 256 lanes in the grid-stride launch (0: 16384).  The host
+ * arrays `in` and `out` hold device pointers. */
+#define LFM_CALIB_STREAMING 1
+#define LFM_CALIB_VEC2 2
 int lfm_calib_stream_f32(const float* const* in, int32_t nin, float* const* out, int32_t nout,
-                         int64_t n, int32_t streaming, void* stream);
+                         int64_t n, int32_t flags, int32_t grid, void* stream);
 
 #ifdef __cplusplus
 }

@@ -157,11 +157,11 @@ class Evolver:
             self._params[key] = p
         return p
 
-    def autotune(self, reps: int = 6, chunks=(0,)) -> dict:
-        """Time every tile shape (and the given chunk sizes) of the stencil
-        kernel on the current state and keep the fastest.  The state is
-        saved and restored, so tuning does not advance the simulation.
-        Returns {(tile, chunk): ms per kernel}."""
+    def autotune(self, reps: int = 6, chunks=(0, 8, 16, 32, 64)) -> dict:
+        """Time every tile shape and the given plane chunks (0: the library's
+        choice) of the stencil kernel on the current state and keep the
+        fastest.  The state is saved and restored, so tuning does not
+        advance the simulation.  Returns {(tile, chunk): ms per kernel}."""
         saved = ([t.clone() for t in self._pr], [t.clone() for t in self._pi] if self._pi else None,
                  [t.clone() for t in self._chi], (self._cur, self._prev, self._damped, self._cc,
                                                   self._cp, self._oc, self._op))
@@ -176,7 +176,8 @@ class Evolver:
             # the two-step kernel has one tile shape; tune its chunk
             combos = [(0, ch) for ch in dict.fromkeys((0, 8, 16, 32, 64)) if ch <= N]
         else:
-            combos = [(t, ch) for t in range(1, _abi.LFM_TILE_VARIANTS + 1) for ch in chunks]
+            combos = [(t, ch) for t in range(1, _abi.LFM_TILE_VARIANTS + 1)
+                      for ch in dict.fromkeys(min(c, N) for c in chunks)]
         for tile, ch in combos:
             self.tile, self.chunk = tile, ch
             self.kernel_events, self.kernel_event_steps = [], []

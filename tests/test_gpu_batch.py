@@ -42,16 +42,20 @@ def _configs(level, sizes, rng):
     return [mk(N, kappa=(1 / 63) * rng.uniform(0.5, 2)) for N in sizes]
 
 
-@pytest.mark.parametrize("level", [0, 1, 2])
-def test_batch_equals_single_lattice_runs(gpu, level):
+@pytest.mark.parametrize("level,group_bytes,cycles",
+                         [(0, None, 1), (1, None, 1), (2, None, 1), (0, 1 << 21, 2),
+                          (1, 1 << 21, 1), (2, 1 << 20, 2)])
+def test_batch_equals_single_lattice_runs(gpu, level, group_bytes, cycles):
     import torch
     import lfm_hip as L
     sizes = [24, 40, 16, 72, 33] if level < 2 else [16, 24, 40]
-    log(f"batch vs single level {level} sizes {sizes}")
+    log(f"batch vs single level {level} sizes {sizes} group_bytes {group_bytes} cycles {cycles}")
     rng = np.random.default_rng(11 + level)
     cfgs = _configs(level, sizes, rng)
     ics = [_ics(level, N, rng) for N in sizes]
-    batch = L.LatticeBatch(cfgs, backend="hip")
+    batch = L.LatticeBatch(cfgs, backend="hip", group_bytes=group_bytes, cycles=cycles)
+    if group_bytes is not None:
+        assert len(batch.groups) >= 3, batch.groups
     singles = []
     for b, (c, (r, i)) in enumerate(zip(cfgs, ics)):
         batch.set_psi_real(b, r)
@@ -86,22 +90,23 @@ def test_batch_equals_single_lattice_runs(gpu, level):
     assert e <= 1e-5 and e2 <= 1e-5
 
 
-def test_graph_replay_equals_eager(gpu):
+@pytest.mark.parametrize("group_bytes,cycles", [(None, 1), (1 << 20, 2)])
+def test_graph_replay_equals_eager(gpu, group_bytes, cycles):
     import torch
     import lfm_hip as L
-    log("graph replay vs eager")
+    log(f"graph replay vs eager, group_bytes {group_bytes} cycles {cycles}")
     rng = np.random.default_rng(3)
     sizes = [32, 48, 20]
     cfgs = _configs(1, sizes, rng)
     ics = [_ics(1, N, rng) for N in sizes]
     out = []
     for graph in (True, False):
-        batch = L.LatticeBatch(cfgs, backend="hip")
+        batch = L.LatticeBatch(cfgs, backend="hip", group_bytes=group_bytes, cycles=cycles)
         for b, (r, i) in enumerate(ics):
             batch.set_psi_real(b, r)
             batch.set_psi_imag(b, i)
         for rep in range(4):
-            batch.evolve(10, graph=graph)    # 1 eager + 3 replays of 3 steps each time
+            batch.evolve(10 if cycles == 1 else 14, graph=graph)   # eager steps + replays
             torch.cuda.synchronize()
             _ = batch.get_chi(0).sum()       # eager host work between replays
         out.append([batch.get_psi_real(b) for b in range(3)] + [batch.get_chi(b) for b in range(3)])

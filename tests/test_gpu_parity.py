@@ -553,3 +553,27 @@ def test_two_step_kernel_equals_single_steps(gpu, level, N, periodic):
         d = float(np.abs(a.astype(np.float64) - b).max())
         log(f"  {name}: max|diff| {d:.3e}")
         np.testing.assert_array_equal(a, b, err_msg=name)
+
+
+@pytest.mark.parametrize("flags,grid", [(0, 0), (1, 4096), (2, 65536), (3, 7)])
+def test_calibration_stream_kernel(gpu, flags, grid):
+    """The bandwidth calibration kernel computes out[w] = (w+1) * sum_r in[r]
+    for every variant (plain / non-temporal, 4- / 8-byte lanes, grid sizes)."""
+    import ctypes
+    import torch
+    from lfm_hip import _abi
+    lib = _abi.load()
+    n, nin, nout = 100_002, 7, 3
+    g = torch.Generator(device="cpu").manual_seed(5)
+    ins = [torch.rand(n, generator=g).cuda() for _ in range(nin)]
+    outs = [torch.full((n,), -1.0, device="cuda") for _ in range(nout)]
+    ia = (ctypes.c_void_p * nin)(*[t.data_ptr() for t in ins])
+    oa = (ctypes.c_void_p * nout)(*[t.data_ptr() for t in outs])
+    _abi.check(lib.lfm_calib_stream_f32(ia, nin, oa, nout, n, flags, grid,
+                                        torch.cuda.current_stream().cuda_stream), "calib")
+    torch.cuda.synchronize()
+    s = torch.zeros(n, device="cuda")
+    for t in ins:
+        s = s + t
+    for w, o in enumerate(outs):
+        np.testing.assert_array_equal(o.cpu().numpy(), (s * float(w + 1)).cpu().numpy())
