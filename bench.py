@@ -340,7 +340,8 @@ def batch_tile(tuning):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 100; batch64 500 and adjoint128 400, as specified)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="color512", choices=sorted(WORKLOADS) + ["copy", "batch64", "adjoint128"])
     ap.add_argument("--size", type=int, default=None, help="override the lattice size")
@@ -353,6 +354,8 @@ def main():
     ap.add_argument("--sa-transform", default="auto", choices=["auto", "fused", "separate"],
                     help="COLOR S_a: k-axis passes fused into the step kernel, or five separate passes")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"batch64": 500, "adjoint128": 400}.get(args.workload, 100)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the leapfrog path here runs on a single MI355X "
               "(multi-GPU slab decomposition is design-only); run with --gpus 1.",
