@@ -633,13 +633,15 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled_batch(BatchArenas A,
     tile_body<L, BJ, RJ, NT>(P, kp, d.y, d.z, d.w, min(d.w + Lb.CH, Lb.N));
 }
 
-int choose_chunk(int N, int tiles) {
+int choose_chunk(int N, int tiles, int min_ch = 8) {
     // Aim for ~8192 workgroups (32 per CU on 256 CUs) so every CU keeps
-    // several tiles in flight; keep >= 8 planes per chunk so the two halo
-    // planes a chunk re-reads cost at most a quarter of its traffic.
+    // several tiles in flight, with at least min_ch planes per chunk: 8 when
+    // the step streams from HBM (the two halo planes a chunk re-reads cost at
+    // most a quarter of its traffic), fewer when the lattice's fields stay in
+    // the last-level cache and a small lattice needs the extra workgroups.
     const int nch = (8192 + tiles - 1) / tiles;
     int ch = (N + nch - 1) / nch;
-    if (ch < 8) ch = 8;
+    if (ch < min_ch) ch = min_ch;
     return ch < N ? ch : N;
 }
 
@@ -649,7 +651,7 @@ int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
     const int N = kp.N;
     const int nkt = (N + TK - 1) / TK;
     const int njt = (N + TJ - 1) / TJ;
-    const int CH = kp.chunk > 0 ? min(kp.chunk, N) : choose_chunk(N, nkt * njt);
+    const int CH = kp.chunk > 0 ? min(kp.chunk, N) : choose_chunk(N, nkt * njt, NT ? 8 : 2);
     const int nch = (N + CH - 1) / CH;
     const long long nblocks = (long long)nkt * njt * nch;
     hipLaunchKernelGGL((step_tiled<L, BJ, RJ, NT>), dim3((unsigned)nblocks), dim3(64 * BJ), 0, st, P,

@@ -89,7 +89,7 @@ class LatticeBatch:
         self._cur, self._prev, self._chi_cur, self._chi_prev = 0, 1, 0, 1
         self._damped = False
         self.step = 0
-        self._graph = None
+        self._graphs = {}
         self._plans = []
         self.tile, self.chunk = 0, 0
         if int(cycles) < 1:
@@ -120,7 +120,7 @@ class LatticeBatch:
         groups.append((b0, self.nlat))
         self.group_bytes = group_bytes
         self.groups = groups
-        self._graph = None
+        self._graphs = {}
 
     # ------------------------------------------------------------- plan
     def _make_plan(self, tile: int, chunk: int) -> None:
@@ -148,7 +148,7 @@ class LatticeBatch:
         self._free_plan()
         self._plans = plans
         self.tile, self.chunk = tile, chunk
-        self._graph = None
+        self._graphs = {}
 
     def _free_plan(self):
         for h in self._plans:
@@ -189,7 +189,6 @@ class LatticeBatch:
                     if self._pi is not None:
                         self._view(self._pi[self._prev], b, True).mul_(a)
         self._damped = False
-        self._graph = None
 
     def _dev(self, arr) -> torch.Tensor:
         if isinstance(arr, torch.Tensor):
@@ -217,7 +216,6 @@ class LatticeBatch:
         d = self._dev(arr) - self._chi0[b]
         self._view(self._chi[self._chi_cur], b).copy_(d)
         self._view(self._chi[self._chi_prev], b).copy_(d)
-        self._graph = None
 
     def get_psi_real(self, b: int) -> np.ndarray:
         return self._view(self._pr[self._cur], b, True).cpu().numpy().reshape(self._shape(b))
@@ -306,10 +304,11 @@ class LatticeBatch:
         per = 3 * self.cycles
         reps = (steps - done) // per
         if reps:
-            if self._graph is None or self._graph_key != self._state_key():
-                self._capture()
+            g = self._graphs.get(self._state_key())
+            if g is None:
+                g = self._capture()
             for _ in range(reps):
-                self._graph.replay()
+                g.replay()
             done += per * reps
         while done < steps:
             self._step()
@@ -319,7 +318,9 @@ class LatticeBatch:
     def _state_key(self):
         return (self._cur, self._prev, self._chi_cur, self._chi_prev, self._damped)
 
-    def _capture(self) -> None:
+    def _capture(self):
+        """Capture the launches of one replay from the current (steady) state;
+        graphs are kept per starting state, whose slots fix their pointers."""
         key = self._state_key()
         saved = (self._cur, self._prev, self._chi_cur, self._chi_prev)
         s = torch.cuda.Stream(device=self.device)
@@ -337,7 +338,8 @@ class LatticeBatch:
         torch.cuda.current_stream(self.device).wait_stream(s)
         # a whole number of rotations returns every slot to where it was
         self._cur, self._prev, self._chi_cur, self._chi_prev = saved
-        self._graph, self._graph_key = g, key
+        self._graphs[key] = g
+        return g
 
     def autotune(self, steps: int = 6, tiles=None, chunks=(8, 16, 32)) -> dict:
         """Time tile shapes x plane chunks of the batched kernel on the

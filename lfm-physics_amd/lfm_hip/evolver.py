@@ -51,6 +51,10 @@ def get_backend(preference="auto"):
 class Evolver:
     """Double-buffer-equivalent leapfrog evolver on the GPU.
 
+    graphs: evolve() replays whole slot-rotation cycles from HIP graphs once
+    the rotation is steady (a graph of GRAPH_CYCLES cycles for long runs);
+    the launches are the eager ones, so the results are the same bits.
+
     mask_policy: "reference" (default) applies the cos^2 mask whatever the
     boundary type, as lfm/core/evolver.py:126 does; "boundary_type" drops it
     for BoundaryType.PERIODIC (truly periodic, mask-free kernel).
@@ -63,8 +67,10 @@ class Evolver:
     reference does.
     """
 
+    GRAPH_CYCLES = 8
+
     def __init__(self, config: SimulationConfig, backend="auto", mask_policy="reference",
-                 chi_storage="delta", sa_transform="auto", two_step=False):
+                 chi_storage="delta", sa_transform="auto", two_step=False, graphs=True):
         self.config = config
         self.backend = get_backend(backend)
         self._lib = _abi.load()
@@ -144,6 +150,10 @@ class Evolver:
         # kernel_event_steps the number of steps that launch advanced.
         self.kernel_events: list | None = None
         self.kernel_event_steps: list = []
+        # evolve() replays whole slot-rotation cycles from HIP graphs once the
+        # rotation is in its steady state (see _cycle_graph).
+        self.graphs = bool(graphs)
+        self._graphs = {}
 
     # ------------------------------------------------------------------ steps
     def _p(self, flags):
@@ -157,7 +167,7 @@ class Evolver:
             self._params[key] = p
         return p
 
-    def autotune(self, reps: int = 6, chunks=(0, 8, 16, 32, 64)) -> dict:
+    def autotune(self, reps: int = 6, chunks=(0, 2, 4, 8, 16, 32, 64)) -> dict:
         """Time every tile shape and the given plane chunks (0: the library's
         choice) of the stencil kernel on the current state and keep the
         fastest.  The state is saved and restored, so tuning does not
@@ -337,16 +347,101 @@ class Evolver:
             self.kernel_event_steps.append(steps)
         return e
 
+    # ------------------------------------------------------- graph replay
+    def _key(self):
+        return (self._cur, self._prev, self._cc, self._cp, self._oc, self._op, self._damped)
+
+    def _rotate(self, key, pair):
+        """The slot bookkeeping of one _step (pair=False, chi evolving) or one
+        _step2 (pair=True), applied to a state key without launching."""
+        cur, prev, cc, cp, oc, op, _ = key
+        n = len(self._pr)
+        free = lambda *u: next(q for q in range(n) if q not in u)
+        if pair:
+            n1 = free(cur, prev)
+            c1 = free(cc, cp)
+            return (free(cur, prev, n1), n1, free(cc, cp, c1), c1, c1, cc, True)
+        cn = free(cc, cp)
+        return (free(cur, prev), cur, cn, cc, cc, cp, True)
+
+    def _cycle_graph(self, cycles: int = 1):
+        """(graph, launches, steps) replaying `cycles` whole rotation cycles
+        from the current state, or None when the state is not steady or
+        graphs are off.  A cycle is the shortest launch sequence that returns
+        every slot reference to where it started, so a replay leaves the
+        bookkeeping unchanged.  Graphs are kept per starting state (the
+        pointers, flags and parameters of every launch are fixed by it) and
+        cycle count."""
+        if not self.graphs or self.kernel_events is not None or not self._damped:
+            return None
+        if self._cc == self._cp:
+            return None
+        if self.level == 2 and self.sa_enabled:
+            if self._scv.method != 2:
+                return None          # rocFFT launches stay out of graphs
+            # the S_a flags must be at the values every step leaves them at
+            flags = (self._psisq_valid, self._spec_valid, self._sa_stale)
+            if flags != ((False, True, True) if self._fused else (True, False, False)):
+                return None
+        pair = self.two_step and self.N <= 1023
+        key0 = self._key()
+        k, m = key0, 0
+        while True:
+            k, m = self._rotate(k, pair), m + 1
+            if k == key0 or m > 6:
+                break
+        if k != key0:
+            return None
+        m *= cycles
+        gkey = key0 + (self.tile, self.chunk, pair, cycles)
+        hit = self._graphs.get(gkey)
+        if hit is not None:
+            return hit
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                for _ in range(m):
+                    self._step2() if pair else self._step()
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        # Capture recorded the launches without running them.  The bookkeeping
+        # it advanced is back at the start (a whole cycle) and the S_a flags
+        # are at their per-step values, so the state still describes the
+        # buffers.
+        if self._key() != key0:
+            raise RuntimeError("slot rotation did not return to its start after a captured cycle")
+        hit = (g, m, 2 * m if pair else m)
+        self._graphs[gkey] = hit
+        return hit
+
     def evolve(self, steps: int, callback=None, freeze_chi: bool = False) -> None:
         """Run `steps` leapfrog steps; callback(evolver, step) every
         config.report_interval steps (lfm/core/evolver.py:137-180).  REAL and
         COMPLEX run pairs of steps per launch when no callback falls between
-        them; the result is the same as single steps."""
+        them; the result is the same as single steps.  Once the slot rotation
+        is steady, whole rotation cycles are replayed from HIP graphs (same
+        launches, so the same bits as eager stepping)."""
         report = self.config.report_interval
         left = int(steps)
         pair_ok = self.two_step and not freeze_chi and self.N <= 1023
         while left > 0:
             to_report = report - self.step % report if (callback is not None and report > 0) else left
+            cyc = None if freeze_chi else self._cycle_graph()
+            if cyc is not None and min(left, to_report) >= cyc[2]:
+                # long runs replay a graph of several cycles (fewer host
+                # launches per step), then single cycles
+                big = self._cycle_graph(self.GRAPH_CYCLES) if min(left, to_report) >= \
+                    self.GRAPH_CYCLES * cyc[2] else None
+                for g, _, n in ((big or cyc), cyc):
+                    while min(left, to_report) >= n:
+                        g.replay()
+                        self.step += n
+                        left -= n
+                        to_report -= n
+                if callback is not None and report > 0 and self.step % report == 0:
+                    callback(self, self.step)
+                continue
             if pair_ok and left >= 2 and to_report >= 2:
                 self._step2()
                 self.step += 2
