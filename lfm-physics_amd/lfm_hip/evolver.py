@@ -472,6 +472,25 @@ class Evolver:
     def native_chi_prev(self) -> torch.Tensor:
         return self._chi[self._cp]
 
+    def native_chi_refs(self) -> tuple:
+        """The four reference chi buffers (current set's chi, other set's chi,
+        current set's prev, other set's prev) as device views, in storage
+        units; some may alias one slot."""
+        ch = self._chi
+        return ch[self._cc], ch[self._oc], ch[self._cp], ch[self._op]
+
+    def use_absolute_chi(self) -> None:
+        """Switch the chi buffers to absolute storage (the reference's
+        representation), in place, so that code writing absolute chi through
+        native views sees what it expects.  No-op when already absolute."""
+        if not self._chi_off:
+            return
+        for t in self._chi:
+            t.add_(self._chi_off)
+        self._chi_off = 0.0
+        self.chi_storage = "absolute"
+        self._graphs.clear()
+
     def chi_tensor(self) -> torch.Tensor:
         """Absolute chi as a new device tensor."""
         return self._chi[self._cc] + self._chi_off if self._chi_off else self._chi[self._cc].clone()

@@ -1,8 +1,10 @@
 """``Simulation``: the facade over :class:`Evolver` (lfm/simulation.py:47-1565).
 
 Kept: field properties, ``run`` (metrics every ``report_interval``, the
-``evolve_chi=False`` freeze), ``metrics``, ``total_energy``, S_a access and
-npz checkpoints with the reference's keys.  Metrics reductions run in FP64 on
+``evolve_chi=False`` freeze), ``run_with_snapshots`` (snapshots and per-step
+hooks), the zero-copy ``_native_*`` device views the reference's experiment
+hooks write through, ``metrics``, ``total_energy``, S_a access and npz
+checkpoints with the reference's keys.  Metrics reductions run in FP64 on
 the GPU; only the cluster count (scipy.ndimage.label) pulls |Psi|^2 to the
 host, as the reference does.  Initial-condition helpers and experiment
 drivers of the reference are outside this package.
@@ -86,6 +88,51 @@ class Simulation:
     @sa_fields.setter
     def sa_fields(self, value) -> None:
         self._evolver.set_sa_fields(value)
+
+    @property
+    def energy_density(self) -> np.ndarray:
+        """|Psi|^2 summed over components, (N, N, N) float32."""
+        return self._evolver.get_energy_density()
+
+    # -- zero-copy device views (lfm/simulation.py:209-252) -----------------
+    # Live views of the evolver's active buffers (torch tensors on the GPU);
+    # writes through them are seen by the next step, as the reference's
+    # experiment hooks (barriers, sources, detectors) expect.
+    def _psi_shape(self):
+        N = self.config.grid_size
+        return (self._evolver._n_psi, N, N, N) if self._evolver.level == 2 else (N, N, N)
+
+    def _native_psi_real(self):
+        """Current Psi real buffer as a device view, (N, N, N) or (3, N, N, N)."""
+        return self._evolver.native_psi_real().view(self._psi_shape())
+
+    def _native_psi_imag(self):
+        """Current Psi imaginary buffer as a device view (None at REAL)."""
+        pi = self._evolver.native_psi_imag()
+        return None if pi is None else pi.view(self._psi_shape())
+
+    def _native_chi(self):
+        """Current chi buffer as an (N, N, N) device view in absolute units.
+        The first call switches the evolver to absolute chi storage, the
+        reference's representation, so writes of absolute values are right."""
+        ev = self._evolver
+        ev.use_absolute_chi()
+        N = self.config.grid_size
+        return ev.native_chi().view(N, N, N)
+
+    def _native_chi_pair(self):
+        """The four reference chi buffers as (N, N, N) device views, absolute
+        units: (current, other set's current, current prev, other set's prev).
+        Writing one state into all four zeroes the chi velocity there, as
+        barrier enforcement does (lfm/experiment/barrier.py:295-330)."""
+        ev = self._evolver
+        ev.use_absolute_chi()
+        N = self.config.grid_size
+        return tuple(t.view(N, N, N) for t in ev.native_chi_refs())
+
+    def _to_device(self, arr):
+        """A host array as a flat float32 device tensor (the backend's format)."""
+        return self._evolver.backend.from_numpy(np.asarray(arr, np.float32).ravel())
 
     def get_chi(self):
         return self._evolver.get_chi()
@@ -233,6 +280,57 @@ class Simulation:
             self._snapshot()
 
         self._evolver.evolve(steps, callback=_cb, freeze_chi=not evolve_chi)
+
+    def run_with_snapshots(self, steps: int, snapshot_every: int = 100, fields=None,
+                           callback=None, step_callback=None, record_metrics: bool = True,
+                           evolve_chi: bool = True) -> list[dict]:
+        """Run and keep host copies of `fields` (any of "chi", "psi_real",
+        "psi_imag", "energy_density"; default ["chi"]) every `snapshot_every`
+        steps and at the end (lfm/simulation.py:1253-1395).  `callback` fires
+        after each batch; `step_callback(sim, step)` fires after every step
+        (report_interval is 1 meanwhile, and metrics are then recorded once
+        per batch instead of every report_interval)."""
+        self._auto_equilibrate()
+        fields = ["chi"] if fields is None else fields
+        snapshots: list[dict] = []
+        n_full, remainder = divmod(int(steps), int(snapshot_every))
+
+        def take():
+            snap = {"step": self.step}
+            if "chi" in fields:
+                snap["chi"] = self.chi.copy()
+            if "psi_real" in fields:
+                snap["psi_real"] = self.psi_real.copy()
+            if "psi_imag" in fields and self.psi_imag is not None:
+                snap["psi_imag"] = self.psi_imag.copy()
+            if "energy_density" in fields:
+                snap["energy_density"] = self.energy_density.copy()
+            return snap
+
+        batches = [snapshot_every] * n_full + ([remainder] if remainder > 0 else [])
+        if step_callback is not None:
+            orig = self.config.report_interval
+            self.config.report_interval = 1
+            try:
+                for n in batches:
+                    self.run(n, callback=step_callback, record_metrics=False,
+                             evolve_chi=evolve_chi)
+                    if callback is not None:
+                        callback(self, self.step)
+                    snapshots.append(take())
+                    if record_metrics:
+                        m = self.metrics()
+                        m["step"] = float(self.step)
+                        self._history.append(m)
+            finally:
+                self.config.report_interval = orig
+        else:
+            for n in batches:
+                self.run(n, callback=None, record_metrics=record_metrics, evolve_chi=evolve_chi)
+                if callback is not None:
+                    callback(self, self.step)
+                snapshots.append(take())
+        return snapshots
 
     # -- analysis -------------------------------------------------------------
     def get_interior_mask(self) -> np.ndarray:
