@@ -82,6 +82,14 @@ class Simulation:
             self._evolver.get_psi_real().shape)
 
     @property
+    def psi_imag_prev(self):
+        """Psi imaginary part at the last run() start or callback (the
+        reference's _psi_i_prev), None at REAL."""
+        if self._snap_i is None:
+            return None
+        return self._snap_i.cpu().numpy().reshape(self._evolver.get_psi_imag().shape)
+
+    @property
     def sa_fields(self):
         return self._evolver.get_sa_fields()
 
@@ -161,6 +169,14 @@ class Simulation:
     def set_chi(self, v):
         self._evolver.set_chi(v)
 
+    def set_psi_real_current(self, v):
+        """Only the active current Psi_r buffer (prev untouched), for driven
+        sources in step callbacks (lfm/simulation.py:190-197)."""
+        self._evolver.set_psi_real_current(v)
+
+    def set_psi_imag_current(self, v):
+        self._evolver.set_psi_imag_current(v)
+
     # -- initial conditions -------------------------------------------------
     def place_soliton(self, position, amplitude=None, sigma=None, phase=0.0, velocity=None):
         """Add a Gaussian soliton to Psi (lfm/simulation.py:495-625).  At the
@@ -211,6 +227,32 @@ class Simulation:
         self._equilibrated = False
         if velocity is not None:
             self._velocity_boosts.append((tuple(velocity), env ** 2))
+
+    def place_solitons(self, positions, amplitude=None, sigma=None, phases=None, colors=None):
+        """Add several Gaussian solitons (lfm/simulation.py:724-778): at the
+        COLOR level soliton s goes to colour colors[s] (round robin by
+        default); otherwise all are summed into the single component."""
+        from .fields import gaussian_soliton, place_solitons
+        cfg = self.config
+        N = cfg.grid_size
+        amp = amplitude if amplitude is not None else cfg.e_amplitude
+        sig = sigma if sigma is not None else cfg.sigma
+        if cfg.field_level == FieldLevel.COLOR:
+            pr, pi = place_solitons(N, positions, amp, sig, phases=phases, colors=colors,
+                                    n_colors=cfg.n_colors)
+        else:
+            pr = np.zeros((N, N, N), np.float32)
+            pi = np.zeros((N, N, N), np.float32)
+            for pos, ph in zip(positions, [0.0] * len(positions) if phases is None else phases):
+                r, i = gaussian_soliton(N, pos, amp, sig, ph)
+                pr += r
+                pi += i
+        ev = self._evolver
+        ev.set_psi_real(ev.get_psi_real() + pr)
+        if cfg.field_level != FieldLevel.REAL:
+            ev.set_psi_imag(ev.get_psi_imag() + pi)
+        self._solitons_placed = True
+        self._equilibrated = False
 
     def equilibrate(self) -> None:
         """Poisson-equilibrate chi from |Psi|^2 (GOV-04) on the GPU
@@ -280,6 +322,32 @@ class Simulation:
             self._snapshot()
 
         self._evolver.evolve(steps, callback=_cb, freeze_chi=not evolve_chi)
+
+    def run_driven(self, steps: int, chi_forcing, record_metrics: bool = False) -> None:
+        """Overwrite chi with chi_forcing(t) (a scalar or an (N, N, N) array or
+        device tensor) before every step, t = step * dt (lfm/simulation.py:
+        1190-1251); metrics every report_interval steps when record_metrics."""
+        ev = self._evolver
+        N = self.config.grid_size
+        dt = self.config.dt
+        base = self.step
+        self._equilibrated = True
+        self._snapshot()
+        for s in range(int(steps)):
+            chi = chi_forcing((base + s) * dt)
+            if not isinstance(chi, torch.Tensor):
+                chi = np.asarray(chi, dtype=np.float32)
+                if chi.ndim == 0:
+                    chi = np.full((N, N, N), float(chi), dtype=np.float32)
+            elif chi.ndim == 0:
+                chi = torch.full((N, N, N), float(chi), dtype=torch.float32, device=ev.device)
+            ev.set_chi(chi)
+            ev.evolve(1)
+            self._snapshot()
+            if record_metrics and (base + s + 1) % self.config.report_interval == 0:
+                m = self.metrics()
+                m["step"] = float(base + s + 1)
+                self._history.append(m)
 
     def run_with_snapshots(self, steps: int, snapshot_every: int = 100, fields=None,
                            callback=None, step_callback=None, record_metrics: bool = True,

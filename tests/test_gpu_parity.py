@@ -707,3 +707,35 @@ def test_run_with_snapshots_and_native_hooks_match_oracle(gpu):
             assert e <= TOL64 and ec <= TOL64, (step, e, ec)
             k += 1
     assert k == len(snaps)
+
+
+def test_run_driven_matches_oracle(gpu):
+    """run_driven (lfm/simulation.py:1190-1251): chi overwritten by a
+    time-periodic scalar forcing before every step, against the FP64 oracle
+    doing the same (all four chi buffers := forcing, then one step)."""
+    import math
+    import lfm_hip as L
+    N, steps = 24, 40
+    log(f"run_driven COMPLEX {N}^3")
+    cfg = L.gravity_em(N)
+    cfg.report_interval = 10
+    sim = L.Simulation(cfg, backend="hip")
+    sim.place_solitons([(8.0, 12.0, 12.0), (16.0, 12.0, 12.5)], amplitude=3.0, sigma=2.5,
+                       phases=[0.0, 1.0])
+    r, i = sim.psi_real.copy(), sim.psi_imag.copy()
+    force = lambda t: cfg.chi0 + 0.5 * math.sin(7.0 * t)
+    sim.run_driven(steps, force, record_metrics=True)
+    assert sim.step == steps and len(sim.history) == steps // 10
+    st = O.OracleState(1, N, r, i, np.full(N ** 3, cfg.chi0), O.boundary_mask(N, cfg.boundary_fraction),
+                       oracle_params(cfg))
+    for s in range(steps):
+        v = float(np.float32(force(s * cfg.dt)))
+        for buf in (st.c[0], st.c[1], st.cp[0], st.cp[1]):
+            buf.fill(v)
+        st.step(1)
+    norm0 = float(np.linalg.norm(np.concatenate([r.ravel(), i.ravel()]).astype(np.float64)))
+    psi = np.concatenate([sim.psi_real.ravel(), sim.psi_imag.ravel()])
+    e = float(np.linalg.norm(psi - np.concatenate([st.psi_r, st.psi_i])) / norm0)
+    ec = O.rel_l2(sim.chi, st.chi)
+    log(f"  psi error / |psi_0| {e:.2e}, chi rel L2 {ec:.2e}")
+    assert e <= TOL64 and ec <= TOL64
