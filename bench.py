@@ -366,6 +366,54 @@ def run_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def run_relax(args):
+    """Eigenmode relaxation at the reference's default N = 64: the Gaussian
+    seed (amplitude 14, sigma 4), one Poisson chi, then `steps` imaginary-time
+    steps (each: stencil update with the max|E| fold, then the rescale) with a
+    convergence check every 200 steps and an unreachable tolerance, so every
+    step runs.  CPU baseline: the oracle's float32 NumPy relaxation step."""
+    import numpy as np
+    import torch
+    from lfm_hip.eigenmode import _Relaxer, gaussian_seed
+    from lfm_hip.evolver import get_backend
+    N = args.size or 64
+    steps = args.steps
+    chi0 = 19.0
+    dt = float(np.float32(0.8 / (chi0 * chi0 + 12.0)))
+    R = _Relaxer(gaussian_seed(N, (N // 2,) * 3, 14.0, max(3.0, N / 16.0)), N, chi0, 1 / 63, 0.3,
+                 get_backend("hip"))
+    R.poisson()
+    R.relax(14.0, dt, max(args.warmup, 1), 200, -1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    R.relax(14.0, dt, steps, 200, -1.0)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    out = {"metric": METRIC, "value": N ** 3 * steps / t / 1e9, "unit": "GLUPS", "n_gpus": 1,
+           "steps": steps, "warmup": args.warmup, "ms_per_step": t * 1e3 / steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": f"eigenmode relaxation {N}^3 (imaginary-time steps, "
+                                  "max renormalisation, check every 200)", "grid": N},
+           "finite": bool(torch.isfinite(R.E).all())}
+    if not args.no_cpu_baseline:
+        from oracle import lfm_oracle as O
+        E = R.E.cpu().numpy().reshape(N, N, N).copy()
+        chi2 = (R.chi2.cpu().numpy().reshape(N, N, N)).astype(np.float32)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0:
+            En = E + np.float32(dt) * (O.laplacian_19pt(E) - chi2 * E)
+            En *= np.float32(14.0 / float(np.max(np.abs(En))))
+            E = En
+            n += 1
+        dt_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": N ** 3 * n / dt_cpu / 1e9, "unit": "GLUPS", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle NumPy float32 relaxation step {N}^3, {n} steps "
+                                         f"in {dt_cpu:.1f} s (one thread)"}
+    print(json.dumps(out), flush=True)
+
+
 def batch_tile(tuning):
     if not tuning:
         return None
@@ -377,9 +425,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default: 100; batch64 500 and adjoint128 400, as specified)")
+                    help="timed steps (default: 100; batch64 500, adjoint128 400, relax64 1000)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="color512", choices=sorted(WORKLOADS) + ["copy", "batch64", "adjoint128"])
+    ap.add_argument("--workload", default="color512",
+                    choices=sorted(WORKLOADS) + ["copy", "batch64", "adjoint128", "relax64"])
     ap.add_argument("--size", type=int, default=None, help="override the lattice size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--direct", action="store_true", help="use the untiled kernel")
@@ -391,7 +440,7 @@ def main():
                     help="COLOR S_a: k-axis passes fused into the step kernel, or five separate passes")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"batch64": 500, "adjoint128": 400}.get(args.workload, 100)
+        args.steps = {"batch64": 500, "adjoint128": 400, "relax64": 1000}.get(args.workload, 100)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the leapfrog path here runs on a single MI355X "
               "(multi-GPU slab decomposition is design-only); run with --gpus 1.",
@@ -406,6 +455,9 @@ def main():
         return
     if args.workload == "adjoint128":
         run_adjoint(args)
+        return
+    if args.workload == "relax64":
+        run_relax(args)
         return
     if args.workload == "copy":
         # Device-to-device copy of a known size: the HBM calibration for the

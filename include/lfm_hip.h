@@ -239,6 +239,25 @@ int lfm_chi_stats_f64(const float* chi, double chi_offset, int32_t N, double bou
                       double well_threshold, double void_threshold, double* partials,
                       double* out, void* stream);
 
+/* Eigenmode relaxation (lfm/particles/solver.py:620-826, the imaginary-time
+ * loop of relax_eigenmode).  One step: out = E + dt (lap E - chi2 E) with the
+ * periodic 19-point Laplacian, and max |out| folded into max_slot.  A slot is
+ * LFM_RELAX_SLOT_WORDS device words (the max is spread over 64 of them, one
+ * per 128-byte line, holding non-negative float bits); the caller zeroes a
+ * slot before its first use.  3 <= N <= 2048; out must not overlap E or
+ * chi2. */
+#define LFM_RELAX_SLOT_WORDS 2048
+int lfm_relax_step_f32(const float* E, const float* chi2, float* out, uint32_t* max_slot,
+                       int32_t N, float dt, void* stream);
+/* The step's renormalisation: E *= amplitude / max (the max held in
+ * max_slot) when max > 1e-8.  When E_prev is not NULL, max |E - E_prev| after
+ * the rescale is folded into diff_slot (zeroed by the caller; E_prev and
+ * diff_slot go together).  clear_slot, when not NULL, is zeroed (the next
+ * step's max slot; it must not overlap max_slot or diff_slot). */
+int lfm_relax_scale_f32(float* E, const float* E_prev, const uint32_t* max_slot,
+                        uint32_t* clear_slot, uint32_t* diff_slot, float amplitude, int64_t n,
+                        void* stream);
+
 /* Bandwidth calibration (no counterpart in the reference; a measurement aid
  * for benchmarks): out[w][t] = (w+1) * sum_r in[r][t] for t < n, nin read and
  * nout write streams of n floats (nin <= 32, 1 <= nout <= 32).  flags:

@@ -23,6 +23,7 @@
 //   chi' = max(chi', -chi0); chi' = m chi0 + (1-m) chi'; Psi' *= (1-m).
 #include "lfm_internal.h"
 #include "lfm_fft.h"
+#include "lfm_stencil.h"
 
 #include <algorithm>
 #include <type_traits>
@@ -31,21 +32,8 @@
 namespace lfm {
 namespace {
 
-constexpr float WF = 1.0f / 3.0f;  // face weight   (lfm/constants.py:135)
-constexpr float WE = 1.0f / 6.0f;  // edge weight   (lfm/constants.py:138)
-
-// Stencil and current arithmetic with every rounding spelled out (fmaf and
-// non-contractible products), so kernels that evaluate the same site in
-// different code (one step or two per launch) round identically.
-// In-plane partial A = WF (4 faces) + WE (4 edges) - 4 f; off-plane partial
-// B = WF f + WE (4 faces); lap(i) = (A(i) + B(i-1)) + B(i+1).
-__device__ __forceinline__ float lap_part_a(float faces, float diag, float ctr) {
-    return fmaf(WF, faces, fmaf(WE, diag, __fmul_rn(-4.0f, ctr)));
-}
-__device__ __forceinline__ float lap_part_b(float ctr, float faces) {
-    return fmaf(WF, ctr, __fmul_rn(WE, faces));
-}
-// a b - c d (one current component)
+// Current arithmetic with every rounding spelled out, as the Laplacian
+// partials in lfm_stencil.h: a b - c d (one current component)
 __device__ __forceinline__ float cross2(float a, float b, float c, float d) {
     return fmaf(a, b, -__fmul_rn(c, d));
 }

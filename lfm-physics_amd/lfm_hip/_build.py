@@ -18,7 +18,7 @@ LIB_NAME = "liblfm_hip.so"
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 SOURCES = ["lfm_abi.hip", "lfm_step.hip", "lfm_scv.hip", "lfm_reduce.hip", "lfm_adjoint.hip",
-           "lfm_calib.hip"]
+           "lfm_calib.hip", "lfm_relax.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -70,3 +70,14 @@ def boundary_mask_host(N: int, boundary_fraction: float) -> np.ndarray:
         t = np.clip((R - r_freeze) / (r_max - r_freeze), 0.0, 1.0)
         out[i] = (np.sin(0.5 * np.pi * t) ** 2).astype(np.float32)
     return out.ravel()
+
+
+def laplacian_19pt_host(f: np.ndarray) -> np.ndarray:
+    """Periodic 19-point Laplacian of an (N, N, N) host array: faces 1/3,
+    edges 1/6, centre -4 (lfm/core/stencils.py:26-71), for host-side
+    analysis such as the relaxation's eigenvalue estimate."""
+    sh = lambda a, s, ax: np.roll(a, s, axis=ax)
+    faces = sum(sh(f, s, ax) for ax in range(3) for s in (1, -1))
+    edges = sum(sh(sh(f, s1, a1), s2, a2) for a1, a2 in ((0, 1), (0, 2), (1, 2))
+                for s1 in (1, -1) for s2 in (1, -1))
+    return faces / 3.0 + edges / 6.0 - 4.0 * f

@@ -32,6 +32,12 @@ Pinning (what checks each function independently of this restatement):
 * ``energy_components``-- plane-wave closed forms; np.gradient is the NumPy
                           library routine the reference calls.
 * ``chi_statistics`` / ``interior_mask`` -- counted by brute force.
+* ``relax_*``          -- a plane wave in uniform chi is a fixed shape of the
+                          rescaled imaginary-time step and its eigenvalue is
+                          sqrt(chi^2 - lambda(k)); a mode's amplitude against
+                          the k = 0 mode decays by the closed-form factor
+                          (1 + dt (lambda(k) - chi^2)) / (1 - dt chi^2) per
+                          step; shell mask counted by brute force.
 """
 
 from __future__ import annotations
@@ -381,6 +387,86 @@ def poisson_chi(psi_sq, N, chi0, kappa, e0_sq=0.0, frozen_bf=None):
     if frozen_bf is not None:
         chi[~interior_mask(N, frozen_bf)] = chi0
     return chi
+
+
+# ---------------------------------------------------------------------------
+# Eigenmode relaxation (lfm/particles/solver.py:454-534,620-826,828-999)
+# ---------------------------------------------------------------------------
+def relax_shell_mask(N, boundary_fraction=0.3):
+    """True in the frozen outer shell: r >= (N/2)(1 - bf) about the centre
+    (N-1)/2, with the reference's float32 coordinates
+    (particles/solver.py:517-524)."""
+    c = (N - 1) / 2.0
+    x = np.arange(N, dtype=np.float32)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    r = np.sqrt((X - c) ** 2 + (Y - c) ** 2 + (Z - c) ** 2)
+    return r >= (N * 0.5) * (1.0 - boundary_fraction)
+
+
+def relax_seed(N, position, amplitude, sigma):
+    """l = 0 seed amplitude * exp(-r^2 / sigma^2), computed in float64 and
+    stored as float32 (particles/solver.py:917-922)."""
+    idx = np.arange(N, dtype=np.float64)
+    X, Y, Z = np.meshgrid(idx, idx, idx, indexing="ij")
+    cx, cy, cz = position
+    r2 = ((X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2) / (sigma ** 2)
+    return (amplitude * np.exp(-r2)).astype(np.float32)
+
+
+def relax_chi(E, N, chi0, kappa, shell):
+    """Poisson chi of one relaxation cycle: chi0 + (-kappa E^2 / k^2, DC
+    dropped), clipped to [0.1, chi0], chi0 on the shell
+    (particles/solver.py:760-772)."""
+    chi = poisson_chi(np.asarray(E, np.float64) ** 2, N, chi0, kappa)
+    chi = np.clip(np.where(chi < 0.0, 0.1, chi), 0.1, chi0)
+    chi[shell] = chi0
+    return chi
+
+
+def relax_eigenmode(E, N, chi0, kappa, amplitude, dt_E, max_cycles, steps_per_cycle,
+                    check_interval, tolerance, boundary_fraction, dtype=np.float64):
+    """Poisson / imaginary-time relaxation cycles of particles/solver.py:742-826:
+    per cycle chi from E, then up to steps_per_cycle steps of
+    E <- E + dt_E (lap E - chi^2 E) rescaled to max|E| = amplitude, stopping
+    early when max|dE| < tolerance at a check; converged when a whole cycle
+    changes E by less than tolerance (after the first cycle).  Returns
+    (E, chi, converged, total_steps, cycle) with a final Poisson chi."""
+    shell = relax_shell_mask(N, boundary_fraction)
+    E = np.asarray(E, dtype=dtype).reshape(N, N, N).copy()
+    converged, total, cycle = False, 0, 0
+    for cycle in range(max_cycles):
+        chi = relax_chi(E, N, chi0, kappa, shell).astype(dtype)
+        chi2 = chi * chi
+        E_prev = E.copy()
+        e_conv = False
+        for step in range(1, steps_per_cycle + 1):
+            E_new = E + dt_E * (laplacian_19pt(E) - chi2 * E)
+            m = float(np.max(np.abs(E_new)))
+            if m > 1e-8:
+                E_new *= amplitude / m
+            if step % check_interval == 0:
+                if float(np.max(np.abs(E_new - E))) < tolerance:
+                    e_conv = True
+                    E = E_new
+                    total += step
+                    break
+            E = E_new
+        if not e_conv:
+            total += steps_per_cycle
+        if float(np.max(np.abs(E - E_prev))) < tolerance and cycle > 0:
+            converged = True
+            break
+    chi = relax_chi(E, N, chi0, kappa, shell)
+    return E, chi, converged, total, cycle
+
+
+def relax_eigenvalue(E, chi):
+    """omega = sqrt(<E|(-lap + chi^2)|E> / <E|E>) in float64
+    (particles/solver.py:973-977)."""
+    E = np.asarray(E, np.float64)
+    chi = np.asarray(chi, np.float64).reshape(E.shape)
+    HE = -laplacian_19pt(E) + chi * chi * E
+    return float(np.sqrt(max(float(np.sum(E * HE) / max(float(np.sum(E * E)), 1e-30)), 0.0)))
 
 
 # ---------------------------------------------------------------------------

@@ -152,3 +152,15 @@ def test_ic_builders_match_formulas():
     a = fields.seed_noise(N, 1e-3, rng=np.random.default_rng(42))[0]
     b = (np.random.default_rng(42).standard_normal((N, N, N)) * 1e-3).astype(np.float32)
     np.testing.assert_array_equal(a, b)
+
+
+def test_python_constants_match_header_defines():
+    """Every LFM_* constant the Python binding defines equals the header's."""
+    from lfm_hip import _abi
+    text = open(HEADER).read()
+    defs = {k: int(v, 0) for k, v in re.findall(r"#define\s+(LFM_[A-Z0-9_]+)\s+\(?(-?(?:0x)?[0-9a-fA-F]+)\)?",
+                                                 text)}
+    shared = [k for k in dir(_abi) if k.startswith("LFM_") and k in defs]
+    assert {"LFM_F_MASK", "LFM_F_CHI_DELTA", "LFM_RELAX_SLOT_WORDS"} <= set(shared)
+    for k in shared:
+        assert getattr(_abi, k) == defs[k], k

@@ -315,3 +315,68 @@ def test_poisson_single_mode():
     outside = ~O.interior_mask(N, 0.3)
     assert np.all(frozen[outside] == 19.0)
     np.testing.assert_allclose(frozen[~outside], chi[~outside], atol=0)
+
+
+# ---------------------------------------------------- eigenmode relaxation
+def test_relax_plane_wave_is_a_fixed_shape():
+    """kappa = 0 keeps chi = chi0 everywhere, so a plane wave of amplitude A
+    is multiplied by 1 + dt (lambda(k) - chi0^2) per step and rescaled back:
+    it is a fixed point of the rescaled step, and its eigenvalue estimate is
+    sqrt(chi0^2 - lambda(k)) exactly."""
+    N, chi0, A = 16, 19.0, 5.0
+    f, k = plane(N, (2, 1, 0), 0.4)
+    E0 = A * f / np.abs(f).max()
+    dt = 0.8 / (chi0 * chi0 + 12.0)
+    E, chi, conv, total, cyc = O.relax_eigenmode(E0, N, chi0, 0.0, A, dt, max_cycles=2,
+                                                 steps_per_cycle=40, check_interval=20,
+                                                 tolerance=1e-300, boundary_fraction=0.3)
+    assert np.all(chi == chi0) and total == 80 and not conv
+    np.testing.assert_allclose(E, E0, atol=1e-11)
+    lam = lam_symbol(*k)
+    assert math.isclose(O.relax_eigenvalue(E, chi), math.sqrt(chi0 * chi0 - lam), rel_tol=1e-12)
+
+
+def test_relax_mode_decay_closed_form():
+    """E = a + b cos(k.x) in uniform chi: the step multiplies the k = 0 part
+    by 1 - dt chi^2 and the k part by 1 + dt (lambda(k) - chi^2); the rescale
+    multiplies both alike, so b/a shrinks by their ratio every step."""
+    N, chi0 = 16, 19.0
+    f, k = plane(N, (1, 2, 3))
+    a, b = 4.0, 1.0
+    dt = 0.8 / (chi0 * chi0 + 12.0)
+    steps = 25
+    E, _, _, total, _ = O.relax_eigenmode(a + b * f, N, chi0, 0.0, a + b, dt, max_cycles=1,
+                                          steps_per_cycle=steps, check_interval=1000,
+                                          tolerance=1e-300, boundary_fraction=0.3)
+    assert total == steps
+    lam = lam_symbol(*k)
+    r = (1 + dt * (lam - chi0 * chi0)) / (1 - dt * chi0 * chi0)
+    a_n = E.mean()
+    b_n = 2.0 * np.sum(E * f) / f.size     # projection on cos(k.x) (<cos^2> = 1/2)
+    assert math.isclose(b_n / a_n, (b / a) * r ** steps, rel_tol=1e-10)
+    # the rescale holds max|E| at the amplitude
+    assert math.isclose(float(np.abs(E).max()), a + b, rel_tol=1e-12)
+
+
+def test_relax_shell_mask_brute_force():
+    N, bf = 9, 0.3
+    m = O.relax_shell_mask(N, bf)
+    c = (N - 1) / 2.0
+    cnt = sum(1 for i in range(N) for j in range(N) for k in range(N)
+              if math.sqrt((i - c) ** 2 + (j - c) ** 2 + (k - c) ** 2) >= (N * 0.5) * (1 - bf))
+    assert m.sum() == cnt and m[0, 0, 0] and not m[4, 4, 4]
+
+
+def test_relax_chi_clip_and_shell():
+    """A deep E^2 source drives chi below 0.1 at the centre: the Poisson chi is
+    clipped to [0.1, chi0] and held at chi0 on the shell."""
+    N, chi0 = 16, 19.0
+    E = np.zeros((N, N, N))
+    E[6:10, 6:10, 6:10] = 60.0
+    shell = O.relax_shell_mask(N, 0.3)
+    chi = O.relax_chi(E, N, chi0, 1 / 63, shell)
+    raw = O.poisson_chi(E ** 2, N, chi0, 1 / 63)
+    assert raw.min() < 0.1 and chi.min() == 0.1 and chi.max() <= chi0
+    assert np.all(chi[shell] == chi0)
+    inner = ~shell
+    np.testing.assert_allclose(chi[inner], np.clip(raw[inner], 0.1, chi0), atol=0)
