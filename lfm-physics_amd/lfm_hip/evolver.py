@@ -364,26 +364,33 @@ class Evolver:
         cn = free(cc, cp)
         return (free(cur, prev), cur, cn, cc, cc, cp, True)
 
-    def _cycle_graph(self, cycles: int = 1):
+    def _sa_flags(self):
+        return (self._psisq_valid, self._spec_valid, self._sa_stale) if self.sa_enabled else ()
+
+    def _hooked_step(self, hook) -> None:
+        """One step, then (with a device hook) the device step counter and
+        the hook, as evolve runs them."""
+        self._step()
+        if hook is not None:
+            self._step_dev.add_(1)
+            hook(self, self._step_dev)
+
+    def _cycle_graph(self, cycles: int = 1, hook=None):
         """(graph, launches, steps) replaying `cycles` whole rotation cycles
         from the current state, or None when the state is not steady or
         graphs are off.  A cycle is the shortest launch sequence that returns
         every slot reference to where it started, so a replay leaves the
-        bookkeeping unchanged.  Graphs are kept per starting state (the
-        pointers, flags and parameters of every launch are fixed by it) and
-        cycle count."""
+        bookkeeping unchanged; the S_a flags must also come back to their
+        values (every step leaves them alike), else the capture is dropped.
+        Graphs are kept per starting state (the pointers, flags and
+        parameters of every launch are fixed by it), cycle count and hook."""
         if not self.graphs or self.kernel_events is not None or not self._damped:
             return None
         if self._cc == self._cp:
             return None
-        if self.level == 2 and self.sa_enabled:
-            if self._scv.method != 2:
-                return None          # rocFFT launches stay out of graphs
-            # the S_a flags must be at the values every step leaves them at
-            flags = (self._psisq_valid, self._spec_valid, self._sa_stale)
-            if flags != ((False, True, True) if self._fused else (True, False, False)):
-                return None
-        pair = self.two_step and self.N <= 1023
+        if self.level == 2 and self.sa_enabled and self._scv.method != 2:
+            return None              # rocFFT launches stay out of graphs
+        pair = self.two_step and self.N <= 1023 and hook is None
         key0 = self._key()
         k, m = key0, 0
         while True:
@@ -393,45 +400,64 @@ class Evolver:
         if k != key0:
             return None
         m *= cycles
-        gkey = key0 + (self.tile, self.chunk, pair, cycles)
-        hit = self._graphs.get(gkey)
-        if hit is not None:
-            return hit
+        flags0 = self._sa_flags()
+        gkey = key0 + flags0 + (self.tile, self.chunk, pair, cycles, hook)
+        if gkey in self._graphs:
+            return self._graphs[gkey]
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))
         g = torch.cuda.CUDAGraph()
         with torch.cuda.stream(s):
             with torch.cuda.graph(g, stream=s):
                 for _ in range(m):
-                    self._step2() if pair else self._step()
+                    self._step2() if pair else self._hooked_step(hook)
         torch.cuda.current_stream(self.device).wait_stream(s)
         # Capture recorded the launches without running them.  The bookkeeping
-        # it advanced is back at the start (a whole cycle) and the S_a flags
-        # are at their per-step values, so the state still describes the
-        # buffers.
+        # it advanced is back at the start (a whole cycle); when the S_a flags
+        # are too, the state still describes the buffers and every replay
+        # repeats the same launches.  Otherwise the flags are restored and the
+        # state is marked as not replayable.
         if self._key() != key0:
             raise RuntimeError("slot rotation did not return to its start after a captured cycle")
+        if self._sa_flags() != flags0:
+            (self._psisq_valid, self._spec_valid, self._sa_stale) = flags0
+            self._graphs[gkey] = None
+            return None
         hit = (g, m, 2 * m if pair else m)
         self._graphs[gkey] = hit
         return hit
 
-    def evolve(self, steps: int, callback=None, freeze_chi: bool = False) -> None:
+    def evolve(self, steps: int, callback=None, freeze_chi: bool = False,
+               device_hook=None) -> None:
         """Run `steps` leapfrog steps; callback(evolver, step) every
         config.report_interval steps (lfm/core/evolver.py:137-180).  REAL and
         COMPLEX run pairs of steps per launch when no callback falls between
         them; the result is the same as single steps.  Once the slot rotation
         is steady, whole rotation cycles are replayed from HIP graphs (same
-        launches, so the same bits as eager stepping)."""
+        launches, so the same bits as eager stepping).
+
+        device_hook(evolver, step): called after every step with `step` a
+        0-d int64 device tensor holding the step number.  It must only queue
+        device work (torch ops on the native views, no host reads), so it is
+        captured into the replayed graphs with the steps: a per-step barrier
+        or source then costs no host launches."""
         report = self.config.report_interval
         left = int(steps)
-        pair_ok = self.two_step and not freeze_chi and self.N <= 1023
+        hook = device_hook
+        pair_ok = self.two_step and not freeze_chi and self.N <= 1023 and hook is None
+        if hook is not None:
+            if freeze_chi:
+                raise ValueError("device_hook with freeze_chi is not supported")
+            if not hasattr(self, "_step_dev"):
+                self._step_dev = torch.zeros((), dtype=torch.int64, device=self.device)
+            self._step_dev.fill_(self.step)
         while left > 0:
             to_report = report - self.step % report if (callback is not None and report > 0) else left
-            cyc = None if freeze_chi else self._cycle_graph()
+            cyc = None if freeze_chi else self._cycle_graph(hook=hook)
             if cyc is not None and min(left, to_report) >= cyc[2]:
                 # long runs replay a graph of several cycles (fewer host
                 # launches per step), then single cycles
-                big = self._cycle_graph(self.GRAPH_CYCLES) if min(left, to_report) >= \
+                big = self._cycle_graph(self.GRAPH_CYCLES, hook) if min(left, to_report) >= \
                     self.GRAPH_CYCLES * cyc[2] else None
                 for g, _, n in ((big or cyc), cyc):
                     while min(left, to_report) >= n:
@@ -447,7 +473,10 @@ class Evolver:
                 self.step += 2
                 left -= 2
             else:
-                self._step(freeze_chi=freeze_chi)
+                if hook is None:
+                    self._step(freeze_chi=freeze_chi)
+                else:
+                    self._hooked_step(hook)
                 self.step += 1
                 left -= 1
             if callback is not None and report > 0 and self.step % report == 0:

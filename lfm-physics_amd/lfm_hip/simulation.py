@@ -306,9 +306,12 @@ class Simulation:
         self._snap_i = pi.clone() if pi is not None else None
 
     def run(self, steps: int, callback=None, record_metrics: bool = True,
-            evolve_chi: bool = True) -> None:
+            evolve_chi: bool = True, device_hook=None) -> None:
         """Advance `steps` steps; a metrics snapshot goes to :attr:`history`
-        every report_interval steps (lfm/simulation.py:1123-1188)."""
+        every report_interval steps (lfm/simulation.py:1123-1188).
+        device_hook(evolver, step_tensor) runs after every step as device work
+        captured into the replayed graphs (see Evolver.evolve); write
+        barriers and sources through the evolver's native views there."""
         self._auto_equilibrate()
         self._snapshot()
 
@@ -321,7 +324,8 @@ class Simulation:
                 callback(self, step)
             self._snapshot()
 
-        self._evolver.evolve(steps, callback=_cb, freeze_chi=not evolve_chi)
+        self._evolver.evolve(steps, callback=_cb, freeze_chi=not evolve_chi,
+                             device_hook=device_hook)
 
     def run_driven(self, steps: int, chi_forcing, record_metrics: bool = False) -> None:
         """Overwrite chi with chi_forcing(t) (a scalar or an (N, N, N) array or
@@ -351,13 +355,14 @@ class Simulation:
 
     def run_with_snapshots(self, steps: int, snapshot_every: int = 100, fields=None,
                            callback=None, step_callback=None, record_metrics: bool = True,
-                           evolve_chi: bool = True) -> list[dict]:
+                           evolve_chi: bool = True, device_hook=None) -> list[dict]:
         """Run and keep host copies of `fields` (any of "chi", "psi_real",
         "psi_imag", "energy_density"; default ["chi"]) every `snapshot_every`
         steps and at the end (lfm/simulation.py:1253-1395).  `callback` fires
         after each batch; `step_callback(sim, step)` fires after every step
         (report_interval is 1 meanwhile, and metrics are then recorded once
-        per batch instead of every report_interval)."""
+        per batch instead of every report_interval).  device_hook: as in
+        :meth:`run`, a per-step device-side hook kept inside graph replay."""
         self._auto_equilibrate()
         fields = ["chi"] if fields is None else fields
         snapshots: list[dict] = []
@@ -382,7 +387,7 @@ class Simulation:
             try:
                 for n in batches:
                     self.run(n, callback=step_callback, record_metrics=False,
-                             evolve_chi=evolve_chi)
+                             evolve_chi=evolve_chi, device_hook=device_hook)
                     if callback is not None:
                         callback(self, self.step)
                     snapshots.append(take())
@@ -394,7 +399,8 @@ class Simulation:
                 self.config.report_interval = orig
         else:
             for n in batches:
-                self.run(n, callback=None, record_metrics=record_metrics, evolve_chi=evolve_chi)
+                self.run(n, callback=None, record_metrics=record_metrics, evolve_chi=evolve_chi,
+                         device_hook=device_hook)
                 if callback is not None:
                     callback(self, self.step)
                 snapshots.append(take())

@@ -739,3 +739,117 @@ def test_run_driven_matches_oracle(gpu):
     ec = O.rel_l2(sim.chi, st.chi)
     log(f"  psi error / |psi_0| {e:.2e}, chi rel L2 {ec:.2e}")
     assert e <= TOL64 and ec <= TOL64
+
+
+def _device_barrier_source_hook(N, dt, chi0, ncomp=1):
+    """A barrier (chi held at a template in all four buffers, Psi zeroed in a
+    slab with a slit) and a sine-driven plane source (on every colour), written
+    as device work on the step tensor, for Evolver.evolve(device_hook=...)."""
+    import torch
+    z = torch.arange(N, dtype=torch.float32, device="cuda") - N / 2
+    env = torch.exp(-(z[:, None] ** 2) / 18.0 - (z[None, :] ** 2) / 18.0)
+    keep = torch.ones((N, N, N), device="cuda")
+    keep[:, 20:22, :] = 0.0
+    keep[:, 20:22, 14:18] = 1.0
+    chi_t = torch.full((N, N, N), chi0, device="cuda")
+    chi_t[:, 20:22, :] = 2.0 * chi0
+    chi_t[:, 20:22, 14:18] = chi0
+    shape = (N, N, N)
+    pshape = (ncomp, N, N, N)
+
+    def hook(ev, step):
+        amp = (dt * dt * 0.5) * torch.sin(step.to(torch.float32) * (1.3 * dt))
+        pr, pi = ev.native_psi_real().view(pshape), ev.native_psi_imag().view(pshape)
+        pr[:, 8] += amp * env
+        pi[:, 8] += (0.5 * amp) * env
+        for buf in ev.native_chi_refs():
+            buf.view(shape).copy_(chi_t)
+        pr.mul_(keep)
+        pi.mul_(keep)
+    return hook
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_device_hook_graph_replay_equals_eager(gpu, level):
+    """A per-step device hook is captured into the replayed cycle graphs:
+    results equal eager stepping with the same hook bit for bit, over several
+    evolve calls with host reads in between (COLOR with the S_a transform:
+    the hook's Psi writes invalidate |Psi|^2, and the replayed steps must
+    recompute S_a from Psi)."""
+    import time
+    import torch
+    import lfm_hip as L
+    N = 32
+    log(f"device hook graph vs eager, level {level}")
+    cfg = L.gravity_em(N) if level == 1 else L.full_physics(N)
+    r, i = make_ics(level, N, seed=3)
+    evs = {}
+    for g in (True, False):
+        ev = L.Evolver(cfg, backend="hip", graphs=g)
+        ev.set_psi_real(r)
+        ev.set_psi_imag(i)
+        ev.use_absolute_chi()
+        evs[g] = ev
+    hook = _device_barrier_source_hook(N, cfg.dt, float(cfg.chi0), 3 if level == 2 else 1)
+    for n in (7, 30, 11):
+        for ev in evs.values():
+            ev.evolve(n, device_hook=hook)
+        torch.cuda.synchronize()
+        a, b = evs[True], evs[False]
+        np.testing.assert_array_equal(a.get_psi_real(), b.get_psi_real())
+        np.testing.assert_array_equal(a.get_psi_imag(), b.get_psi_imag())
+        np.testing.assert_array_equal(a.get_chi(), b.get_chi())
+    used = [k for k, v in evs[True]._graphs.items() if v is not None and hook in k]
+    log(f"  hooked graphs captured: {len(used)}")
+    assert used
+    # the hook costs no host launches inside replay
+    for g, ev in evs.items():
+        ev.evolve(48, device_hook=hook)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev.evolve(96, device_hook=hook)
+        torch.cuda.synchronize()
+        log(f"  {'graphs' if g else 'eager '}: {(time.perf_counter() - t0) / 96 * 1e6:.1f} us per hooked step")
+
+
+def test_device_hook_matches_oracle(gpu):
+    """The device-side barrier / source hook against the FP64 oracle running
+    the same hook between its steps (Psi error <= 1e-5 of |Psi_0|)."""
+    import math
+    import torch
+    import lfm_hip as L
+    N, steps = 32, 40
+    log(f"device hook vs FP64 oracle, COMPLEX {N}^3")
+    cfg = L.gravity_em(N)
+    r, i = make_ics(1, N, seed=5)
+    sim = L.Simulation(cfg, backend="hip")
+    sim.set_psi_real(r)
+    sim.set_psi_imag(i)
+    sim.evolver.use_absolute_chi()
+    chi0, dt = float(cfg.chi0), cfg.dt
+    sim.run(steps, record_metrics=False, device_hook=_device_barrier_source_hook(N, dt, chi0))
+    z = np.arange(N) - N / 2
+    env = np.exp(-(z[:, None] ** 2) / 18.0 - (z[None, :] ** 2) / 18.0)
+    keep = np.ones((N, N, N))
+    keep[:, 20:22, :] = 0.0
+    keep[:, 20:22, 14:18] = 1.0
+    chi_t = np.full((N, N, N), chi0)
+    chi_t[:, 20:22, :] = 2.0 * chi0
+    chi_t[:, 20:22, 14:18] = chi0
+    st = O.OracleState(1, N, r, i, np.full(N ** 3, chi0), O.boundary_mask(N, cfg.boundary_fraction),
+                       oracle_params(cfg))
+    for step in range(1, steps + 1):
+        st.step(1)
+        a = st.cur
+        amp = dt * dt * 0.5 * math.sin(step * 1.3 * dt)
+        st.r[a].reshape(N, N, N)[8] += amp * env
+        st.i[a].reshape(N, N, N)[8] += 0.5 * amp * env
+        for buf in (st.c[0], st.c[1], st.cp[0], st.cp[1]):
+            np.copyto(buf, chi_t.ravel())
+        st.r[a] *= keep.ravel()
+        st.i[a] *= keep.ravel()
+    norm0 = float(np.linalg.norm(np.concatenate([r.ravel(), i.ravel()]).astype(np.float64)))
+    psi = np.concatenate([sim.psi_real.ravel(), sim.psi_imag.ravel()])
+    e = float(np.linalg.norm(psi - np.concatenate([st.psi_r, st.psi_i])) / norm0)
+    log(f"  psi error / |psi_0| {e:.2e}")
+    assert e <= TOL64
