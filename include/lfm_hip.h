@@ -179,7 +179,11 @@ int lfm_clip_nonneg_f32(const float* x, float* y, int64_t n, void* stream);
  * inside a lattice).  One lfm_batch_step_f32 call advances every lattice by one
  * step with a single launch (rotation-mode outputs only); `flags` adds
  * LFM_F_DAMP_PREV / LFM_F_FREEZE_CHI / LFM_F_CHI_DELTA to every lattice.
- * COLOR lattices must have kappa_tube == 0 (the S_a transform is per lattice). */
+ * COLOR lattices with kappa_tube > 0 read their raw S_a from the `sa` arena
+ * and write |Psi'_a|^2 to `psisq_out` (both 3 fields per lattice, laid out as
+ * the Psi arenas; the per-lattice S_a transforms run between steps, as
+ * lfm_scv_apply / lfm_scv_apply_psisq); `sa` is required when a lattice has
+ * kappa_tube > 0, and both are ignored otherwise. */
 int lfm_batch_plan_create(int32_t level, int32_t nlat, const lfm_params_t* params, int32_t tile,
                           int32_t chunk, void** plan);
 int lfm_batch_plan_destroy(void* plan);
@@ -188,7 +192,7 @@ int32_t lfm_batch_workgroups(void* plan);
 int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_prev, const float* pi,
                        const float* pi_prev, const float* chi, const float* chi_prev,
                        const float* mask, float* pr_out, float* pi_out, float* chi_out,
-                       int32_t flags, void* stream);
+                       const float* sa, float* psisq_out, int32_t flags, void* stream);
 
 /* Reverse-mode (adjoint) sweep of the COMPLEX leapfrog, for gradients of a
  * loss of the final state.  It has no counterpart in the reference (which

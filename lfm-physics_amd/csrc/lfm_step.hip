@@ -596,6 +596,8 @@ struct BatchArenas {
     const float* pr; const float* prp; const float* pi; const float* pip;
     const float* chi; const float* chip; const float* mask;
     float* pr_out; float* pi_out; float* chi_out;
+    const float* sa;   // COLOR: raw S_a of every lattice (3 fields each), or NULL
+    float* psisq;      // COLOR: |Psi'_a|^2 of the new fields, or NULL
 };
 
 template <int L, int BJ, int RJ, bool NT>
@@ -617,6 +619,11 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled_batch(BatchArenas A,
     if (A.pi) { P.pi = A.pi + oc; P.pip = A.pip + oc; P.pi_out = A.pi_out + oc; }
     P.mask = (kp.flags & LFM_F_MASK) ? A.mask + o : nullptr;
     P.pr_out = A.pr_out + oc; P.chi_out = A.chi_out + o;
+    if (A.sa && kp.kt > 0.0f) {
+        // the S_a term of this lattice, and |Psi'|^2 for its next transform
+        P.sa = A.sa + oc;
+        P.psisq = A.psisq ? A.psisq + oc : nullptr;
+    }
     P.n3 = n3;
     tile_body<L, BJ, RJ, NT>(P, kp, d.y, d.z, d.w, min(d.w + Lb.CH, Lb.N));
 }
@@ -1339,6 +1346,7 @@ int launch_batch(const BatchArenas& A, const BatchLattice* lat, const int4* desc
 
 struct BatchPlan {
     int level = 0, nlat = 0, tile = 0, ndesc = 0;
+    bool scv = false;            // some lattice has kappa_tube > 0
     long long total = 0;
     BatchLattice* d_lat = nullptr;
     int4* d_desc = nullptr;
@@ -1636,10 +1644,6 @@ extern "C" int lfm_batch_plan_create(int32_t level, int32_t nlat, const lfm_para
             set_error("lfm_batch_plan_create: lattice %d has N=%d outside [8, 1023]", b, p.N);
             return LFM_E_ARG;
         }
-        if (level == 2 && p.kappa_tube > 0.0f) {
-            set_error("lfm_batch_plan_create: the S_a transform is not batched (lattice %d has kappa_tube > 0)", b);
-            return LFM_E_ARG;
-        }
         BatchLattice& L = lat[b];
         L.off = off;
         L.N = p.N;
@@ -1669,6 +1673,7 @@ extern "C" int lfm_batch_plan_create(int32_t level, int32_t nlat, const lfm_para
     }
     auto* P = new BatchPlan();
     P->level = level; P->nlat = nlat; P->tile = tile; P->ndesc = n; P->total = off;
+    for (int b = 0; b < nlat; ++b) P->scv = P->scv || (level == 2 && params[b].kappa_tube > 0.0f);
     hipError_t e = hipMalloc((void**)&P->d_lat, sizeof(BatchLattice) * nlat);
     if (e == hipSuccess) e = hipMalloc((void**)&P->d_desc, sizeof(int4) * n);
     if (e == hipSuccess) e = hipMemcpy(P->d_lat, lat.data(), sizeof(BatchLattice) * nlat, hipMemcpyHostToDevice);
@@ -1704,7 +1709,8 @@ extern "C" int lfm_batch_plan_destroy(void* plan) {
 extern "C" int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_prev,
                                   const float* pi, const float* pi_prev, const float* chi,
                                   const float* chi_prev, const float* mask, float* pr_out,
-                                  float* pi_out, float* chi_out, int32_t flags, void* stream) {
+                                  float* pi_out, float* chi_out, const float* sa, float* psisq_out,
+                                  int32_t flags, void* stream) {
     auto* P = static_cast<BatchPlan*>(plan);
     if (!P || !pr || !pr_prev || !chi || !chi_prev || !mask || !pr_out || !chi_out) {
         set_error("lfm_batch_step_f32: required pointer is NULL");
@@ -1714,19 +1720,25 @@ extern "C" int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_p
         set_error("lfm_batch_step_f32: complex levels need pi, pi_prev and pi_out");
         return LFM_E_ARG;
     }
+    if (P->scv && !sa) {
+        set_error("lfm_batch_step_f32: lattices with kappa_tube > 0 need the S_a arena");
+        return LFM_E_ARG;
+    }
     const size_t ns = size_t(P->total) * (P->level == 2 ? 3 : 1);
-    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask};
-    const size_t nin[] = {ns, ns, ns, ns, size_t(P->total), size_t(P->total), size_t(P->total)};
-    float* outs[] = {pr_out, pi_out, chi_out};
-    const size_t nout[] = {ns, ns, size_t(P->total)};
-    for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 7; ++b)
+    const float* sa_in = P->scv ? sa : nullptr;
+    float* sq_out = P->scv ? psisq_out : nullptr;
+    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask, sa_in};
+    const size_t nin[] = {ns, ns, ns, ns, size_t(P->total), size_t(P->total), size_t(P->total), ns};
+    float* outs[] = {pr_out, pi_out, chi_out, sq_out};
+    const size_t nout[] = {ns, ns, size_t(P->total), ns};
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 8; ++b)
             if (outs[a] && overlaps(outs[a], nout[a] * 4, ins[b], nin[b] * 4)) {
                 set_error("lfm_batch_step_f32: output %d overlaps input %d", a, b);
                 return LFM_E_ARG;
             }
     BatchArenas A{pr, pr_prev, P->level ? pi : nullptr, P->level ? pi_prev : nullptr, chi, chi_prev,
-                  mask, pr_out, P->level ? pi_out : nullptr, chi_out};
+                  mask, pr_out, P->level ? pi_out : nullptr, chi_out, sa_in, sq_out};
     const int fo = flags & (LFM_F_DAMP_PREV | LFM_F_FREEZE_CHI | LFM_F_CHI_DELTA);
     hipStream_t st = (hipStream_t)stream;
     switch (P->level) {

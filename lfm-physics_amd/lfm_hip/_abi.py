@@ -53,7 +53,7 @@ SIGNATURES = [
     ("lfm_batch_plan_destroy", ctypes.c_int, [_vp]),
     ("lfm_batch_total_sites", _i64, [_vp]),
     ("lfm_batch_workgroups", _i32, [_vp]),
-    ("lfm_batch_step_f32", ctypes.c_int, [_vp] * 11 + [_i32, _vp]),
+    ("lfm_batch_step_f32", ctypes.c_int, [_vp] * 13 + [_i32, _vp]),
     ("lfm_adjoint_partials_len", _i64, [_i32]),
     ("lfm_calib_stream_f32", ctypes.c_int, [_vp, _i32, _vp, _i32, ctypes.c_int64, _i32, _i32, _vp]),
     ("lfm_relax_step_f32", ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, ctypes.c_float, _vp]),

@@ -17,8 +17,11 @@ fields from the cache.  Lattices are independent, so the order changes no
 result bit.  One group (the default) measured fastest: the extra, smaller
 launches cost more than the cache saves.
 
-Lattices of one batch share a field level; COLOR lattices must not use the
-S_a transform (kappa_tube == 0), which runs per lattice.
+Lattices of one batch share a field level.  COLOR lattices with
+kappa_tube > 0 keep their S_a and |Psi'|^2 in two more arenas; before each
+step the S_a transform runs per lattice (one plan per lattice size, the
+|Psi'|^2 the previous step wrote as its input), then the one batched launch
+reads S_a and writes the next |Psi'|^2.
 """
 from __future__ import annotations
 
@@ -55,9 +58,6 @@ class LatticeBatch:
         if len(levels) != 1:
             raise ValueError("all lattices of a batch must share one field level")
         self.level = levels.pop()
-        if self.level == 2 and any(c.kappa_tube > 0 for c in configs):
-            raise NotImplementedError("COLOR lattices with kappa_tube > 0 need the per-lattice "
-                                      "S_a transform; use Evolver for them")
         if self.level == 2 and any(c.n_colors != 3 for c in configs):
             raise NotImplementedError("the COLOR step is defined for 3 colours")
         self.backend = get_backend(backend)
@@ -86,6 +86,21 @@ class LatticeBatch:
             if m:
                 self._view(self.mask, b).copy_(torch.from_numpy(
                     boundary_mask_host(c.grid_size, c.boundary_fraction)))
+        # S_a (COLOR lattices with kappa_tube > 0): raw S_a and |Psi'|^2 arenas,
+        # one transform plan per lattice size
+        self.scv = [self.level == 2 and c.kappa_tube > 0 for c in configs]
+        self._sa = self._psisq = None
+        self._scv_plans = {}
+        if any(self.scv):
+            self._sa = torch.zeros(npsi, dtype=f32, device=dev)
+            self._psisq = torch.empty(npsi, dtype=f32, device=dev)
+            for b, on in enumerate(self.scv):
+                if on and self.sizes[b] not in self._scv_plans:
+                    self._scv_plans[self.sizes[b]] = ScvPlan(self.sizes[b], 3)
+        # per lattice: its |Psi'|^2 arena span matches its current Psi (it has
+        # stepped since its Psi was last set), so S_a comes from it, as the
+        # single-lattice Evolver does after its first step
+        self._psisq_ok = [False] * self.nlat
         self._cur, self._prev, self._chi_cur, self._chi_prev = 0, 1, 0, 1
         self._damped = False
         self.step = 0
@@ -134,7 +149,7 @@ class LatticeBatch:
                     c = self.configs[b]
                     arr[b - b0] = params(c.grid_size, c.dt ** 2, c.kappa, c.lambda_self, c.chi0,
                                          c.e0_sq, c.epsilon_w, c.kappa_c, c.epsilon_cc,
-                                         c.kappa_string, 0.0,
+                                         c.kappa_string, c.kappa_tube if self.scv[b] else 0.0,
                                          _abi.LFM_F_MASK if self.has_mask[b] else 0)
                 h = ctypes.c_void_p()
                 _abi.check(self._lib.lfm_batch_plan_create(self.level, b1 - b0, arr, int(tile),
@@ -202,6 +217,7 @@ class LatticeBatch:
         self._undamp()
         self._view(self._pr[self._cur], b, True).copy_(d)
         self._view(self._pr[self._prev], b, True).copy_(d)
+        self._psisq_ok[b] = False
 
     def set_psi_imag(self, b: int, arr) -> None:
         if self._pi is None:
@@ -210,6 +226,7 @@ class LatticeBatch:
         self._undamp()
         self._view(self._pi[self._cur], b, True).copy_(d)
         self._view(self._pi[self._prev], b, True).copy_(d)
+        self._psisq_ok[b] = False
 
     def set_chi(self, b: int, arr) -> None:
         """Current and prev chi of lattice b := arr."""
@@ -252,18 +269,39 @@ class LatticeBatch:
         self._chi_prev, self._chi_cur = cc, cn
         self._damped = True
 
+    def _transforms(self, g: int, cur: int) -> None:
+        """S_a of every lattice of group g with kappa_tube > 0, from the
+        |Psi|^2 the previous step wrote (or from Psi after a set)."""
+        b0, b1 = self.groups[g]
+        for b in range(b0, b1):
+            if not self.scv[b]:
+                continue
+            c, plan = self.configs[b], self._scv_plans[self.sizes[b]]
+            sa = self._view(self._sa, b, True)
+            if self._psisq_ok[b]:
+                plan.apply_psisq(self._view(self._psisq, b, True), c.sa_gamma, c.sa_d, sa)
+            else:
+                plan.apply(self._view(self._pr[cur], b, True), self._view(self._pi[cur], b, True),
+                           c.sa_gamma, c.sa_d, sa)
+
     def _launch(self, g: int, slots) -> None:
         cur, prev, nxt, cc, cp, cn, flags = slots
+        if self._sa is not None:
+            self._transforms(g, cur)
         o = int(self.offsets[self.groups[g][0]]) * 4
         op = o * self.ncomp
         pr, pi, chi = self._pr, self._pi, self._chi
+        sa = self._sa.data_ptr() + op if self._sa is not None else None
+        sq = self._psisq.data_ptr() + op if self._sa is not None else None
         rc = self._lib.lfm_batch_step_f32(
             self._plans[g], pr[cur].data_ptr() + op, pr[prev].data_ptr() + op,
             pi[cur].data_ptr() + op if pi else None, pi[prev].data_ptr() + op if pi else None,
             chi[cc].data_ptr() + o, chi[cp].data_ptr() + o, self.mask.data_ptr() + o,
             pr[nxt].data_ptr() + op, pi[nxt].data_ptr() + op if pi else None,
-            chi[cn].data_ptr() + o, flags, stream_handle())
+            chi[cn].data_ptr() + o, sa, sq, flags, stream_handle())
         _abi.check(rc, "lfm_batch_step_f32")
+        b0, b1 = self.groups[g]
+        self._psisq_ok[b0:b1] = [True] * (b1 - b0)
 
     def _step(self) -> None:
         """One step of every lattice (one launch per group)."""
@@ -291,14 +329,17 @@ class LatticeBatch:
         steps = int(steps)
         if steps <= 0:
             return
-        if not graph:
+        if not graph or any(p.method != 2 for p in self._scv_plans.values()):
+            # rocFFT plans (non-power-of-two S_a sizes) stay out of graphs
             for _ in range(steps):
                 self._step()
             self.step += steps
             return
         done = 0
-        # reach the steady state: prev damped and the chi slots distinct
-        while done < steps and (not self._damped or self._chi_cur == self._chi_prev):
+        # reach the steady state: prev damped, the chi slots distinct and every
+        # lattice's S_a taken from its |Psi'|^2
+        while done < steps and (not self._damped or self._chi_cur == self._chi_prev
+                                or not all(self._psisq_ok)):
             self._step()
             done += 1
         per = 3 * self.cycles
@@ -367,6 +408,7 @@ class LatticeBatch:
             for dst, src in zip(self._chi, snap[2]):
                 dst.copy_(src)
             self._cur, self._prev, self._chi_cur, self._chi_prev, self._damped = snap[3]
+            self._psisq_ok = [False] * self.nlat    # the |Psi'|^2 arena holds tuning steps
         best = min(res, key=res.get)
         self._make_plan(*best)
         return res

@@ -132,3 +132,37 @@ def test_batched_sweep_rows_match_simulation(gpu):
         assert row["chi_min"] == ev.chi_stats()["chi_min"]
         e = ev.energy_sums()
         assert abs(row["energy_total"] - e["energy_total"]) <= 1e-12 * abs(e["energy_total"])
+
+
+@pytest.mark.parametrize("sizes,group_bytes", [([16, 32, 24], None), ([32, 16, 64, 32], 1 << 22)])
+def test_batch_color_with_sa_equals_single_runs(gpu, sizes, group_bytes):
+    """COLOR full_physics lattices with the S_a transform (kappa_tube > 0):
+    per-lattice transforms between batched launches give every lattice
+    exactly its single-lattice Evolver run (24^3 uses a rocFFT plan and stays
+    eager; the power-of-two batch replays graphs, in groups for the second
+    case)."""
+    import torch
+    import lfm_hip as L
+    log(f"batch COLOR + S_a vs single, sizes {sizes} group_bytes {group_bytes}")
+    rng = np.random.default_rng(23)
+    cfgs = [L.full_physics(N, kappa=(1 / 63) * rng.uniform(0.5, 2)) for N in sizes]
+    assert all(c.kappa_tube > 0 for c in cfgs)
+    ics = [_ics(2, N, rng) for N in sizes]
+    batch = L.LatticeBatch(cfgs, backend="hip", group_bytes=group_bytes)
+    singles = []
+    for b, (c, (r, i)) in enumerate(zip(cfgs, ics)):
+        batch.set_psi_real(b, r)
+        batch.set_psi_imag(b, i)
+        ev = L.Evolver(c, backend="hip")
+        ev.set_psi_real(r)
+        ev.set_psi_imag(i)
+        singles.append(ev)
+    for n in (10, 14):
+        batch.evolve(n)
+        for ev in singles:
+            ev.evolve(n)
+        torch.cuda.synchronize()
+        for b, ev in enumerate(singles):
+            np.testing.assert_array_equal(batch.get_psi_real(b), ev.get_psi_real(), err_msg=f"psi_r {b}")
+            np.testing.assert_array_equal(batch.get_psi_imag(b), ev.get_psi_imag(), err_msg=f"psi_i {b}")
+            np.testing.assert_array_equal(batch.get_chi(b), ev.get_chi(), err_msg=f"chi {b}")
