@@ -114,18 +114,22 @@ def test_graph_replay_equals_eager(gpu, group_bytes, cycles):
         np.testing.assert_array_equal(a, b)
 
 
-def test_batched_sweep_rows_match_simulation(gpu):
+@pytest.mark.parametrize("color", [False, True])
+def test_batched_sweep_rows_match_simulation(gpu, color):
     import lfm_hip as L
     from lfm_hip.batch import sweep
-    log("batched sweep vs per-value Simulation")
-    base = L.SimulationConfig(grid_size=24, field_level=L.FieldLevel.COMPLEX)
+    log(f"batched sweep vs per-value Simulation ({'COLOR full_physics' if color else 'COMPLEX'})")
+    N = 16 if color else 24
+    mk = (lambda **k: L.full_physics(N, **k)) if color else \
+        (lambda **k: L.SimulationConfig(grid_size=N, field_level=L.FieldLevel.COMPLEX, **k))
+    base = mk()
     vals = [1 / 63, 1.5 / 63, 2 / 63]
     rows = sweep(base, "kappa", vals, 30, soliton={"amplitude": 4.0}, backend="hip")
     assert [r["kappa"] for r in rows] == vals
     for v, row in zip(vals, rows):
-        cfg = L.SimulationConfig(grid_size=24, field_level=L.FieldLevel.COMPLEX, kappa=v)
+        cfg = mk(kappa=v)
         sim = L.Simulation(cfg, backend="hip")
-        sim.place_soliton((12, 12, 12), amplitude=4.0)
+        sim.place_soliton((N // 2, N // 2, N // 2), amplitude=4.0)
         sim.equilibrate()
         sim.run(30, record_metrics=False)
         ev = sim.evolver
