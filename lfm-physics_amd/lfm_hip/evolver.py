@@ -93,9 +93,10 @@ class Evolver:
         # two_step: REAL / COMPLEX advance two steps per launch (lfm_step2_*,
         # bit-identical to single steps), which needs a fourth rotation slot:
         # both outputs must differ from both inputs.  Off by default: measured
-        # on MI355X it moves 1.5x fewer HBM bytes but is not faster (COMPLEX
-        # 512^3 0.99 vs 0.95 ms per step, REAL 64^3 11.2 vs 9.5 us): its
-        # ring recomputation and two register queues hold it at 2 waves/SIMD.
+        # on MI355X it moves fewer HBM bytes (COMPLEX 512^3: 29.3 against 38.2
+        # B per site and step) but is not faster (1.04 vs 0.92 ms per step;
+        # REAL 64^3 slower still, too few workgroups): its ring recomputation,
+        # 254 VGPRs and 61 KB of LDS hold it at 2 waves/SIMD.
         self.two_step = bool(two_step) and lvl != FieldLevel.COLOR
         ns = 4 if self.two_step else 3
         self._pr = [torch.zeros(n, dtype=f32, device=dev) for _ in range(ns)]
