@@ -205,7 +205,8 @@ def run_batch(args):
         ics.append((r + nr, i + ni))
     steps = args.steps
     sites = sum(c.grid_size ** 3 for c in cfgs)
-    batch = L.LatticeBatch(cfgs, backend="hip")
+    batch = L.LatticeBatch(cfgs, backend="hip", tile=0 if args.tile == "auto" else int(args.tile),
+                           chunk=args.chunk)
 
     def reset_batch():
         for b, (r, i) in enumerate(ics):
