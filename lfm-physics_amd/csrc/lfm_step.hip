@@ -1305,7 +1305,7 @@ template <int L, bool NT>
 int launch_step2_t(const Ptrs2& P2, const KParams& kp, hipStream_t st) {
     const int N = kp.N;
     const int nkt = (N + T2K - 1) / T2K, njt = (N + T2J - 1) / T2J;
-    const int CH = kp.chunk > 0 ? min(kp.chunk, N) : choose_chunk(N, nkt * njt);
+    const int CH = kp.chunk > 0 ? min(kp.chunk, N) : choose_chunk(N, nkt * njt, NT ? 8 : 2);
     const int nch = (N + CH - 1) / CH;
     const long long nblocks = (long long)nkt * njt * nch;
     hipLaunchKernelGGL((step2_tiled<L, NT>), dim3((unsigned)nblocks), dim3(T2T), 0, st, P2, kp, nkt,

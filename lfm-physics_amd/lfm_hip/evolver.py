@@ -185,7 +185,7 @@ class Evolver:
             combos = [(0, ch) for ch in dict.fromkeys((0, N, N // 2, N // 4, N // 8)) if ch == 0 or ch >= 4]
         elif pairs:
             # the two-step kernel has one tile shape; tune its chunk
-            combos = [(0, ch) for ch in dict.fromkeys((0, 8, 16, 32, 64)) if ch <= N]
+            combos = [(0, ch) for ch in dict.fromkeys((0, 2, 4, 8, 16, 32, 64)) if ch <= N]
         else:
             combos = [(t, ch) for t in range(1, _abi.LFM_TILE_VARIANTS + 1)
                       for ch in dict.fromkeys(min(c, N) for c in chunks)]
