@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define LFM_ABI_VERSION 1
+#define LFM_ABI_VERSION 2
 
 /* Error codes. */
 #define LFM_OK 0

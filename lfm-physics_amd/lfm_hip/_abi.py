@@ -23,6 +23,7 @@ LFM_F_DAMP_PREV = 0x02
 LFM_F_FREEZE_CHI = 0x04
 LFM_F_DIRECT = 0x08
 LFM_F_CHI_DELTA = 0x10
+LFM_ABI_VERSION = 2
 LFM_TILE_VARIANTS = 6
 LFM_RELAX_SLOT_WORDS = 2048
 
@@ -102,8 +103,9 @@ def load(path: str | None = None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.lfm_abi_version() != 1:
-        raise ImportError(f"liblfm_hip ABI version {lib.lfm_abi_version()} != 1")
+    if lib.lfm_abi_version() != LFM_ABI_VERSION:
+        raise ImportError(f"liblfm_hip ABI version {lib.lfm_abi_version()} != {LFM_ABI_VERSION}: "
+                          "rebuild the library")
     _lib = lib
     return lib
 
