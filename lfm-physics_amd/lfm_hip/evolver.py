@@ -68,6 +68,7 @@ class Evolver:
     """
 
     GRAPH_CYCLES = 8
+    GRAPH_CACHE = 24     # captured graphs kept per Evolver
 
     def __init__(self, config: SimulationConfig, backend="auto", mask_policy="reference",
                  chi_storage="delta", sa_transform="auto", two_step=False, graphs=True):
@@ -405,6 +406,9 @@ class Evolver:
         gkey = key0 + flags0 + (self.tile, self.chunk, pair, cycles, hook)
         if gkey in self._graphs:
             return self._graphs[gkey]
+        while len(self._graphs) >= self.GRAPH_CACHE:
+            # bounded cache (each hook object keys its own graphs): drop the oldest
+            self._graphs.pop(next(iter(self._graphs)))
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))
         g = torch.cuda.CUDAGraph()
