@@ -164,3 +164,17 @@ def test_python_constants_match_header_defines():
     assert {"LFM_F_MASK", "LFM_F_CHI_DELTA", "LFM_RELAX_SLOT_WORDS"} <= set(shared)
     for k in shared:
         assert getattr(_abi, k) == defs[k], k
+
+
+def test_params_struct_layout():
+    """LfmParams mirrors lfm_params_t: two int32, ten float32, two int32,
+    packed in declaration order (no padding)."""
+    from lfm_hip import _abi
+    names = [n for n, _ in _abi.LfmParams._fields_]
+    assert names[:2] == ["N", "flags"] and names[-2:] == ["tile", "chunk"]
+    assert ctypes.sizeof(_abi.LfmParams) == 14 * 4
+    assert _abi.LfmParams.tile.offset == 12 * 4
+    text = open(HEADER).read()
+    body = re.search(r"typedef struct lfm_params \{(.*?)\} lfm_params_t;", text, re.S).group(1)
+    fields = re.findall(r"[A-Za-z_0-9]+(?=[,;])", body)
+    assert fields == names, (fields, names)
