@@ -853,3 +853,40 @@ def test_device_hook_matches_oracle(gpu):
     e = float(np.linalg.norm(psi - np.concatenate([st.psi_r, st.psi_i])) / norm0)
     log(f"  psi error / |psi_0| {e:.2e}")
     assert e <= TOL64
+
+
+@pytest.mark.parametrize("periodic,steps", [(True, 1000), (False, 300)])
+def test_real64_energy_trace_matches_oracle(gpu, periodic, steps):
+    """The REAL 64^3 gravity_only configuration (one soliton, amplitude 6,
+    width 3, N(0, 1e-3) noise, E_prev = E): total energy with the true
+    leapfrog prev every 100 steps, GPU (graph-replayed steps, FP64 sums)
+    against the FP64 oracle.  Every checkpoint within 1e-5 relative, and the
+    drift E(t) - E(0) within 1e-5 of E(0) of the oracle's drift."""
+    import torch
+    import lfm_hip as L
+    N = 64
+    log(f"REAL 64^3 energy trace, periodic={periodic}, {steps} steps")
+    kw = dict(boundary_type=L.BoundaryType.PERIODIC) if periodic else {}
+    cfg = L.gravity_only(N, **kw)
+    E0, _ = make_ics(0, N, seed=42)
+    ev = L.Evolver(cfg, backend="hip", mask_policy="boundary_type" if periodic else "reference")
+    ev.set_psi_real(E0)
+    mask = np.zeros(N ** 3) if periodic else O.boundary_mask(N, cfg.boundary_fraction)
+    st = O.OracleState(0, N, E0, None, np.full(N ** 3, cfg.chi0), mask, oracle_params(cfg))
+    sh = (N, N, N)
+
+    def oracle_energy():
+        return O.total_energy(st.psi_r.reshape(sh), st.psi_r_prev.reshape(sh), st.chi.reshape(sh), cfg.dt)
+
+    g0, o0 = ev.total_energy(), oracle_energy()
+    worst_e = worst_d = 0.0
+    for t in range(100, steps + 1, 100):
+        ev.evolve(100)
+        st.step(100)
+        torch.cuda.synchronize()
+        g, o = ev.total_energy(), oracle_energy()
+        worst_e = max(worst_e, abs(g - o) / abs(o))
+        worst_d = max(worst_d, abs((g - g0) - (o - o0)) / abs(o0))
+    log(f"  E(0) {o0:.6e}, drift {(o - o0) / o0:+.3e} (oracle) {(g - g0) / g0:+.3e} (GPU); "
+        f"worst rel {worst_e:.2e}, worst drift diff {worst_d:.2e}")
+    assert worst_e <= 1e-5 and worst_d <= 1e-5
