@@ -890,3 +890,31 @@ def test_real64_energy_trace_matches_oracle(gpu, periodic, steps):
     log(f"  E(0) {o0:.6e}, drift {(o - o0) / o0:+.3e} (oracle) {(g - g0) / g0:+.3e} (GPU); "
         f"worst rel {worst_e:.2e}, worst drift diff {worst_d:.2e}")
     assert worst_e <= 1e-5 and worst_d <= 1e-5
+
+
+def test_color64_configuration_vs_fp64_oracle(gpu):
+    """The COLOR full_physics configuration at the size it is checked against
+    the CPU oracle (64^3): three blobs, one colour dominant each, centres
+    uniform in the ball of radius 0.35 N/2 about the centre, amplitude 4,
+    width 3, random phases, chi = 19, S_a every step (the fused k-pass path
+    the Evolver picks at 64^3), 100 steps through Evolver.evolve (graph
+    replay): every field within 1e-5 relative L2 of the FP64 oracle."""
+    import torch
+    import lfm_hip as L
+    sys.path.insert(0, os.path.dirname(HERE))
+    import bench
+    N, steps = 64, 100
+    log(f"COLOR full_physics {N}^3 configuration vs FP64 oracle, {steps} steps")
+    cfg = L.full_physics(N)
+    pr_t, pi_t = bench.gpu_ics(bench.WORKLOADS["color512"], N)
+    pr, pi = pr_t.cpu().numpy(), pi_t.cpu().numpy()
+    ev = L.Evolver(cfg, backend="hip")
+    assert ev._fused
+    ev.set_psi_real(pr)
+    ev.set_psi_imag(pi)
+    ev.evolve(steps)
+    torch.cuda.synchronize()
+    st = O.OracleState(2, N, pr, pi, np.full(N ** 3, cfg.chi0), O.boundary_mask(N, cfg.boundary_fraction),
+                       oracle_params(cfg))
+    st.step(steps)
+    check_fields(ev, st, TOL64, "color64_config")
