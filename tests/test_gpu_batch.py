@@ -170,3 +170,24 @@ def test_batch_color_with_sa_equals_single_runs(gpu, sizes, group_bytes):
             np.testing.assert_array_equal(batch.get_psi_real(b), ev.get_psi_real(), err_msg=f"psi_r {b}")
             np.testing.assert_array_equal(batch.get_psi_imag(b), ev.get_psi_imag(), err_msg=f"psi_i {b}")
             np.testing.assert_array_equal(batch.get_chi(b), ev.get_chi(), err_msg=f"chi {b}")
+
+
+def test_batched_sweep_2d_rows(gpu):
+    """sweep_2d (lfm/sweep.py:85-157): one row per (value1, value2) pair in
+    the reference's order, each equal to its own batched lattice run alone."""
+    import lfm_hip as L
+    from lfm_hip.batch import sweep, sweep_2d
+    log("batched sweep_2d")
+    base = L.SimulationConfig(grid_size=16, field_level=L.FieldLevel.COMPLEX)
+    k_vals, l_vals = [1 / 63, 2 / 63], [0.0, 4 / 31]
+    rows = sweep_2d(base, "kappa", k_vals, "lambda_self", l_vals, 20, soliton={"amplitude": 3.0},
+                    backend="hip", metric_names=["chi_min", "energy_total"])
+    assert [(r["kappa"], r["lambda_self"]) for r in rows] == [(k, l) for k in k_vals for l in l_vals]
+    assert all(set(r) == {"kappa", "lambda_self", "chi_min", "energy_total"} for r in rows)
+    for r in rows:
+        from copy import deepcopy
+        c = deepcopy(base)
+        c.lambda_self = r["lambda_self"]
+        one = sweep(c, "kappa", [r["kappa"]], 20, soliton={"amplitude": 3.0}, backend="hip",
+                    metric_names=["chi_min", "energy_total"])[0]
+        assert one["chi_min"] == r["chi_min"] and one["energy_total"] == r["energy_total"]
