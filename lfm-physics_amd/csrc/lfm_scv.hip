@@ -187,6 +187,35 @@ struct ColArgs {
 template <int N>
 constexpr int ncol_for() { return N >= 1024 ? 8 : 16; }
 
+// The i-axis filter on a line held as positions t + e*N/8: multiply by
+// gamma/(gamma + D k^2) (FILTER 0) or -1/k^2 (FILTER 1, zero at k = 0), times
+// the 1/N^3 normalisation, at (k_i = position, k_j = ky, k_kk = kk).  The
+// quotient is the hardware reciprocal refined by one Newton step (within
+// 1 ulp of the correctly rounded one).
+template <int N, int FILTER>
+__device__ __forceinline__ void filter_line(float2 (&v)[8], int t, int kk, float ky,
+                                            const ColArgs& A) {
+    constexpr int TC = N / 8;
+    const float two_pi_over_n = 6.283185307179586f / float(N);
+    const float kz = two_pi_over_n * float(kk);
+    const float k2jk = ky * ky + kz * kz;
+    const float num = FILTER == 0 ? A.gamma * A.scale : -A.scale;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int i = t + e * TC;
+        const int fi = i < (N + 1) / 2 ? i : i - N;
+        const float kx = two_pi_over_n * float(fi);
+        const float k2 = kx * kx + k2jk;
+        const float d = FILTER == 0 ? A.gamma + A.D * k2 : k2;
+        float r = __builtin_amdgcn_rcpf(d);
+        r = fmaf(fmaf(-d, r, 1.0f), r, r);
+        float h = num * r;
+        if (FILTER != 0 && !(k2 > 0.0f)) h = 0.0f;
+        v[e].x *= h;
+        v[e].y *= h;
+    }
+}
+
 // Workgroup = NCOL adjacent kk columns x TC threads per column; lane order
 // is column-fastest so each load/store instruction moves whole 128-byte rows
 // and LDS rows ([pos][col]) are read and written contiguously.  A workgroup
@@ -194,7 +223,10 @@ constexpr int ncol_for() { return N >= 1024 ? 8 : 16; }
 // group's loads are issued before the current group is transformed), so
 // the N rows of a slab, which for the i axis lie 8 N Hp bytes apart, are
 // opened once per `rounds` groups.
-template <int N>
+// MODE (0 forward, 1 inverse, 2 forward + filter + inverse) and FILTER are
+// template parameters so the butterflies' rotations and twiddle conjugates
+// fold at compile time.
+template <int N, int MODE, int FILTER>
 __global__ __launch_bounds__(ncol_for<N>() * (N / 8)) void fft_columns(ColArgs A) {
     constexpr int NCOL = ncol_for<N>();
     constexpr int TC = N / 8;
@@ -207,56 +239,57 @@ __global__ __launch_bounds__(ncol_for<N>() * (N / 8)) void fft_columns(ColArgs A
     const int slab = blockIdx.x / nbs;         // a * N + x
     const int a = slab / N, x = slab % N;
     const int col = threadIdx.x % NCOL, t = threadIdx.x / NCOL;
-    float2* sbase = A.spec + (long long)a * N * N * A.Hp + (long long)x * A.slab_stride + col;
+    float2* sbase = A.spec + (long long)a * N * N * A.Hp + (long long)x * A.slab_stride;
+    const int ng = min(A.rounds, NCB - cb0);   // column groups this workgroup takes
     load_twiddles<N>(twl, A.tw);
     auto at = [col](int pos) { return pos * NCOL + col; };
     const float two_pi_over_n = 6.283185307179586f / float(N);
     const int fj = x < (N + 1) / 2 ? x : x - N;
     const float ky = two_pi_over_n * float(fj);
     float2 v[8], vn[8];
+    // Loads carry no predicate: lanes past the last live column read column
+    // H - 1, a line the live lanes fetch anyway.  Branch-free loads let the
+    // compiler wait only for the current group's values, so the next
+    // group's loads stay in flight while this one is transformed.
     auto load = [&](float2 (&dst)[8], int cb) {
-        const bool live = cb * NCOL + col < H;
-        const float2* b = sbase + cb * NCOL;
+        const float2* b = sbase + min(cb * NCOL + col, H - 1);
 #pragma unroll
-        for (int e = 0; e < 8; ++e)
-            dst[e] = live ? ld2nt<N>(b + (long long)(t + e * TC) * A.row_stride) : make_float2(0.f, 0.f);
+        for (int e = 0; e < 8; ++e) dst[e] = ld2nt<N>(b + (long long)(t + e * TC) * A.row_stride);
     };
-    load(v, cb0);
-    for (int r = 0; r < A.rounds; ++r) {
-        const int cb = cb0 + r;
-        const bool more = r + 1 < A.rounds && cb + 1 < NCB;
-        if (more) load(vn, cb + 1);
+    // `pending` (the prefetched group, or none) is waited for before this
+    // group's stores issue: the wait counter covers loads and stores
+    // together, and a wait issued while stores are pending drains them too.
+    auto transform_store = [&](int cb, float2 (*pending)[8]) {
         const int kk = cb * NCOL + col;
         // the first stage needs no twiddles; the first boundary's barriers
         // publish the table before any stage reads it
-        fft_line<N>(v, lds, t, A.mode == 1 ? 1 : -1, twl, at);
-        if (A.mode == 2) {
-            // filter at (k_i = position, k_j = x, k_kk = kk)
-            const float kz = two_pi_over_n * float(kk);
-            const float k2jk = ky * ky + kz * kz;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int i = t + e * TC;
-                const int fi = i < (N + 1) / 2 ? i : i - N;
-                const float kx = two_pi_over_n * float(fi);
-                const float k2 = kx * kx + k2jk;
-                float h;
-                if (A.filter == 0) h = A.gamma / (A.gamma + A.D * k2) * A.scale;
-                else h = (k2 > 0.0f) ? -A.scale / k2 : 0.0f;
-                v[e].x *= h;
-                v[e].y *= h;
-            }
+        fft_line<N>(v, lds, t, MODE == 1 ? 1 : -1, twl, at);
+        if constexpr (MODE == 2) {
+            filter_line<N, FILTER>(v, t, kk, ky, A);
             fft_line<N>(v, lds, t, 1, twl, at);
         }
+        if (pending) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) asm volatile("" ::"v"((*pending)[e].x), "v"((*pending)[e].y));
+        }
         if (kk < H) {
-            float2* b = sbase + cb * NCOL;
+            float2* b = sbase + kk;
 #pragma unroll
             for (int e = 0; e < 8; ++e) st2nt<N>(b + (long long)(t + e * TC) * A.row_stride, v[e]);
         }
-        if (!more) break;
+    };
+    load(v, cb0);
+    // wait for the first group here, so no load is pending where the loop
+    // is entered and the loop's waits need not cover earlier stores
+#pragma unroll
+    for (int e = 0; e < 8; ++e) asm volatile("" ::"v"(v[e].x), "v"(v[e].y));
+    for (int r = 0; r + 1 < ng; ++r) {
+        load(vn, cb0 + r + 1);
+        transform_store(cb0 + r, &vn);
 #pragma unroll
         for (int e = 0; e < 8; ++e) v[e] = vn[e];
     }
+    transform_store(cb0 + ng - 1, nullptr);
 }
 
 // Lines along the contiguous k axis.  Two real lines p0 = 2q, p1 = 2q+1 form
@@ -356,7 +389,11 @@ int launch_columns(const ColArgs& A, hipStream_t st) {
     constexpr int H = N / 2 + 1;
     constexpr int NCB = (H + NCOL - 1) / NCOL;
     const long long nb = (long long)A.nc * N * ((NCB + A.rounds - 1) / A.rounds);
-    hipLaunchKernelGGL(fft_columns<N>, dim3(unsigned(nb)), dim3(NCOL * (N / 8)), 0, st, A);
+    const dim3 g{unsigned(nb)}, b{unsigned(NCOL * (N / 8))};
+    if (A.mode == 0) hipLaunchKernelGGL((fft_columns<N, 0, 0>), g, b, 0, st, A);
+    else if (A.mode == 1) hipLaunchKernelGGL((fft_columns<N, 1, 0>), g, b, 0, st, A);
+    else if (A.filter == 0) hipLaunchKernelGGL((fft_columns<N, 2, 0>), g, b, 0, st, A);
+    else hipLaunchKernelGGL((fft_columns<N, 2, 1>), g, b, 0, st, A);
     LFM_LAUNCH_CHECK("fft_columns");
     return LFM_OK;
 }
@@ -429,8 +466,15 @@ void destroy(ScvPlan* p) {
     delete p;
 }
 
-// Column groups per workgroup for the j passes and the i pass.
-constexpr int kRoundsJ = 2, kRoundsI = 4;
+// Column groups per workgroup: 4 along j; along i a workgroup takes its
+// whole slab (N rows of N/2+1 columns), which opens each row of the slab
+// once.  Measured at 512^3 (three passes): j/i rounds 2/4 2.10 ms, 4/4
+// 2.08, 2/17 2.05, 4/17 2.04, 6/17 2.09, 4/9 2.08.
+constexpr int kRoundsJ = 4;
+int whole_slab_rounds(int N) {
+    const int ncol = N >= 1024 ? 8 : 16;   // ncol_for<N>()
+    return (N / 2 + 1 + ncol - 1) / ncol;
+}
 
 // The three column passes of the direct method on the spectrum in place.
 int run_columns(ScvPlan* p, int filter, float gamma, float D, hipStream_t st, int ncomp_used) {
@@ -444,7 +488,7 @@ int run_columns(ScvPlan* p, int filter, float gamma, float D, hipStream_t st, in
     if (rc) return rc;
     // along i: forward, filter, inverse in one pass; slab (a, j), rows i
     A.slab_stride = p->Hp; A.row_stride = (long long)N * p->Hp; A.mode = 2;
-    A.rounds = kRoundsI;
+    A.rounds = whole_slab_rounds(N);
     rc = columns(N, A, st);
     if (rc) return rc;
     // inverse along j
