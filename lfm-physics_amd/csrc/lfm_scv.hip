@@ -361,18 +361,33 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_c2r(const float2* 
     const float2* s0 = spec + pair * 2 * Hp;
     const float2* s1 = s0 + Hp;
     load_twiddles<N>(twl, tw);
+    // The two half spectra go through LDS once (coalesced, each value read
+    // once from HBM); the mirrored reads X[N-k] = conj X[k] then come from LDS.
+    constexpr int H = N / 2 + 1;
+    static_assert(2 * H <= PS, "both half spectra fit the line's LDS slot");
+    float2* L = lds + q * PS;
+#pragma unroll
+    for (int e = 0; e < (H + TC - 1) / TC; ++e) {
+        const int k = t + e * TC;
+        if (k < H) {
+            L[k] = ld2nt<N>(s0 + k);
+            L[H + k] = ld2nt<N>(s1 + k);
+        }
+    }
+    __syncthreads();
     float2 v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int k = t + e * TC;
         const bool lo = k <= N / 2;
         const int m = lo ? k : N - k;
-        float2 a = s0[m], b = s1[m];
+        float2 a = L[m], b = L[H + m];
         if (m == 0 || m == N / 2) { a.y = 0.0f; b.y = 0.0f; }
         if (!lo) { a.y = -a.y; b.y = -b.y; }
         v[e] = make_float2(a.x - b.y, a.y + b.x);
     }
-    float2* L = lds + q * PS;
+    // the transform's first exchange starts with a barrier, which orders
+    // these reads before its writes
     fft_line<N>(v, L, t, 1, twl, [](int pos) { return pos + pos / 8; });
     float* r0 = out + pair * 2 * N;
     float* r1 = r0 + N;
