@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define LFM_ABI_VERSION 2
+#define LFM_ABI_VERSION 3
 
 /* Error codes. */
 #define LFM_OK 0
@@ -248,10 +248,17 @@ int lfm_chi_stats_f64(const float* chi, double chi_offset, int32_t N, double bou
  * periodic 19-point Laplacian, and max |out| folded into max_slot.  A slot is
  * LFM_RELAX_SLOT_WORDS device words (the max is spread over 64 of them, one
  * per 128-byte line, holding non-negative float bits); the caller zeroes a
- * slot before its first use.  3 <= N <= 2048; out must not overlap E or
- * chi2. */
+ * slot before its first use.
+ * scale_slot, when not NULL, folds the previous step's renormalisation in:
+ * E holds that step's output before its rescale and scale_slot its max, and
+ * every E value is read as E * (amplitude / max) (the same float32 product
+ * lfm_relax_scale_f32 would have stored when max > 1e-8), so a step costs
+ * one launch.  clear_slot, when not NULL, is zeroed (the slot two steps on);
+ * it must differ from max_slot and scale_slot.  3 <= N <= 2048; out must not
+ * overlap E or chi2. */
 #define LFM_RELAX_SLOT_WORDS 2048
 int lfm_relax_step_f32(const float* E, const float* chi2, float* out, uint32_t* max_slot,
+                       const uint32_t* scale_slot, uint32_t* clear_slot, float amplitude,
                        int32_t N, float dt, void* stream);
 /* The step's renormalisation: E *= amplitude / max (the max held in
  * max_slot) when max > 1e-8.  When E_prev is not NULL, max |E - E_prev| after

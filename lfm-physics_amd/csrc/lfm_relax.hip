@@ -1,9 +1,11 @@
 // Imaginary-time eigenmode relaxation (lfm/particles/solver.py:620-826):
 //   E_new = E + dt (lap E - chi^2 E),   E_new *= amplitude / max|E_new|,
-// in frozen chi, between Poisson updates of chi.  Two kernels per step: the
-// stencil update, which also folds max|E_new| into a device word, and the
-// rescale, which reads that word (and, at convergence checks, folds
-// max|E_new - E| into another).  Relaxed lattices are small (the reference
+// in frozen chi, between Poisson updates of chi.  The stencil update folds
+// max|E_new| into a device word; the rescale either runs as its own kernel
+// (which, at convergence checks, also folds max|E_new - E| into another
+// word) or is folded into the next update, which reads the unscaled values
+// and multiplies each by amplitude / max as it loads it: the same float32
+// products, one launch per step.  Relaxed lattices are small (the reference
 // runs them at N = 64) and stay in the last-level cache, so the update
 // gathers its 19 neighbours directly instead of staging tiles.
 #include "lfm_internal.h"
@@ -55,13 +57,36 @@ __device__ __forceinline__ void block_max_publish(float v, uint32_t* slot) {
     }
 }
 
+// amplitude / max formed in double and rounded once, as the reference's
+// `E_new *= amplitude / E_max` applies it; 1 (no rescale) when max <= 1e-8
+__device__ __forceinline__ float rescale_factor(float mx, float amplitude, bool& scale) {
+    scale = double(mx) > 1e-8;
+    return scale ? float(double(amplitude) / double(mx)) : 1.0f;
+}
+
 // One thread per site on a (k, j, i) grid of 64 x 4 x 1 workgroups, so the
 // site coordinates come from the launch, not from integer division.
-__global__ __launch_bounds__(kThreads) void relax_step_kernel(const float* __restrict__ E,
+
+__global__ __launch_bounds__(kThreads) void relax_step_kernel(const float* __restrict__ E_raw,
                                                               const float* __restrict__ chi2,
                                                               float* __restrict__ out,
-                                                              uint32_t* max_slot, int N, float dt) {
+                                                              uint32_t* max_slot,
+                                                              const uint32_t* scale_slot,
+                                                              uint32_t* clear_slot,
+                                                              float amplitude, int N, float dt) {
     const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+    bool scale = false;
+    const float s = scale_slot ? rescale_factor(slot_read(scale_slot), amplitude, scale) : 1.0f;
+    // E values as the previous step's rescale leaves them
+    struct Rd {
+        const float* p;
+        float s;
+        bool on;
+        __device__ __forceinline__ float operator[](size_t t) const {
+            const float v = p[t];
+            return on ? __fmul_rn(v, s) : v;
+        }
+    } const E{E_raw, s, scale};
     float m = 0.0f;
     if (k < N && j < N) {
         const size_t nn = size_t(N) * N;
@@ -86,6 +111,8 @@ __global__ __launch_bounds__(kThreads) void relax_step_kernel(const float* __res
         out[t] = v;
         m = fabsf(v);
     }
+    if (clear_slot && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.y == 0)
+        clear_slot[threadIdx.x * kStride] = 0u;   // 64 lanes, one word per line
     block_max_publish(m, max_slot);
 }
 
@@ -95,11 +122,8 @@ __global__ __launch_bounds__(kThreads) void relax_scale_kernel(float* __restrict
                                                                uint32_t* clear_slot,
                                                                uint32_t* diff_slot, float amplitude,
                                                                size_t n) {
-    const float mx = slot_read(max_slot);
-    // amplitude / E_max is formed in double and applied as one float32
-    // multiply, as the reference's `E_new *= amplitude / E_max` is
-    const bool scale = double(mx) > 1e-8;
-    const float s = scale ? float(double(amplitude) / double(mx)) : 1.0f;
+    bool scale = false;
+    const float s = rescale_factor(slot_read(max_slot), amplitude, scale);
     float d = 0.0f;
     for (size_t t = size_t(blockIdx.x) * kThreads + threadIdx.x; t < n;
          t += size_t(gridDim.x) * kThreads) {
@@ -125,6 +149,7 @@ unsigned grid_of(size_t n) {
 using namespace lfm;
 
 extern "C" int lfm_relax_step_f32(const float* E, const float* chi2, float* out, uint32_t* max_slot,
+                                  const uint32_t* scale_slot, uint32_t* clear_slot, float amplitude,
                                   int32_t N, float dt, void* stream) {
     if (!E || !chi2 || !out || !max_slot || N < 3 || N > 2048) {
         set_error("lfm_relax_step_f32: bad argument (pointers non-NULL, 3 <= N <= 2048)");
@@ -135,9 +160,16 @@ extern "C" int lfm_relax_step_f32(const float* E, const float* chi2, float* out,
         set_error("lfm_relax_step_f32: out overlaps an input");
         return LFM_E_ARG;
     }
+    constexpr size_t SB = LFM_RELAX_SLOT_WORDS * 4;
+    if ((scale_slot && overlaps(scale_slot, SB, max_slot, SB)) ||
+        (clear_slot && (overlaps(clear_slot, SB, max_slot, SB) ||
+                        (scale_slot && overlaps(clear_slot, SB, scale_slot, SB))))) {
+        set_error("lfm_relax_step_f32: max_slot, scale_slot and clear_slot must be distinct");
+        return LFM_E_ARG;
+    }
     const dim3 grid(unsigned((N + 63) / 64), unsigned((N + 3) / 4), unsigned(N));
     hipLaunchKernelGGL(relax_step_kernel, grid, dim3(64, 4), 0, (hipStream_t)stream, E, chi2, out,
-                       max_slot, N, dt);
+                       max_slot, scale_slot, clear_slot, amplitude, N, dt);
     LFM_LAUNCH_CHECK("relax_step_kernel");
     return LFM_OK;
 }

@@ -23,7 +23,7 @@ LFM_F_DAMP_PREV = 0x02
 LFM_F_FREEZE_CHI = 0x04
 LFM_F_DIRECT = 0x08
 LFM_F_CHI_DELTA = 0x10
-LFM_ABI_VERSION = 2
+LFM_ABI_VERSION = 3
 LFM_TILE_VARIANTS = 6
 LFM_RELAX_SLOT_WORDS = 2048
 
@@ -57,7 +57,8 @@ SIGNATURES = [
     ("lfm_batch_step_f32", ctypes.c_int, [_vp] * 13 + [_i32, _vp]),
     ("lfm_adjoint_partials_len", _i64, [_i32]),
     ("lfm_calib_stream_f32", ctypes.c_int, [_vp, _i32, _vp, _i32, ctypes.c_int64, _i32, _i32, _vp]),
-    ("lfm_relax_step_f32", ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, ctypes.c_float, _vp]),
+    ("lfm_relax_step_f32", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_float, _i32,
+                                          ctypes.c_float, _vp]),
     ("lfm_relax_scale_f32", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_int64,
                                            _vp]),
     ("lfm_adjoint_complex_seed_f32", ctypes.c_int, [_vp] * 8 + [_vp, _vp]),

@@ -90,6 +90,7 @@ class _Relaxer:
         self.sw = _abi.LFM_RELAX_SLOT_WORDS
         self.slots = torch.zeros(4 * self.sw, dtype=torch.int32, device=self.dev)
         self.plan = ScvPlan(N, 1)
+        self._graphs = {}
 
     def slot(self, i: int) -> int:
         return self.slots.data_ptr() + 4 * self.sw * i
@@ -110,28 +111,87 @@ class _Relaxer:
         self.chi.masked_fill_(self.shell, self.chi0)
         torch.mul(self.chi, self.chi, out=self.chi2)
 
+    # Steps between convergence checks run as fused launches (the previous
+    # step's rescale folded into the update), so E holds the last output
+    # before its rescale; six steps return the three max slots and the two
+    # E buffers to where they started, so blocks of six replay from graphs.
+    GRAPH_STEPS = 6
+
+    def _step(self, k: int, scale_k, amplitude: float, dt: float) -> None:
+        """One update into slot k; scale_k: slot of the pending rescale of E
+        (None when E holds rescaled values)."""
+        _abi.check(self.lib.lfm_relax_step_f32(
+            self.E.data_ptr(), self.chi2.data_ptr(), self.E_alt.data_ptr(), self.slot(k),
+            None if scale_k is None else self.slot(scale_k), self.slot((k + 1) % 3),
+            float(amplitude), self.N, float(dt), stream_handle()), "lfm_relax_step_f32")
+        self.E, self.E_alt = self.E_alt, self.E
+
+    def _rescale(self, k: int, amplitude: float, E_prev=None) -> None:
+        """E *= amplitude / max(slot k); with E_prev, max|E - E_prev| into slot 3."""
+        _abi.check(self.lib.lfm_relax_scale_f32(
+            self.E.data_ptr(), None if E_prev is None else E_prev.data_ptr(), self.slot(k), None,
+            None if E_prev is None else self.slot(3), float(amplitude), self.n3, stream_handle()),
+            "lfm_relax_scale_f32")
+
+    def _block_graph(self, k0: int, amplitude: float, dt: float):
+        """Graph of GRAPH_STEPS fused steps from slot phase k0 (E pending the
+        rescale held in slot (k0 - 1) % 3), keyed by the phase, the buffer
+        order and the scalars."""
+        key = (k0, self.E.data_ptr(), float(amplitude), float(dt))
+        g = self._graphs.get(key)
+        if g is None:
+            s = torch.cuda.Stream(device=self.dev)
+            s.wait_stream(torch.cuda.current_stream(self.dev))
+            g = torch.cuda.CUDAGraph()
+            E0, E1 = self.E, self.E_alt
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(g, stream=s):
+                    for u in range(self.GRAPH_STEPS):
+                        k = (k0 + u) % 3
+                        self._step(k, (k - 1) % 3, amplitude, dt)
+            torch.cuda.current_stream(self.dev).wait_stream(s)
+            # capture recorded the launches without running them; six swaps
+            # leave the buffers in their starting order
+            assert self.E is E0 and self.E_alt is E1
+            self._graphs[key] = g
+        return g
+
     def relax(self, amplitude: float, dt: float, steps: int, check_interval: int,
-              tolerance: float) -> tuple[bool, int]:
-        """Up to `steps` relaxation steps; (converged, steps taken)."""
-        lib, st = self.lib, stream_handle()
+              tolerance: float, graphs: bool = True) -> tuple[bool, int]:
+        """Up to `steps` relaxation steps; (converged, steps taken).  E holds
+        rescaled values on return."""
         self.slots.zero_()
-        for step in range(1, steps + 1):
+        pending = None         # slot whose max E still has to be rescaled by
+        step = 0
+        while step < steps:
+            step += 1
             k = step % 3
-            _abi.check(lib.lfm_relax_step_f32(self.E.data_ptr(), self.chi2.data_ptr(),
-                                              self.E_alt.data_ptr(), self.slot(k), self.N,
-                                              float(dt), st), "lfm_relax_step_f32")
-            check = step % check_interval == 0
-            if check:
+            if step % check_interval == 0:
+                # convergence check: max|E_n - E_{n-1}| over rescaled fields
+                if pending is not None:
+                    self._rescale(pending, amplitude)
+                    pending = None
+                self._step(k, None, amplitude, dt)
                 self.slots[3 * self.sw:].zero_()
-            _abi.check(lib.lfm_relax_scale_f32(
-                self.E_alt.data_ptr(), self.E.data_ptr() if check else None, self.slot(k),
-                self.slot((k + 1) % 3), self.slot(3) if check else None, float(amplitude),
-                self.n3, st), "lfm_relax_scale_f32")
-            self.E, self.E_alt = self.E_alt, self.E
-            if check:
-                dE = self.slot_max(3)
-                if dE < tolerance:
+                self._rescale(k, amplitude, E_prev=self.E_alt)
+                if self.slot_max(3) < tolerance:
                     return True, step
+                continue
+            # fused steps up to the next check; whole blocks of six from graphs
+            nxt = min(steps, (step // check_interval + 1) * check_interval - 1)
+            if graphs and pending is not None:
+                while nxt - step + 1 >= self.GRAPH_STEPS:
+                    self._block_graph(k, amplitude, dt).replay()
+                    step += self.GRAPH_STEPS
+                    k = step % 3
+                    pending = (k - 1) % 3
+                if step > nxt:
+                    step -= 1
+                    continue
+            self._step(k, pending, amplitude, dt)
+            pending = k
+        if pending is not None:
+            self._rescale(pending, amplitude)
         return False, steps
 
 

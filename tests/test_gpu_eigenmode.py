@@ -56,3 +56,51 @@ def test_relax_converging_run_matches_oracle(gpu):
     assert sol.converged == conv and sol.cycles == total
     e = float(np.linalg.norm(sol.psi_r.astype(np.float64) - E) / np.linalg.norm(E))
     assert e <= 1e-5 and abs(sol.eigenvalue - omega) / omega <= 1e-5
+
+
+@pytest.mark.parametrize("steps,check", [(37, 10), (200, 50), (13, 200)])
+def test_fused_relax_steps_equal_separate_rescale(gpu, steps, check):
+    """Steps with the previous rescale folded into the update (eager and
+    replayed from six-step graphs) give the same bits as one update launch
+    and one rescale launch per step, including the convergence-check value."""
+    import torch
+    from lfm_hip import _abi
+    from lfm_hip.backend import stream_handle
+    from lfm_hip.eigenmode import _Relaxer, gaussian_seed
+    from lfm_hip.evolver import get_backend
+    N, amp = 40, 9.0
+    dt = float(np.float32(0.8 / (19.0 ** 2 + 12.0)))
+    log(f"fused vs separate relaxation steps={steps} check={check}")
+    seed = gaussian_seed(N, (N // 2 - 1, N // 2, N // 2 + 2), amp, 3.5)
+    runs = []
+    for mode in ("graphs", "eager", "separate"):
+        R = _Relaxer(seed, N, 19.0, 1 / 63, 0.3, get_backend("hip"))
+        R.poisson()
+        if mode != "separate":
+            R.relax(amp, dt, steps, check, -1.0, graphs=(mode == "graphs"))
+            diff = R.slot_max(3) if steps >= check else None
+        else:
+            lib, st = _abi.load(), stream_handle()
+            R.slots.zero_()
+            diff = None
+            for n in range(1, steps + 1):
+                k = n % 3
+                _abi.check(lib.lfm_relax_step_f32(R.E.data_ptr(), R.chi2.data_ptr(),
+                                                  R.E_alt.data_ptr(), R.slot(k), None,
+                                                  R.slot((k + 1) % 3), float(amp), N, float(dt),
+                                                  st), "step")
+                R.E, R.E_alt = R.E_alt, R.E
+                chk = n % check == 0
+                if chk:
+                    R.slots[3 * R.sw:].zero_()
+                _abi.check(lib.lfm_relax_scale_f32(R.E.data_ptr(),
+                                                   R.E_alt.data_ptr() if chk else None, R.slot(k),
+                                                   None, R.slot(3) if chk else None, float(amp),
+                                                   R.n3, st), "scale")
+                if chk:
+                    diff = R.slot_max(3)
+        torch.cuda.synchronize()
+        runs.append((R.E.cpu().numpy(), diff))
+    for E, d in runs[:2]:
+        np.testing.assert_array_equal(E, runs[2][0])
+        assert d == runs[2][1]
