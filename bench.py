@@ -418,11 +418,16 @@ def run_relax(args):
     print(json.dumps(out), flush=True)
 
 
+def tune_key(tile, chunk):
+    """Autotuner entry label: tile/chunk, or "direct" for the one-thread-per-site kernel."""
+    return "direct" if tile < 0 else f"{tile}/{chunk}"
+
+
 def batch_tile(tuning):
     if not tuning:
         return None
     best = min(tuning, key=tuning.get)
-    return {"best": list(best), "ms": {f"{t}/{c}": round(v, 4) for (t, c), v in tuning.items()}}
+    return {"best": list(best), "ms": {tune_key(t, c): round(v, 4) for (t, c), v in tuning.items()}}
 
 
 def main():
@@ -569,7 +574,7 @@ def main():
                      "steps_per_launch": 2 if pairs else 1},
         "finite": ok,
         "tile": {"shape": ev.tile, "chunk": ev.chunk,
-                 "tuned_ms": {f"{t}/{c}": round(v, 4) for (t, c), v in tuning.items()} if tuning else None},
+                 "tuned_ms": {tune_key(t, c): round(v, 4) for (t, c), v in tuning.items()} if tuning else None},
     }
     if not args.no_cpu_baseline and args.size is None:
         cb = cpu_baseline(w)

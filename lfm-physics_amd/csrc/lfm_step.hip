@@ -12,8 +12,10 @@
 //                       lap(i) = A(i) + B(i-1) + B(i+1)
 //                   A register queue carries A and B from plane to plane, so
 //                   the i-direction never re-reads HBM.
-//  * step_direct -- one thread per site, 19 scattered loads per field; kept
-//                   as an independent second implementation for testing.
+//  * step_direct -- one thread per site, 19 scattered loads per field: an
+//                   independent second implementation for testing, and the
+//                   faster one on small cache-resident lattices (32^3),
+//                   where the tiled march has too few planes to hide latency.
 //
 // Physics per site (reference: lfm/core/backends/kernel_source.py:86-233 for
 // COLOR, :377-417 REAL, :483-538 COMPLEX; numpy_backend.py:80-382):
@@ -299,7 +301,8 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
 }
 
 // ---------------------------------------------------------------------------
-// Direct kernel: one thread per site.
+// Direct kernel: one thread per site, on a (k, j, i) grid of 64 x 4
+// workgroups, so the coordinates come from the launch.
 // ---------------------------------------------------------------------------
 template <int L>
 __global__ __launch_bounds__(256) void step_direct(Ptrs P, KParams kp) {
@@ -307,15 +310,15 @@ __global__ __launch_bounds__(256) void step_direct(Ptrs P, KParams kp) {
     constexpr int C = Lv<L>::NC;
     const int N = kp.N;
     const size_t n3 = P.n3;
-    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= n3) return;
-    const int i = int(idx / (size_t(N) * N));
-    const int j = int((idx / N) % N);
-    const int k = int(idx % N);
+    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+    if (k >= N || j >= N) return;
+    const size_t idx = (size_t(i) * N + j) * N + k;
     const size_t NN = size_t(N) * N;
-    const size_t rc = size_t(i) * NN, rp = size_t((i + 1) % N) * NN, rm = size_t((i + N - 1) % N) * NN;
-    const size_t cc = size_t(j) * N, cp = size_t((j + 1) % N) * N, cm = size_t((j + N - 1) % N) * N;
-    const size_t dc = k, dp = (k + 1) % N, dm = (k + N - 1) % N;
+    const int ip = i + 1 == N ? 0 : i + 1, im = i == 0 ? N - 1 : i - 1;
+    const int jp = j + 1 == N ? 0 : j + 1, jm = j == 0 ? N - 1 : j - 1;
+    const size_t rc = size_t(i) * NN, rp = size_t(ip) * NN, rm = size_t(im) * NN;
+    const size_t cc = size_t(j) * N, cp = size_t(jp) * N, cm = size_t(jm) * N;
+    const size_t dc = k, dp = k + 1 == N ? 0 : k + 1, dm = k == 0 ? N - 1 : k - 1;
     // neighbour offsets: 6 faces then 12 edges
     const size_t nb[18] = {rp + cc + dc, rm + cc + dc, rc + cp + dc, rc + cm + dc, rc + cc + dp,
                            rc + cc + dm, rp + cp + dc, rp + cm + dc, rm + cp + dc, rm + cm + dc,
@@ -666,8 +669,8 @@ bool streaming(size_t sites) {
 template <int L>
 int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
     if ((kp.flags & LFM_F_DIRECT) || kp.N > 1023) {
-        const unsigned nb = unsigned((P.n3 + 255) / 256);
-        hipLaunchKernelGGL((step_direct<L>), dim3(nb), dim3(256), 0, st, P, kp);
+        const dim3 grid(unsigned((kp.N + 63) / 64), unsigned((kp.N + 3) / 4), unsigned(kp.N));
+        hipLaunchKernelGGL((step_direct<L>), grid, dim3(64, 4), 0, st, P, kp);
         LFM_LAUNCH_CHECK("step_direct");
         return LFM_OK;
     }

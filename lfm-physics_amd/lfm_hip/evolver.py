@@ -68,6 +68,7 @@ class Evolver:
     """
 
     GRAPH_CYCLES = 8
+    DIRECT = -1          # `tile` value selecting the one-thread-per-site kernel
     GRAPH_CACHE = 24     # captured graphs kept per Evolver
 
     def __init__(self, config: SimulationConfig, backend="auto", mask_policy="reference",
@@ -190,6 +191,9 @@ class Evolver:
         else:
             combos = [(t, ch) for t in range(1, _abi.LFM_TILE_VARIANTS + 1)
                       for ch in dict.fromkeys(min(c, N) for c in chunks)]
+            if N <= 128:
+                # small cache-resident lattices: the direct kernel can win
+                combos.append((self.DIRECT, 0))
         for tile, ch in combos:
             self.tile, self.chunk = tile, ch
             self.kernel_events, self.kernel_event_steps = [], []
@@ -221,6 +225,7 @@ class Evolver:
 
     def _step(self, freeze_chi: bool = False, direct: bool = False) -> None:
         lib = self._lib
+        direct = direct or self.tile == self.DIRECT
         cur, prev = self._cur, self._prev
         nxt = self._free(self._pr, cur, prev)
         flags = _abi.LFM_F_CHI_DELTA if self._chi_off else 0
