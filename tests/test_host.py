@@ -178,3 +178,65 @@ def test_params_struct_layout():
     body = re.search(r"typedef struct lfm_params \{(.*?)\} lfm_params_t;", text, re.S).group(1)
     fields = re.findall(r"[A-Za-z_0-9]+(?=[,;])", body)
     assert fields == names, (fields, names)
+
+
+def test_relax_fused_step_bookkeeping():
+    """The relaxation loop's launch sequence (host logic only, launches mocked):
+    step n writes max slot n % 3; it reads the previous step's slot as its
+    pending rescale unless E was rescaled in between; every convergence check
+    rescales and folds max|dE| at a multiple of the check interval; E is
+    rescaled on return.  Graph-replayed blocks of six steps follow the same
+    sequence as eager steps."""
+    from lfm_hip import eigenmode as em
+
+    class Mock(em._Relaxer):
+        def __init__(self):
+            self.log, self.E, self.E_alt, self.sw, self._graphs = [], "A", "B", 1, {}
+
+            class Slots:
+                def __getitem__(self, i):
+                    return self
+
+                def zero_(self):
+                    pass
+            self.slots = Slots()
+
+        def _step(self, k, scale_k, amplitude, dt):
+            self.log.append(("step", k, scale_k))
+            self.E, self.E_alt = self.E_alt, self.E
+
+        def _rescale(self, k, amplitude, E_prev=None):
+            self.log.append(("rescale", k, E_prev is not None))
+
+        def slot_max(self, i):
+            return 1.0
+
+        def _block_graph(self, k0, amplitude, dt):
+            me = self
+
+            class G:
+                def replay(self):
+                    for u in range(me.GRAPH_STEPS):
+                        k = (k0 + u) % 3
+                        me._step(k, (k - 1) % 3, amplitude, dt)
+            return G()
+
+    for steps in (1, 5, 6, 7, 12, 13, 199, 200, 201, 405):
+        for ci in (1, 2, 7, 200):
+            for graphs in (True, False):
+                R = Mock()
+                R.relax(14.0, 0.01, steps, ci, -1.0, graphs=graphs)
+                n, pending, checks = 0, None, []
+                for ev in R.log:
+                    if ev[0] == "step":
+                        n += 1
+                        assert ev[1] == n % 3 and ev[2] == pending, (steps, ci, n, ev, pending)
+                        pending = ev[1]
+                    else:
+                        assert ev[1] == pending, (steps, ci, ev, pending)
+                        if ev[2]:
+                            checks.append(n)
+                        pending = None
+                assert n == steps and pending is None
+                assert checks == [m for m in range(1, steps + 1) if m % ci == 0]
+                assert R.E == ("A" if steps % 2 == 0 else "B")
