@@ -159,12 +159,15 @@ __global__ __launch_bounds__(AB) void adj_fused(AdjArgs A) {
     // the one after) and the pointwise data of a site one iteration ahead, so
     // global-load latency stays off each iteration's barrier-bound path.
     float sa[2][6], sb[2][6];
+    // Loads carry no per-lane predicate (idle staging positions read a valid
+    // in-lattice address and discard it; absent optional fields read one
+    // shared word): branch-free loads let the compiler place its waits where
+    // the values are used instead of draining everything at each join.
     auto load_plane = [&](float (&v)[2][6], int s) {
         int q = i0 - 1 + s;
         q = q < 0 ? q + N : (q >= N ? q - N : q);
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-            if (!sact[r]) continue;
             const uint32_t o = uint32_t(q) * NN4 + soff[r];
             v[r][0] = ldb(A.Ur, o);
             v[r][1] = ldb(A.Ui, o);
@@ -176,14 +179,23 @@ __global__ __launch_bounds__(AB) void adj_fused(AdjArgs A) {
     };
     // pointwise data of the site finished at iteration s (plane i0 + s - 2)
     float pw[5] = {}, pn[5] = {};
+    const float* mask_src = A.mask ? A.mask : A.chi;
+    const bool has_u2 = A.U2r != nullptr;
+    const float* u2r = has_u2 ? A.U2r : A.chi;
+    const float* u2i = has_u2 ? A.U2i : A.chi;
+    const float* x2 = has_u2 ? A.X2 : A.chi;
     auto load_point = [&](float (&v)[5], int s) {
-        const int im = i0 + s - 2;
-        if (s >= 2 && s < nplanes && live) {
-            const uint32_t ob = uint32_t(im) * NN4 + cb;
-            v[0] = ldb(A.chi, ob);
-            v[1] = A.mask ? ldbp<NT>(A.mask, ob) : 0.0f;
-            if (A.U2r) { v[2] = ldbp<NT>(A.U2r, ob); v[3] = ldbp<NT>(A.U2i, ob); v[4] = ldbp<NT>(A.X2, ob); }
-        }
+        // planes outside the chunk's sites load plane i0 and are never used
+        const int im = (s >= 2 && s < nplanes) ? i0 + s - 2 : i0;
+        const uint32_t ob = uint32_t(im) * NN4 + cb;
+        const uint32_t om = A.mask ? ob : 0u, o2 = has_u2 ? ob : 0u;
+        v[0] = ldb(A.chi, ob);
+        const float mv = ldbp<NT>(mask_src, om);
+        v[1] = A.mask ? mv : 0.0f;
+        const float a2 = ldbp<NT>(u2r, o2), b2 = ldbp<NT>(u2i, o2), c2 = ldbp<NT>(x2, o2);
+        v[2] = has_u2 ? a2 : 0.0f;
+        v[3] = has_u2 ? b2 : 0.0f;
+        v[4] = has_u2 ? c2 : 0.0f;
     };
     load_plane(sa, 0);
     if (nplanes > 1) load_plane(sb, 1);
@@ -242,9 +254,9 @@ __global__ __launch_bounds__(AB) void adj_fused(AdjArgs A) {
         accc[5] = -npw;
         accc[6] = nrw;
         const int im = i0 + s - 2;
-        if (s >= 2 && live) {
-            // site (im, j, k)
-            const uint32_t ob = uint32_t(im) * NN4 + cb;
+        {
+            // site (im, j, k); formed on every lane, stored where it exists
+            const uint32_t ob = uint32_t(im < 0 ? 0 : im) * NN4 + cb;
             const float cs = pw[0], m = pw[1], v2r = pw[2], v2i = pw[3], y2 = pw[4];
             const float chi = A.chi0 + cs;
             const float chi_sq = chi * chi;
@@ -260,15 +272,28 @@ __global__ __launch_bounds__(AB) void adj_fused(AdjArgs A) {
             const float dacc = -kc * S - 4.0f * A.lam * (3.0f * chi_sq - A.chi0 * A.chi0);
             const float xn = cxt * (2.0f + A.dt2 * dacc) + A.dt2 * fin[2] -
                              2.0f * A.dt2 * chi * (cpr * cur + cpi * cui) - y2;
-            kp += double(cxt) * (double(chi) * double(S));
+            const bool site = s >= 2 && live;
+            if (site) kp += double(cxt) * (double(chi) * double(S));
+            float o_r = ur, o_i = ui, o_x = xn;
             if (A.damp_out) {
-                stb<NT>(A.Or, ob, a * ur);
-                stb<NT>(A.Oi, ob, a * ui);
-                stb<NT>(A.Ox, ob, a * clamp_free(cs, a, A.chi0) * xn);
-            } else {
-                stb<NT>(A.Or, ob, ur);
-                stb<NT>(A.Oi, ob, ui);
-                stb<NT>(A.Ox, ob, xn);
+                o_r = a * ur;
+                o_i = a * ui;
+                o_x = a * clamp_free(cs, a, A.chi0) * xn;
+            }
+            // The prefetched point and plane data are waited for here, after
+            // the outputs are formed and before they are stored: loads and
+            // stores share one wait counter, and a wait behind these stores
+            // (at the next iteration's use of the prefetch) would drain them.
+#pragma unroll
+            for (int f = 0; f < 5; ++f) asm volatile("" : "+v"(o_r) : "v"(pn[f]));
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int f = 0; f < 6; ++f) asm volatile("" : "+v"(o_r) : "v"(sb[r][f]));
+            if (site) {
+                stb<NT>(A.Or, ob, o_r);
+                stb<NT>(A.Oi, ob, o_i);
+                stb<NT>(A.Ox, ob, o_x);
             }
         }
         // plane s's centre values belong to the site finished next iteration
