@@ -116,6 +116,7 @@ class _Relaxer:
     # before its rescale; six steps return the three max slots and the two
     # E buffers to where they started, so blocks of six replay from graphs.
     GRAPH_STEPS = 6
+    GRAPH_CACHE = 12      # six phase / buffer-order keys per (amplitude, dt)
 
     def _step(self, k: int, scale_k, amplitude: float, dt: float) -> None:
         """One update into slot k; scale_k: slot of the pending rescale of E
@@ -140,6 +141,8 @@ class _Relaxer:
         key = (k0, self.E.data_ptr(), float(amplitude), float(dt))
         g = self._graphs.get(key)
         if g is None:
+            while len(self._graphs) >= self.GRAPH_CACHE:
+                self._graphs.pop(next(iter(self._graphs)))   # oldest first
             s = torch.cuda.Stream(device=self.dev)
             s.wait_stream(torch.cuda.current_stream(self.dev))
             g = torch.cuda.CUDAGraph()
