@@ -481,11 +481,13 @@ void destroy(ScvPlan* p) {
     delete p;
 }
 
-// Column groups per workgroup: 4 along j; along i a workgroup takes its
-// whole slab (N rows of N/2+1 columns), which opens each row of the slab
-// once.  Measured at 512^3 (three passes): j/i rounds 2/4 2.10 ms, 4/4
-// 2.08, 2/17 2.05, 4/17 2.04, 6/17 2.09, 4/9 2.08.
-constexpr int kRoundsJ = 4;
+// Column groups per workgroup: along j 4 from N = 512 on, 2 below; along i
+// a workgroup takes its whole slab (N rows of N/2+1 columns), which opens
+// each row of the slab once.  Measured (three passes), j/i rounds:
+// 512^3: 2/4 2.10 ms, 4/4 2.08, 2/whole 2.05, 4/whole 2.04, 6/whole 2.09;
+// 256^3: 4/whole 0.218, 2/whole 0.212, 2/4 0.221;
+// 128^3: 4/whole 0.045, 2/whole 0.037, 2/4 0.039.
+int rounds_j(int N) { return N >= 512 ? 4 : 2; }
 int whole_slab_rounds(int N) {
     const int ncol = N >= 1024 ? 8 : 16;   // ncol_for<N>()
     return (N / 2 + 1 + ncol - 1) / ncol;
@@ -498,7 +500,7 @@ int run_columns(ScvPlan* p, int filter, float gamma, float D, hipStream_t st, in
     ColArgs A{p->spec, p->tw, ncomp_used, p->Hp, 0, 0, 0, filter, gamma, D, float(1.0 / n3), 1};
     // forward along j: slab (a, i), rows j
     A.slab_stride = (long long)N * p->Hp; A.row_stride = p->Hp; A.mode = 0;
-    A.rounds = kRoundsJ;
+    A.rounds = rounds_j(N);
     int rc = columns(N, A, st);
     if (rc) return rc;
     // along i: forward, filter, inverse in one pass; slab (a, j), rows i
@@ -508,7 +510,7 @@ int run_columns(ScvPlan* p, int filter, float gamma, float D, hipStream_t st, in
     if (rc) return rc;
     // inverse along j
     A.slab_stride = (long long)N * p->Hp; A.row_stride = p->Hp; A.mode = 1;
-    A.rounds = kRoundsJ;
+    A.rounds = rounds_j(N);
     return columns(N, A, st);
 }
 
