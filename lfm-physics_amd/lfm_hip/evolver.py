@@ -131,14 +131,13 @@ class Evolver:
             # "fused": the k-axis transform passes folded into the step kernel
             # (power-of-two N up to 512).  Its full-row workgroups hold one
             # workgroup per CU at N = 512, so it is latency-bound there; measured
-            # per step (MI355X): N=64 0.047 vs 0.051 ms, N=128 0.138 vs 0.116,
-            # N=256 0.88 vs 0.78, N=512 8.4 vs 6.5 ms for fused vs separate.
-            # "auto" therefore fuses only at N = 64.
+            # per step (MI355X), fused vs separate: N=64 0.044 vs 0.041 ms,
+            # N=128 0.179 vs 0.116, N=256 0.88 vs 0.78, N=512 8.4 vs 6.1 ms.
+            # "auto" therefore keeps the separate passes.
             can_fuse = self._scv.method == 2 and self.N in (64, 128, 256, 512)
             if sa_transform == "fused" and not can_fuse:
                 raise ValueError(f"sa_transform='fused' needs N in (64, 128, 256, 512), got {self.N}")
-            self._fused = can_fuse and (sa_transform == "fused"
-                                        or (sa_transform == "auto" and self.N == 64))
+            self._fused = can_fuse and sa_transform == "fused"
             self._spec_valid = False    # plan spectrum = k-axis spectra of S_a for current Psi
             self._sa_stale = False      # self._sa not updated by fused steps
         else:

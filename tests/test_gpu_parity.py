@@ -582,8 +582,8 @@ def test_calibration_stream_kernel(gpu, flags, grid):
 
 @pytest.mark.parametrize("level,N,extra", [
     ("REAL", 24, {}), ("REAL", 16, {"two_step": True}), ("COMPLEX", 33, {}),
-    ("COMPLEX", 20, {"two_step": True}), ("COLOR", 32, {}), ("COLOR", 64, {}),
-    ("COLOR", 24, {})])
+    ("COMPLEX", 20, {"two_step": True}), ("COLOR", 32, {}),
+    ("COLOR", 64, {"sa_transform": "fused"}), ("COLOR", 24, {})])
 def test_evolver_graph_replay_equals_eager(gpu, level, N, extra):
     """Evolver.evolve replays whole slot-rotation cycles from HIP graphs.
     Several evolve calls with host reads, a mid-run setter and a callback in
@@ -893,24 +893,26 @@ def test_real64_energy_trace_matches_oracle(gpu, periodic, steps):
     assert worst_e <= 1e-5 and worst_d <= 1e-5
 
 
-def test_color64_configuration_vs_fp64_oracle(gpu):
+@pytest.mark.parametrize("sa_transform", ["auto", "fused"])
+def test_color64_configuration_vs_fp64_oracle(gpu, sa_transform):
     """The COLOR full_physics configuration at the size it is checked against
     the CPU oracle (64^3): three blobs, one colour dominant each, centres
     uniform in the ball of radius 0.35 N/2 about the centre, amplitude 4,
-    width 3, random phases, chi = 19, S_a every step (the fused k-pass path
-    the Evolver picks at 64^3), 100 steps through Evolver.evolve (graph
-    replay): every field within 1e-5 relative L2 of the FP64 oracle."""
+    width 3, random phases, chi = 19, S_a every step (the separate transform
+    passes the Evolver picks, as at 512^3, and the fused k-pass path), 100
+    steps through Evolver.evolve (graph replay): every field within 1e-5
+    relative L2 of the FP64 oracle."""
     import torch
     import lfm_hip as L
     sys.path.insert(0, os.path.dirname(HERE))
     import bench
     N, steps = 64, 100
-    log(f"COLOR full_physics {N}^3 configuration vs FP64 oracle, {steps} steps")
+    log(f"COLOR full_physics {N}^3 configuration vs FP64 oracle, {steps} steps, {sa_transform}")
     cfg = L.full_physics(N)
     pr_t, pi_t = bench.gpu_ics(bench.WORKLOADS["color512"], N)
     pr, pi = pr_t.cpu().numpy(), pi_t.cpu().numpy()
-    ev = L.Evolver(cfg, backend="hip")
-    assert ev._fused
+    ev = L.Evolver(cfg, backend="hip", sa_transform=sa_transform)
+    assert ev._fused == (sa_transform == "fused")
     ev.set_psi_real(pr)
     ev.set_psi_imag(pi)
     ev.evolve(steps)
@@ -918,7 +920,7 @@ def test_color64_configuration_vs_fp64_oracle(gpu):
     st = O.OracleState(2, N, pr, pi, np.full(N ** 3, cfg.chi0), O.boundary_mask(N, cfg.boundary_fraction),
                        oracle_params(cfg))
     st.step(steps)
-    check_fields(ev, st, TOL64, "color64_config")
+    check_fields(ev, st, TOL64, f"color64_config_{sa_transform}")
 
 
 def test_complex64_configuration_vs_fp64_oracle(gpu):
