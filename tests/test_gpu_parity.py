@@ -947,3 +947,37 @@ def test_complex64_configuration_vs_fp64_oracle(gpu):
     st = O.OracleState(1, N, pr, pi, np.full(N ** 3, cfg.chi0), np.zeros(N ** 3), oracle_params(cfg))
     st.step(steps)
     check_fields(ev, st, TOL64, "complex64_config")
+
+
+@pytest.mark.parametrize("level,N", [("REAL", 24), ("COMPLEX", 33), ("COLOR", 20)])
+def test_tile_shapes_and_chunks_equal_direct_kernel(gpu, level, N):
+    """Every tile shape at one-plane and ragged chunks gives the same bits over
+    9 masked steps (the autotuner may pick any of them); the one-thread-per-
+    site kernel, which sums the Laplacian and currents in another order,
+    agrees to float32 rounding."""
+    import lfm_hip as L
+    log(f"tile/chunk variants vs direct: {level} N={N}")
+    lvl = getattr(L.FieldLevel, level)
+    cfg = L.full_physics(N) if lvl == L.FieldLevel.COLOR else \
+        L.SimulationConfig(grid_size=N, field_level=lvl, lambda_self=4 / 31)
+    r, i = make_ics(int(lvl), N, seed=17)
+    out = {}
+    for tile, chunk in [(L.Evolver.DIRECT, 0)] + [(t, c) for t in range(1, 7) for c in (1, 3)]:
+        ev = L.Evolver(cfg, backend="hip", graphs=False)
+        ev.tile, ev.chunk = tile, chunk
+        ev.set_psi_real(r)
+        if i is not None:
+            ev.set_psi_imag(i)
+        ev.evolve(9)
+        out[(tile, chunk)] = (ev.get_psi_real(), ev.get_chi_storage()[0],
+                              ev.get_psi_imag() if i is not None else None)
+    ref = out[(1, 1)]
+    for key, got in out.items():
+        for a, b, name in zip(got, ref, ("psi_r", "chi", "psi_i")):
+            if a is None:
+                continue
+            if key[0] == L.Evolver.DIRECT:
+                err = float(np.abs(a.astype(np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+                assert err < 1e-5, (key, name, err)
+            else:
+                np.testing.assert_array_equal(a, b, err_msg=f"{key} {name}")
