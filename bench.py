@@ -502,7 +502,7 @@ def main():
 
     tuning = None
     if args.tile == "auto" and not args.direct:
-        tuning = ev.autotune(chunks=(0, 1, 2, 4, 8, 16, 32, 64) if args.chunk == 0 else (args.chunk,))
+        tuning = ev.autotune(chunks=(0, 1, 2, 4, 8, 16, 32, 64, 128) if args.chunk == 0 else (args.chunk,))
     elif not args.direct:
         ev.tile, ev.chunk = int(args.tile), args.chunk
     # REAL / COMPLEX advance two steps per launch (as Evolver.evolve does)

@@ -169,7 +169,7 @@ class Evolver:
             self._params[key] = p
         return p
 
-    def autotune(self, reps: int = 6, chunks=(0, 1, 2, 4, 8, 16, 32, 64)) -> dict:
+    def autotune(self, reps: int = 6, chunks=(0, 1, 2, 4, 8, 16, 32, 64, 128)) -> dict:
         """Time every tile shape and the given plane chunks (0: the library's
         choice) of the stencil kernel on the current state and keep the
         fastest.  The state is saved and restored, so tuning does not
