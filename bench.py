@@ -555,10 +555,19 @@ def main():
     lvl = ['REAL', 'COMPLEX', 'COLOR'][w['level']]
     kname = (f"step_color_kfft<{N}>" if getattr(ev, "_fused", False)
              else f"step2_tiled<{lvl}>" if pairs else f"step_tiled<{lvl}>")
+    traffic_cfg = None
     if os.path.exists(tfile) and args.size is None and not args.direct:
+        # HBM bytes per launch from the committed counter runs: those of the
+        # tile shape and chunk this run uses when they were profiled, else
+        # those of the profiled configuration, named in traffic_tile_chunk
         rec = json.load(open(tfile))
         if rec.get("kernel") == kname:
-            traffic = rec.get("hbm_bytes_per_launch")
+            by = rec.get("by_tile_chunk") or {}
+            traffic_cfg = f"{ev.tile}/{ev.chunk}"
+            traffic = by.get(traffic_cfg)
+            if traffic is None:
+                traffic_cfg = rec.get("tile_chunk")
+                traffic = rec.get("hbm_bytes_per_launch")
     out = {
         "metric": METRIC, "value": n3 / (ms * 1e-3) / 1e9, "unit": "GLUPS", "n_gpus": 1,
         "steps": args.steps, "ms_per_step": ms,
@@ -569,6 +578,7 @@ def main():
                    "preset": w["preset"], "mask": w["mask"], "kernel": "direct" if args.direct else "tiled"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_tile_chunk": traffic_cfg,
                      "kernel": kname,
                      "kernel_ms": kms, "bytes_per_site": b_alg,
                      "steps_per_launch": 2 if pairs else 1},
