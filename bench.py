@@ -254,17 +254,15 @@ def run_batch(args):
         evols = []
         for c, (r, i) in zip(cfgs, ics):
             ev = L.Evolver(c, backend="hip", graphs=graphs)
-            if graphs:
-                for n in WARMUP_CALLS:     # capture every graph the timed run replays
-                    ev.set_psi_real(r)
-                    ev.set_psi_imag(i)
-                    ev.evolve(n)
-                ev.set_chi(np.full((c.grid_size,) * 3, c.chi0, np.float32))
             ev.set_psi_real(r)
             ev.set_psi_imag(i)
             evols.append(ev)
         for ev in evols:
             ev.evolve(args.warmup)
+            # the graphs the timed call replays from this phase (one cycle and
+            # the multi-cycle graph), captured without running any step
+            if graphs and ev._cycle_graph() is not None:
+                ev._cycle_graph(ev.GRAPH_CYCLES)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for ev in evols:
