@@ -101,6 +101,9 @@ class LatticeBatch:
         # stepped since its Psi was last set), so S_a comes from it, as the
         # single-lattice Evolver does after its first step
         self._psisq_ok = [False] * self.nlat
+        # Psi snapshot arenas (allocated by the first snapshot_psi call)
+        self._snap_r = self._snap_i = None
+        self._snap_ok = [False] * self.nlat
         self._cur, self._prev, self._chi_cur, self._chi_prev = 0, 1, 0, 1
         self._damped = False
         self.step = 0
@@ -421,20 +424,50 @@ class LatticeBatch:
             self._red_out = torch.empty(8, dtype=torch.float64, device=self.device)
         return self._partials, self._red_out
 
-    def metrics(self, b: int, clusters: bool = True) -> dict:
-        """compute_metrics keys for lattice b, with the leapfrog prev as the
-        kinetic reference (FP64 reductions on the GPU)."""
+    def snapshot_psi(self, lattices=None) -> None:
+        """Copy the current Psi of the given lattices (default: all) into the
+        snapshot arenas: the kinetic-energy reference :meth:`metrics` uses
+        with ``kinetic="snapshot"``, as Simulation.run keeps Psi at run start
+        and at every report_interval (lfm/simulation.py:1168-1182)."""
+        if self._snap_r is None:
+            self._snap_r = torch.empty_like(self._pr[0])
+            self._snap_i = torch.empty_like(self._pi[0]) if self._pi is not None else None
+        for b in (range(self.nlat) if lattices is None else lattices):
+            self._view(self._snap_r, b, True).copy_(self._view(self._pr[self._cur], b, True))
+            if self._pi is not None:
+                self._view(self._snap_i, b, True).copy_(self._view(self._pi[self._cur], b, True))
+            self._snap_ok[b] = True
+
+    def metrics(self, b: int, clusters: bool = True, kinetic: str = "leapfrog") -> dict:
+        """compute_metrics keys for lattice b (FP64 reductions on the GPU).
+
+        kinetic: "leapfrog" takes the kinetic term against the true leapfrog
+        prev; "snapshot" against the Psi that :meth:`snapshot_psi` stored,
+        which is what the reference's Simulation.metrics() reports
+        (lfm/simulation.py:1438-1440, its prev being a run-start or
+        report_interval snapshot)."""
         part, out = self._red()
         c = self.configs[b]
         N = self.sizes[b]
-        pr, prp = self._view(self._pr[self._cur], b, True), self._view(self._pr[self._prev], b, True)
+        pr = self._view(self._pr[self._cur], b, True)
         pi = self._view(self._pi[self._cur], b, True) if self._pi else None
-        pip = self._view(self._pi[self._prev], b, True) if self._pi else None
+        if kinetic == "snapshot":
+            if not self._snap_ok[b]:
+                raise ValueError(f"lattice {b} has no Psi snapshot (call snapshot_psi first)")
+            prp = self._view(self._snap_r, b, True)
+            pip = self._view(self._snap_i, b, True) if self._pi else None
+            damp = 0
+        elif kinetic == "leapfrog":
+            prp = self._view(self._pr[self._prev], b, True)
+            pip = self._view(self._pi[self._prev], b, True) if self._pi else None
+            damp = int(self._damped and self.has_mask[b])
+        else:
+            raise ValueError(f"kinetic must be 'leapfrog' or 'snapshot', got {kinetic!r}")
         rc = self._lib.lfm_energy_f64(
             pr.data_ptr(), prp.data_ptr(), pi.data_ptr() if pi is not None else None,
             pip.data_ptr() if pip is not None else None,
             self._view(self._chi[self._chi_cur], b).data_ptr(), self._chi0[b],
-            self._view(self.mask, b).data_ptr(), int(self._damped and self.has_mask[b]),
+            self._view(self.mask, b).data_ptr(), damp,
             self.ncomp, N, float(c.dt), float(c.c), part.data_ptr(), out.data_ptr(),
             stream_handle())
         _abi.check(rc, "lfm_energy_f64")
@@ -496,7 +529,8 @@ class LatticeBatch:
         self.set_chi(b, chi)
 
 
-def _batched_sweep(cfgs, rows, steps, metric_names, equilibrate, soliton, backend, tune):
+def _batched_sweep(cfgs, rows, steps, metric_names, equilibrate, soliton, backend, tune,
+                   kinetic):
     batch = LatticeBatch(cfgs, backend=backend)
     for b, c in enumerate(cfgs):
         if soliton is not None:
@@ -505,14 +539,37 @@ def _batched_sweep(cfgs, rows, steps, metric_names, equilibrate, soliton, backen
             if "velocity" in kw:
                 raise NotImplementedError("moving solitons are not batched")
             batch.place_soliton(b, pos, **kw)
-        if equilibrate:
+        # Simulation.run auto-equilibrates when solitons were placed and
+        # equilibrate() was not called (lfm/simulation.py:965-982), so a
+        # soliton run is equilibrated whatever `equilibrate` says
+        if equilibrate or soliton is not None:
             batch.equilibrate(b)
     if tune:
         batch.autotune()
-    batch.evolve(steps)
+    steps = int(steps)
+    if kinetic == "snapshot":
+        # Simulation.run snapshots Psi at its start and again whenever the
+        # evolver's step count reaches a multiple of report_interval
+        # (lfm/simulation.py:1168-1182, lfm/core/evolver.py:178-179); the
+        # last snapshot is the kinetic reference of metrics()
+        snap_at = {}
+        for b, c in enumerate(cfgs):
+            r = int(c.report_interval)
+            s = (steps // r) * r if r > 0 else 0
+            snap_at.setdefault(s, []).append(b)
+        done = 0
+        for s in sorted(snap_at):
+            batch.evolve(s - done)
+            batch.snapshot_psi(snap_at[s])
+            done = s
+        batch.evolve(steps - done)
+    elif kinetic == "leapfrog":
+        batch.evolve(steps)
+    else:
+        raise ValueError(f"kinetic must be 'snapshot' or 'leapfrog', got {kinetic!r}")
     out = []
     for b, row in enumerate(rows):
-        m = batch.metrics(b)
+        m = batch.metrics(b, kinetic=kinetic)
         if metric_names is not None:
             row.update({k: m.get(k) for k in metric_names})
         else:
@@ -522,15 +579,22 @@ def _batched_sweep(cfgs, rows, steps, metric_names, equilibrate, soliton, backen
 
 
 def sweep(config: SimulationConfig, param: str, values, steps: int, metric_names=None, *,
-          equilibrate: bool = True, soliton=None, backend="auto", tune: bool = False):
+          equilibrate: bool = True, soliton=None, backend="auto", tune: bool = False,
+          kinetic: str = "snapshot"):
     """lfm.sweep.sweep with every value run in one batch (lfm/sweep.py:14-82).
 
     As in the reference, each value is set on a deep copy of `config` with
     setattr (derived fields such as sigma keep the base config's value), an
     optional soliton is placed (at the centre when no position is given),
-    chi is equilibrated, `steps` steps are run and the final metrics are
-    returned as {param: value, **metrics} rows.  Metrics use the true leapfrog
-    prev for the kinetic term.
+    chi is equilibrated (always after a soliton, as Simulation.run's
+    auto-equilibration does), `steps` steps are run and the final metrics
+    are returned as {param: value, **metrics} rows.
+
+    kinetic: "snapshot" (default) gives the reference's rows: the kinetic
+    term is taken against Psi at run start, or at the last multiple of
+    report_interval when `steps` reaches one, as the reference's
+    ``sim.run(...); sim.metrics()`` does.  "leapfrog" takes it against the
+    true leapfrog prev (the physical kinetic energy at the final step).
     """
     cfgs, rows = [], []
     for v in values:
@@ -538,13 +602,15 @@ def sweep(config: SimulationConfig, param: str, values, steps: int, metric_names
         setattr(c, param, v)
         cfgs.append(c)
         rows.append({param: v})
-    return _batched_sweep(cfgs, rows, steps, metric_names, equilibrate, soliton, backend, tune)
+    return _batched_sweep(cfgs, rows, steps, metric_names, equilibrate, soliton, backend, tune,
+                          kinetic)
 
 
 def sweep_2d(config: SimulationConfig, param1: str, values1, param2: str, values2, steps: int,
              metric_names=None, *, equilibrate: bool = True, soliton=None, backend="auto",
-             tune: bool = False):
-    """lfm.sweep.sweep_2d as one batch (lfm/sweep.py:85-157)."""
+             tune: bool = False, kinetic: str = "snapshot"):
+    """lfm.sweep.sweep_2d as one batch (lfm/sweep.py:85-157); `kinetic` as
+    in :func:`sweep`."""
     cfgs, rows = [], []
     for v1 in values1:
         for v2 in values2:
@@ -553,4 +619,5 @@ def sweep_2d(config: SimulationConfig, param1: str, values1, param2: str, values
             setattr(c, param2, v2)
             cfgs.append(c)
             rows.append({param1: v1, param2: v2})
-    return _batched_sweep(cfgs, rows, steps, metric_names, equilibrate, soliton, backend, tune)
+    return _batched_sweep(cfgs, rows, steps, metric_names, equilibrate, soliton, backend, tune,
+                          kinetic)

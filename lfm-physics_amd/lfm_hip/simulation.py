@@ -18,8 +18,25 @@ from pathlib import Path
 import numpy as np
 import torch
 
+from .backend import as_device_array
 from .config import BoundaryType, FieldLevel, SimulationConfig
 from .evolver import Evolver
+
+
+def config_from_checkpoint(data) -> SimulationConfig:
+    """The SimulationConfig a checkpoint's ``config_json`` describes, read as
+    lfm/simulation.py:1537-1543 reads it: enums from their values, and the
+    fields __post_init__ derives (dx, sigma) dropped."""
+    from . import config as _c
+    cfg = json.loads(str(data["config_json"]))
+    cfg["field_level"] = FieldLevel(cfg["field_level"])
+    cfg["boundary_type"] = BoundaryType(cfg["boundary_type"])
+    for k in ("dx", "sigma"):
+        cfg.pop(k, None)
+    for k, enum in (("physical_scale", _c.PhysicsScale), ("chi_mode", _c.ChiMode)):
+        if cfg.get(k) is not None:
+            cfg[k] = enum(cfg[k])
+    return SimulationConfig(**cfg)
 
 
 class Simulation:
@@ -103,21 +120,22 @@ class Simulation:
         return self._evolver.get_energy_density()
 
     # -- zero-copy device views (lfm/simulation.py:209-252) -----------------
-    # Live views of the evolver's active buffers (torch tensors on the GPU);
-    # writes through them are seen by the next step, as the reference's
-    # experiment hooks (barriers, sources, detectors) expect.
+    # Live views of the evolver's active buffers (DeviceArray tensors on the
+    # GPU, with the reference's .fill / .copy spellings); writes through them
+    # are seen by the next step, as the reference's experiment hooks
+    # (barriers, sources, detectors) expect.
     def _psi_shape(self):
         N = self.config.grid_size
         return (self._evolver._n_psi, N, N, N) if self._evolver.level == 2 else (N, N, N)
 
     def _native_psi_real(self):
         """Current Psi real buffer as a device view, (N, N, N) or (3, N, N, N)."""
-        return self._evolver.native_psi_real().view(self._psi_shape())
+        return as_device_array(self._evolver.native_psi_real().view(self._psi_shape()))
 
     def _native_psi_imag(self):
         """Current Psi imaginary buffer as a device view (None at REAL)."""
         pi = self._evolver.native_psi_imag()
-        return None if pi is None else pi.view(self._psi_shape())
+        return None if pi is None else as_device_array(pi.view(self._psi_shape()))
 
     def _native_chi(self):
         """Current chi buffer as an (N, N, N) device view in absolute units.
@@ -126,7 +144,7 @@ class Simulation:
         ev = self._evolver
         ev.use_absolute_chi()
         N = self.config.grid_size
-        return ev.native_chi().view(N, N, N)
+        return as_device_array(ev.native_chi().view(N, N, N))
 
     def _native_chi_pair(self):
         """The four reference chi buffers as (N, N, N) device views, absolute
@@ -136,10 +154,11 @@ class Simulation:
         ev = self._evolver
         ev.use_absolute_chi()
         N = self.config.grid_size
-        return tuple(t.view(N, N, N) for t in ev.native_chi_refs())
+        return tuple(as_device_array(t.view(N, N, N)) for t in ev.native_chi_refs())
 
     def _to_device(self, arr):
-        """A host array as a flat float32 device tensor (the backend's format)."""
+        """A host array as a flat float32 DeviceArray (the backend's format;
+        lfm/simulation.py:253-255)."""
         return self._evolver.backend.from_numpy(np.asarray(arr, np.float32).ravel())
 
     def get_chi(self):
@@ -485,16 +504,7 @@ class Simulation:
         exact=True and the true prev buffers present the run resumes with
         its time derivatives; otherwise current = prev, as the reference."""
         data = np.load(str(path), allow_pickle=False)
-        cfg = json.loads(str(data["config_json"]))
-        cfg["field_level"] = FieldLevel(cfg["field_level"])
-        cfg["boundary_type"] = BoundaryType(cfg["boundary_type"])
-        for k in ("dx", "sigma"):
-            cfg.pop(k, None)
-        for k in ("physical_scale", "chi_mode"):
-            if k in cfg and cfg[k] is not None:
-                from . import config as _c
-                cfg[k] = (_c.PhysicsScale if k == "physical_scale" else _c.ChiMode)(cfg[k])
-        sim = cls(SimulationConfig(**cfg), backend=backend, **kwargs)
+        sim = cls(config_from_checkpoint(data), backend=backend, **kwargs)
         ev = sim._evolver
         ev.set_psi_real(data["psi_real"])
         if "psi_imag" in data:

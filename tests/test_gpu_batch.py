@@ -114,28 +114,48 @@ def test_graph_replay_equals_eager(gpu, group_bytes, cycles):
         np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("color", [False, True])
-def test_batched_sweep_rows_match_simulation(gpu, color):
+@pytest.mark.parametrize("color,report,steps", [(False, 5000, 30), (True, 5000, 30),
+                                                (False, 20, 30), (False, 15, 30)])
+def test_batched_sweep_rows_match_simulation(gpu, color, report, steps):
+    """Every row of the batched sweep equals what the reference's sweep loop
+    builds for that value (lfm/sweep.py:53-78): a Simulation with the soliton
+    at the centre, equilibrate(), run(steps, record_metrics=False), then
+    metrics().  Its kinetic term is taken against the Psi snapshot of run
+    start (report_interval 5000 > steps), of step 20 (report_interval 20), or
+    of step 30 itself (report_interval 15: the last snapshot is the final
+    state, so the kinetic energy is 0), as Simulation.run keeps it.  The
+    kinetic="leapfrog" rows equal the true-prev energy sums instead."""
     import lfm_hip as L
     from lfm_hip.batch import sweep
-    log(f"batched sweep vs per-value Simulation ({'COLOR full_physics' if color else 'COMPLEX'})")
+    log(f"batched sweep vs per-value Simulation ({'COLOR full_physics' if color else 'COMPLEX'}),"
+        f" report_interval {report}, {steps} steps")
     N = 16 if color else 24
-    mk = (lambda **k: L.full_physics(N, **k)) if color else \
-        (lambda **k: L.SimulationConfig(grid_size=N, field_level=L.FieldLevel.COMPLEX, **k))
+    mk = (lambda **k: L.full_physics(N, report_interval=report, **k)) if color else \
+        (lambda **k: L.SimulationConfig(grid_size=N, field_level=L.FieldLevel.COMPLEX,
+                                        report_interval=report, **k))
     base = mk()
     vals = [1 / 63, 1.5 / 63, 2 / 63]
-    rows = sweep(base, "kappa", vals, 30, soliton={"amplitude": 4.0}, backend="hip")
+    rows = sweep(base, "kappa", vals, steps, soliton={"amplitude": 4.0}, backend="hip")
+    rows_lf = sweep(base, "kappa", vals, steps, soliton={"amplitude": 4.0}, backend="hip",
+                    kinetic="leapfrog", metric_names=["energy_total", "chi_min"])
     assert [r["kappa"] for r in rows] == vals
-    for v, row in zip(vals, rows):
+    for v, row, row_lf in zip(vals, rows, rows_lf):
         cfg = mk(kappa=v)
         sim = L.Simulation(cfg, backend="hip")
         sim.place_soliton((N // 2, N // 2, N // 2), amplitude=4.0)
         sim.equilibrate()
-        sim.run(30, record_metrics=False)
-        ev = sim.evolver
-        assert row["chi_min"] == ev.chi_stats()["chi_min"]
-        e = ev.energy_sums()
-        assert abs(row["energy_total"] - e["energy_total"]) <= 1e-12 * abs(e["energy_total"])
+        sim.run(steps, record_metrics=False)
+        m = sim.metrics()
+        assert set(row) == {"kappa"} | set(m)
+        for k, x in m.items():
+            assert row[k] == x, (k, row[k], x)
+        if report == 15:
+            assert row["energy_kinetic"] == 0.0
+        e = sim.evolver.energy_sums()
+        assert row_lf["energy_total"] == e["energy_total"]
+        assert row_lf["chi_min"] == m["chi_min"]
+        log(f"  kappa {v:.5f}: E_total {row['energy_total']:.6e} (snapshot) "
+            f"{row_lf['energy_total']:.6e} (leapfrog)")
 
 
 @pytest.mark.parametrize("sizes,group_bytes", [([16, 32, 24], None), ([32, 16, 64, 32], 1 << 22)])
