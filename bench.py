@@ -184,6 +184,26 @@ def cpu_baseline(w, budget_s=12.0):
                       f"({steps / dt:.2f} steps/s, one thread)"}
 
 
+def batch64_recipe(nlat=64, seed=42):
+    """The batched-sweep workload's lattices: for each of `nlat`, N drawn from
+    {32, 48, 64, 96, 128}, kappa = (1/63) U(0.5, 2), one centred soliton with
+    amplitude U(2, 8) and width U(2, 4), plus N(0, 1e-3) noise on both parts;
+    COMPLEX, default cos^2 mask.  Returns (configs, [(psi_r, psi_i)])."""
+    import numpy as np
+    import lfm_hip as L
+    rng = np.random.default_rng(seed)
+    cfgs, ics = [], []
+    for _ in range(nlat):
+        N = int(rng.choice([32, 48, 64, 96, 128]))
+        kap = (1 / 63) * rng.uniform(0.5, 2)
+        amp, sig = rng.uniform(2, 8), rng.uniform(2, 4)
+        cfgs.append(L.SimulationConfig(grid_size=N, field_level=L.FieldLevel.COMPLEX, kappa=kap))
+        r, i = L.fields.gaussian_soliton(N, (N // 2, N // 2, N // 2), amp, sig)
+        nr, ni = L.fields.seed_noise(N, 1e-3, rng=rng)
+        ics.append((r + nr, i + ni))
+    return cfgs, ics
+
+
 def run_batch(args):
     """Batched sweep workload: 64 independent COMPLEX lattices with uneven
     sizes from {32,48,64,96,128}^3, kappa_b = kappa * U(0.5, 2), one soliton
@@ -197,17 +217,8 @@ def run_batch(args):
     import numpy as np
     import torch
     import lfm_hip as L
-    rng = np.random.default_rng(42)
-    nlat = 64
-    cfgs, ics = [], []
-    for _ in range(nlat):
-        N = int(rng.choice([32, 48, 64, 96, 128]))
-        kap = (1 / 63) * rng.uniform(0.5, 2)
-        amp, sig = rng.uniform(2, 8), rng.uniform(2, 4)
-        cfgs.append(L.SimulationConfig(grid_size=N, field_level=L.FieldLevel.COMPLEX, kappa=kap))
-        r, i = L.fields.gaussian_soliton(N, (N // 2, N // 2, N // 2), amp, sig)
-        nr, ni = L.fields.seed_noise(N, 1e-3, rng=rng)
-        ics.append((r + nr, i + ni))
+    cfgs, ics = batch64_recipe()
+    nlat = len(cfgs)
     steps = args.steps
     sites = sum(c.grid_size ** 3 for c in cfgs)
     batch = L.LatticeBatch(cfgs, backend="hip", tile=0 if args.tile == "auto" else int(args.tile),

@@ -483,7 +483,8 @@ class LatticeBatch:
              "n_clusters": float("nan"), "psi_sq_total": S}
         if clusters:
             from scipy import ndimage
-            psq = self.get_energy_density(b).astype(np.float64)
+            from .simulation import psi_sq_f64
+            psq = psi_sq_f64(self.get_psi_real(b), self.get_psi_imag(b))
             x = np.arange(N, dtype=np.float64)
             X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
             h = N / 2.0

@@ -23,6 +23,18 @@ from .config import BoundaryType, FieldLevel, SimulationConfig
 from .evolver import Evolver
 
 
+def psi_sq_f64(pr, pi=None) -> np.ndarray:
+    """|Psi|^2 summed over colours in float64 from the float32 fields, as
+    compute_metrics forms it for the cluster count (lfm/analysis/metrics.py:
+    83-91)."""
+    sq = lambda a: (np.sum(a.astype(np.float64) ** 2, axis=0) if a.ndim == 4
+                    else a.astype(np.float64) ** 2)
+    out = sq(pr)
+    if pi is not None:
+        out += sq(pi)
+    return out
+
+
 def config_from_checkpoint(data) -> SimulationConfig:
     """The SimulationConfig a checkpoint's ``config_json`` describes, read as
     lfm/simulation.py:1537-1543 reads it: enums from their values, and the
@@ -459,7 +471,7 @@ class Simulation:
     def _count_clusters(self, percentile: float = 90.0) -> int:
         # lfm/analysis/structure.py:111-145 (host: scipy.ndimage.label)
         from scipy import ndimage
-        psq = self._evolver.get_energy_density().astype(np.float64)
+        psq = psi_sq_f64(self._evolver.get_psi_real(), self._evolver.get_psi_imag())
         interior = self.get_interior_mask()
         vals = psq[interior]
         if vals.size == 0:

@@ -240,3 +240,48 @@ def test_relax_fused_step_bookkeeping():
                 assert n == steps and pending is None
                 assert checks == [m for m in range(1, steps + 1) if m % ci == 0]
                 assert R.E == ("A" if steps % 2 == 0 else "B")
+
+
+# ------------------------------------------------------ drop-in array protocol
+def test_device_array_protocol():
+    """DeviceArray (what HipBackend hands out) answers the NumPy/CuPy array
+    calls the reference makes on backend buffers: copy() is a new buffer,
+    fill() and slice assignment write in place, reshape() views share storage
+    and stay DeviceArrays (lfm/core/evolver.py:161-176,355-429;
+    lfm/experiment/barrier.py:321-334).  Checked on CPU tensors; the class
+    adds no device logic."""
+    import torch
+    from lfm_hip.backend import DeviceArray, as_device_array
+    a = as_device_array(torch.arange(27, dtype=torch.float32))
+    assert isinstance(a, DeviceArray)
+    v = a.reshape(3, 3, 3)
+    assert isinstance(v, DeviceArray) and v.data_ptr() == a.data_ptr()
+    snap = a.copy()
+    assert isinstance(snap, DeviceArray) and snap.data_ptr() != a.data_ptr()
+    v.fill(19.0)
+    assert float(a.sum()) == 19.0 * 27 and float(snap.sum()) == sum(range(27))
+    v[1] = 22.0
+    assert float(a[9:18].min()) == 22.0
+    a[:] = snap
+    assert torch.equal(a, torch.arange(27, dtype=torch.float32))
+    v *= torch.zeros(3, 3)
+    assert float(a.abs().sum()) == 0.0
+    d = a.astype(np.float64)
+    assert d.dtype == torch.float64 and d.data_ptr() != a.data_ptr()
+
+
+def test_hip_backend_raises_import_error_without_gpu():
+    """lfm/core/backends/__init__.py:57-59 raises ImportError when the GPU
+    stack is missing; HipBackend raises an ImportError subclass likewise."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from lfm_hip.backend import GpuUnavailableError, HipBackend
+    from lfm_hip.evolver import get_backend
+    assert issubclass(GpuUnavailableError, ImportError)
+    with pytest.raises(ImportError):
+        HipBackend()
+    with pytest.raises(ImportError):
+        get_backend("gpu")
+    with pytest.raises(ValueError):
+        get_backend("nonsense")
