@@ -234,6 +234,22 @@ int lfm_energy_f64(const float* pr, const float* pr_prev, const float* pi,
                    const float* pi_prev, const float* chi, double chi_offset,
                    const float* mask, int32_t damp_prev, int32_t ncomp, int32_t N, double dt,
                    double c, double* partials, double* out, void* stream);
+/* Allocation-free variant of lfm_energy_f64.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results (bit for bit) without
+ * `partials`: the per-workgroup partials go to a static device array of
+ * LFM_ENERGY_NOALLOC_SCRATCH_BYTES (128 KiB) per GPU, held until the process
+ * ends; its address is looked up (hipGetSymbolAddress) on the first call on
+ * each device, and nothing is allocated.  The grid is capped so every N fits;
+ * a size that did not would be refused with LFM_E_SIZE.  All calls on one GPU
+ * share that array: calls that overlap in time on one GPU (other streams,
+ * other host threads) race and must be serialised by the caller. */
+#define LFM_ENERGY_NOALLOC_SCRATCH_BYTES (2048 * 8 * 8)
+int lfm_energy_f64_noalloc(const float* pr, const float* pr_prev, const float* pi,
+                           const float* pi_prev, const float* chi, double chi_offset,
+                           const float* mask, int32_t damp_prev, int32_t ncomp, int32_t N,
+                           double dt, double c, double* out, void* stream);
 /* chi_offset is added to every stored chi value (chi0 for deviation
  * storage, else 0).
  * chi statistics over the interior ball dist((i,j,k), N/2) <= (1-bf) N/2.

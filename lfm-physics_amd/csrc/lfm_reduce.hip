@@ -12,6 +12,7 @@
 // and writes one partial per workgroup.  A one-workgroup kernel sums the
 // partials in a fixed order, so results are deterministic run to run.
 #include <cfloat>
+#include <mutex>
 
 #include "lfm_internal.h"
 
@@ -223,6 +224,41 @@ int nblocks_for(int N) {
     return int(b > RMAXB ? RMAXB : b);
 }
 
+// Partials of lfm_energy_f64_noalloc: one file-scope array per GPU, held
+// until the process ends (RMAXB records of RSLOTS doubles = 128 KiB).  Host
+// code never passes `scratch` itself (on the host it names a stand-in, not
+// the device copy): each device's address comes from hipGetSymbolAddress,
+// taken once per device with that device current, and only that pointer is
+// handed to kernels.
+constexpr int SCRATCH_DOUBLES = RMAXB * RSLOTS;
+__device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The current device's copy of `scratch`; NULL (error set) on failure.
+double* device_scratch() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+        set_error("lfm_energy_f64_noalloc: no current HIP device (or device index >= %d)",
+                  MAX_DEVICES);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error("lfm_energy_f64_noalloc: hipGetSymbolAddress(scratch) on device %d failed: %s",
+                      dev, hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace
 }  // namespace lfm
 
@@ -253,6 +289,25 @@ extern "C" int lfm_energy_f64(const float* pr, const float* pr_prev, const float
     hipLaunchKernelGGL(sum_partials, dim3(1), dim3(RB), 0, st, (const double*)partials, nb, 4, out);
     LFM_LAUNCH_CHECK("sum_partials");
     return LFM_OK;
+}
+
+extern "C" int lfm_energy_f64_noalloc(const float* pr, const float* pr_prev, const float* pi,
+                                      const float* pi_prev, const float* chi, double chi_offset,
+                                      const float* mask, int32_t damp_prev, int32_t ncomp,
+                                      int32_t N, double dt, double c, double* out, void* stream) {
+    // Same kernels and summation order as lfm_energy_f64, so the same bits;
+    // the per-workgroup partials live in this device's `scratch` array.  Every
+    // partial record is written by energy_partial before sum_partials reads
+    // it, so nothing needs clearing between calls.
+    if (N >= 2 && int64_t(nblocks_for(N)) * RSLOTS > SCRATCH_DOUBLES) {
+        set_error("lfm_energy_f64_noalloc: N=%d needs %lld partial doubles, the static scratch "
+                  "holds %d", N, (long long)(int64_t(nblocks_for(N)) * RSLOTS), SCRATCH_DOUBLES);
+        return LFM_E_SIZE;
+    }
+    double* partials = device_scratch();
+    if (!partials) return LFM_E_HIP;
+    return lfm_energy_f64(pr, pr_prev, pi, pi_prev, chi, chi_offset, mask, damp_prev, ncomp, N, dt,
+                          c, partials, out, stream);
 }
 
 extern "C" int lfm_chi_stats_f64(const float* chi, double chi_offset, int32_t N,

@@ -66,6 +66,8 @@ SIGNATURES = [
     ("lfm_reduce_partials_len", _i64, [_i32]),
     ("lfm_energy_f64", ctypes.c_int,
      [_vp, _vp, _vp, _vp, _vp, _f64, _vp, _i32, _i32, _i32, _f64, _f64, _vp, _vp, _vp]),
+    ("lfm_energy_f64_noalloc", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _f64, _vp, _i32, _i32, _i32, _f64, _f64, _vp, _vp]),
     ("lfm_chi_stats_f64", ctypes.c_int, [_vp, _f64, _i32, _f64, _f64, _f64, _vp, _vp, _vp]),
 ]
 
