@@ -11,9 +11,17 @@ the fused stencil kernel.
 
 Prints one JSON line.  `value` is site updates per second of the whole step
 (GLUPS, inputs resident in HBM).  `roofline` is for the dominant kernel, the
-fused COLOR stencil, timed with HIP events on its own launch stream.
-`cpu_baseline` times the NumPy oracle (float32, one core) on a bounded
-64^3 sample of the same workload.
+fused COLOR stencil, timed with HIP events on its own launch stream, on its
+algorithmic 100 B per site update (the |Psi'|^2 it also writes for the next
+transform is reported beside it, 112 B); `step_roofline` is the whole step on
+its 160 B per site.  `batched` and `adjoint` carry the metric's other two
+clauses, each at its own configuration: the 64-lattice batched sweep against
+64 sequential plain runs (lattices/s, 500 steps) and the COMPLEX 128^3
+400-step adjoint against a plain forward run (cost ratios).  `cpu_baseline`
+times the NumPy oracle (float32, one core) on bounded samples of the REAL
+64^3, COMPLEX 128^3 and COLOR 64^3 workloads, with the host's CPU model,
+NumPy version and thread variables.  --quick leaves out the batched and
+adjoint sub-results.
 
 Other workloads (--workload): complex512 (gravity_em, periodic, mask-free
 kernel), real64 (gravity_only 64^3), batch64 (batched sweep of 64 COMPLEX
@@ -27,7 +35,6 @@ import argparse
 import json
 import os
 import platform
-import subprocess
 import sys
 import time
 
@@ -38,15 +45,21 @@ sys.path.insert(0, ROOT)
 METRIC = "GLUPS + %HBM roofline at 512³ COLOR fp32; batched & adjoint speedup vs plain"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per site update of the stencil kernel (float32):
-# every state value read once, every new value written once.
-#   COLOR, mask: reads 6 Psi + 6 Psi_prev + chi + chi_prev + mask + 3 S_a = 72 B,
-#                writes 6 Psi' + chi' = 28 B, plus |Psi_a'|^2 (12 B) that the
-#                kernel emits as the next S_a transform input           -> 112 B
+# Algorithmic HBM bytes per site update (float32): every state value read
+# once, every new value written once (three-buffer rotation).
+#   COLOR, mask, fused stencil kernel: reads 6 Psi + 6 Psi_prev + chi + chi_prev
+#                + mask + 3 S_a = 72 B, writes 6 Psi' + chi' = 28 B      -> 100 B
+#                (b_extra: the kernel also writes |Psi_a'|^2, 12 B, the next
+#                S_a transform's input, which the whole-step figure books to
+#                the transform)
+#   COLOR whole step: the kernel's 100 B + |Psi'|^2 written (12) + real-to-half-
+#                spectrum read and write (12 + 12) + half-spectrum-to-real read
+#                and write (12 + 12)                                     -> 160 B
 #   COMPLEX, no mask: reads Pr, Pi, Pr_p, Pi_p, chi, chi_p; writes Pr', Pi', chi' -> 36 B
 #   REAL, mask: reads E, E_p, chi, chi_p, mask; writes E', chi'         -> 28 B
 WORKLOADS = {
-    "color512": dict(level=2, N=512, preset="full_physics", mask=True, b_alg=112, streams=(18, 10),
+    "color512": dict(level=2, N=512, preset="full_physics", mask=True, b_alg=100, b_extra=12,
+                     b_step=160, streams=(18, 10),
                      desc="COLOR full_physics 512^3 fp32, mask on, S_a FFT every step"),
     # b_alg2 / streams2: per step of the two-step kernel (one launch reads the
     # fields once and writes both steps' Psi and chi)
@@ -150,38 +163,68 @@ def gpu_ics(w, N, seed=42):
     return pr, pi
 
 
-def cpu_baseline(w, budget_s=12.0):
-    """Oracle (NumPy float32, single core) on a bounded sample: N = 64."""
+CPU_CASES = (("real64", 64), ("complex512", 128), ("color512", 64))
+
+
+def host_info():
+    """The host the CPU baseline ran on: core count, CPU model, NumPy version
+    and the thread-count variables (NumPy's element-wise ops and pocketfft run
+    on one thread whatever they say)."""
+    import numpy as np
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS",
+                                          "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS")}
+    return {"nproc": os.cpu_count(), "cpu": model or platform.machine(), "numpy": np.__version__,
+            "threads_env": env}
+
+
+def cpu_case(name, N, budget_s=10.0):
+    """The oracle's float32 NumPy step (one core) on the workload's seeded
+    fields at size N, run for `budget_s` seconds after one untimed step."""
     import numpy as np
     from oracle import lfm_oracle as O
-    import lfm_hip as L
-    N = 64 if w["level"] != 1 else 128
+    w = WORKLOADS[name]
     cfg = make_config(w, N)
-    rng = np.random.default_rng(42)
-    c = N / 2
-    if w["level"] == 2:
-        pos = [tuple(c + 0.35 * c * rng.uniform(-0.5, 0.5, 3)) for _ in range(3)]
-        pr, pi = L.fields.place_solitons(N, pos, 4.0, 3.0, phases=rng.uniform(0, 6.28, 3),
-                                         colors=[0, 1, 2])
-    elif w["level"] == 1:
-        pr, pi = L.fields.gaussian_soliton(N, (c, c, c), 4.0, 3.0, 0.5)
-    else:
-        pr, pi = L.fields.gaussian_soliton(N, (c, c, c), 6.0, 3.0)[0], None
+    pr_t, pi_t = gpu_ics(w, N)
+    pr = pr_t.cpu().numpy()
+    pi = pi_t.cpu().numpy() if pi_t is not None else None
     p = dict(dt2=cfg.dt ** 2, kappa=cfg.kappa, lambda_self=cfg.lambda_self, chi0=cfg.chi0,
              e0_sq=cfg.e0_sq, epsilon_w=cfg.epsilon_w, kappa_c=cfg.kappa_c,
              epsilon_cc=cfg.epsilon_cc, kappa_string=cfg.kappa_string,
              kappa_tube=cfg.kappa_tube, sa_gamma=cfg.sa_gamma, sa_d=cfg.sa_d)
     mask = O.boundary_mask(N, cfg.boundary_fraction) if w["mask"] else np.zeros(N ** 3, np.float32)
-    st = O.OracleState(w["level"], N, pr, pi, np.full(N ** 3, 19.0), mask, p, dtype=np.float32)
+    st = O.OracleState(w["level"], N, pr, pi, np.full(N ** 3, cfg.chi0), mask, p, dtype=np.float32)
     st.step(1)
     steps, t0 = 0, time.perf_counter()
     while time.perf_counter() - t0 < budget_s:
         st.step(1)
         steps += 1
     dt = time.perf_counter() - t0
-    return {"value": N ** 3 * steps / dt / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
-            "sample": f"oracle NumPy float32 {w['preset']} {N}^3, {steps} steps in {dt:.1f} s "
-                      f"({steps / dt:.2f} steps/s, one thread)"}
+    return {"case": f"{['REAL', 'COMPLEX', 'COLOR'][w['level']]} {w['preset']} {N}^3"
+                    f"{'' if w['mask'] else ' periodic'}",
+            "GLUPS": N ** 3 * steps / dt / 1e9, "MLUPS": N ** 3 * steps / dt / 1e6,
+            "steps_per_s": steps / dt, "steps": steps, "seconds": dt}
+
+
+def cpu_baseline(w, budget_s=10.0):
+    """The oracle's float32 NumPy step on one core, timed on bounded samples:
+    REAL gravity_only 64^3, COMPLEX gravity_em 128^3 (periodic) and COLOR
+    full_physics 64^3, each for `budget_s` seconds.  `value` is the case of
+    the measured workload's level."""
+    cases = [cpu_case(name, N, budget_s) for name, N in CPU_CASES]
+    mine = cases[w["level"]]
+    return {"value": mine["GLUPS"], "unit": "GLUPS", "cores": 1, "kind": "port",
+            "sample": f"oracle NumPy float32 {mine['case']}, {mine['steps']} steps in "
+                      f"{mine['seconds']:.1f} s ({mine['steps_per_s']:.2f} steps/s, one thread)",
+            "cases": cases, "host": host_info()}
 
 
 def batch64_recipe(nlat=64, seed=42):
@@ -307,7 +350,7 @@ def run_batch(args):
                         "kernel": "step_tiled_batch<COMPLEX>", "kernel_ms": kms,
                         "bytes_per_site": b_alg},
            "finite": ok, "batch_equals_sequential_lattice0": same}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def run_adjoint(args):
@@ -367,7 +410,7 @@ def run_adjoint(args):
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
         res[name] = {"stride": stride, "wall_s": dt, "cost_ratio": dt / t_fwd,
-                     "loss": float(loss), "grad_kappa": float(k.grad),
+                     "loss": float(loss.detach()), "grad_kappa": float(k.grad),
                      "grad_psi_norm": float(torch.sqrt((pr_t.grad ** 2).sum() + (pi_t.grad ** 2).sum())),
                      "finite": bool(torch.isfinite(pr_t.grad).all() and torch.isfinite(pi_t.grad).all())}
         del pr_t, pi_t, chiT, loss
@@ -378,7 +421,7 @@ def run_adjoint(args):
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"adjoint COMPLEX {N}^3, {T} steps, mask on", "grid": N},
            "forward_s": t_fwd, "forward_GLUPS": N ** 3 * T / t_fwd / 1e9, "tapes": res}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def run_relax(args):
@@ -454,6 +497,8 @@ def main():
                     choices=sorted(WORKLOADS) + ["copy", "batch64", "adjoint128", "relax64"])
     ap.add_argument("--size", type=int, default=None, help="override the lattice size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--quick", action="store_true",
+                    help="COLOR line only: skip the batched and adjoint sub-results")
     ap.add_argument("--direct", action="store_true", help="use the untiled kernel")
     ap.add_argument("--tile", default="auto", help="tile shape 1..6, 0 default, or auto (tune)")
     ap.add_argument("--chunk", type=int, default=0, help="i planes per workgroup (0 auto)")
@@ -474,10 +519,10 @@ def main():
     import lfm_hip as L
 
     if args.workload == "batch64":
-        run_batch(args)
+        print(json.dumps(run_batch(args)), flush=True)
         return
     if args.workload == "adjoint128":
-        run_adjoint(args)
+        print(json.dumps(run_adjoint(args)), flush=True)
         return
     if args.workload == "relax64":
         run_relax(args)
@@ -566,6 +611,8 @@ def main():
     n3 = N ** 3
     b_alg = w["b_alg2"] if pairs else w["b_alg"]
     achieved = b_alg * n3 / (kms * 1e-3) / 1e9
+    # what the kernel must move including its extra outputs (COLOR: |Psi'|^2)
+    b_kernel = b_alg + w.get("b_extra", 0)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     lvl = ['REAL', 'COMPLEX', 'COLOR'][w['level']]
@@ -602,21 +649,49 @@ def main():
         "tile": {"shape": ev.tile, "chunk": ev.chunk,
                  "tuned_ms": {tune_key(t, c): round(v, 4) for (t, c), v in tuning.items()} if tuning else None},
     }
-    if not args.no_cpu_baseline and args.size is None:
-        cb = cpu_baseline(w)
-        try:
-            cb["host"] = {"nproc": os.cpu_count(), "cpu": platform.processor() or
-                          subprocess.run(["sh", "-c", "grep -m1 'model name' /proc/cpuinfo"],
-                                         capture_output=True, text=True).stdout.strip()}
-        except Exception:
-            pass
-        out["cpu_baseline"] = cb
+    if b_kernel != b_alg:
+        # the kernel's rate counting the |Psi'|^2 it also writes
+        out["roofline"]["achieved_incl_extra_outputs"] = b_kernel * n3 / (kms * 1e-3) / 1e9
+        out["roofline"]["bytes_per_site_incl_extra_outputs"] = b_kernel
+    if "b_step" in w:
+        # the whole step (stencil kernel + the five S_a transform passes) on
+        # its algorithmic bytes per site
+        ach = w["b_step"] * n3 / (ms * 1e-3) / 1e9
+        out["step_roofline"] = {"bytes_per_site": w["b_step"], "achieved": ach, "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "ms_per_step": ms}
     if args.size is None and not args.direct:
         # the same access pattern with nothing but streaming: the achievable rate
         ceil = stream_ceiling(*(w["streams2"] if pairs else w["streams"]), n3)
         out["roofline"]["stream_ceiling"] = ceil
-        out["roofline"]["frac_of_stream_ceiling"] = achieved / ceil
+        out["roofline"]["frac_of_stream_ceiling"] = (b_kernel * n3 / (kms * 1e-3) / 1e9) / ceil
+    del ev
+    torch.cuda.empty_cache()
+    if args.workload == "color512" and args.size is None and not args.direct and not args.quick:
+        # the metric's other two clauses: batched sweep and adjoint speedups
+        # against plain runs, at their own configurations
+        out["batched"] = _summary_batch(run_batch(argparse.Namespace(
+            steps=500, warmup=args.warmup, tile="auto", chunk=0)))
+        torch.cuda.empty_cache()
+        out["adjoint"] = _summary_adjoint(run_adjoint(argparse.Namespace(steps=400, size=None)))
+        torch.cuda.empty_cache()
+    if not args.no_cpu_baseline and args.size is None:
+        out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)
+
+
+def _summary_batch(r):
+    keep = ("value", "unit", "steps", "ms_per_step", "config", "batched", "sequential_graphs",
+            "sequential_eager", "speedup_vs_sequential", "roofline", "finite",
+            "batch_equals_sequential_lattice0")
+    return {k: r[k] for k in keep if k in r}
+
+
+def _summary_adjoint(r):
+    out = {k: r[k] for k in ("value", "unit", "steps", "ms_per_step", "config", "forward_s",
+                             "forward_GLUPS") if k in r}
+    out["cost_ratio"] = {name: t["cost_ratio"] for name, t in r["tapes"].items()}
+    out["tapes"] = r["tapes"]
+    return out
 
 
 if __name__ == "__main__":

@@ -11,27 +11,49 @@ oracle); fed float32 arrays it mirrors the reference's float32 NumPy path
 (the CPU baseline).  Unlike the reference ``step_color`` it never drops to
 float32 scratch, so the float64 run is float64 end to end.
 
-Pinning (what checks each function independently of this restatement):
+Pinning (what checks each function independently of this restatement;
+tests/test_oracle_pins.py, test_oracle_golden.py, test_oracle_legacy.py):
 
 * ``laplacian_19pt``   -- plane-wave symbol closed form, x^2 -> 2, constants
-                          -> 0, cubic symmetry (tests/test_oracle_pins.py).
-* ``boundary_mask``    -- centre 0 / corner 1 / analytic taper values at known
-                          radii; golden vectors from the reference.
+                          -> 0, cubic symmetry.
+* ``boundary_mask``    -- centre 0 / corner 1 / taper values at known radii;
+                          bit-equal to the reference's create_boundary_mask at
+                          N = 8, 16 (two boundary fractions) and by SHA-256 at
+                          N = 64, 128 (tests/golden/step_real_complex.npz).
 * ``step_real``        -- exact discrete dispersion recurrence for GOV-01,
                           scalar GOV-02 recurrences (gravity, Mexican hat,
-                          clamp), mask blend; golden vectors.
+                          clamp), mask blend; the reference's own masked,
+                          damped-prev outputs (three steps, N = 8 float32 bit
+                          for bit, N = 8 and 16 float64 to 1e-14); the legacy
+                          FP64 integrator (lfm/core/integrator.py:128-160,
+                          restated in legacy_integrator.py) with no mask.
 * ``step_complex``     -- as REAL plus the plane-wave current closed form
-                          j = A^2 * sum_d sin(k_d); golden vectors.
+                          j = A^2 * sum_d sin(k_d); the reference's masked,
+                          damped-prev outputs as for REAL; the legacy
+                          integrator with epsilon_w = 0 and no mask.
 * ``step_color``       -- colour special cases (singlet f_c = 0, one colour
                           f_c = 2/3, CCV/SCV/eps_cc vanish for identical
                           colours, one-colour CCV = 2/3 sum j^2); golden
                           vectors from the reference ``step_color`` (all
-                          terms on, including SCV).
+                          terms on, including SCV, N = 8); the legacy
+                          integrator (f_c and cross-colour terms) with no
+                          mask, no currents, CCV or SCV.
 * ``helmholtz_sa``     -- single-mode closed form S = A + B*g/(g+D k^2) cos;
-                          DC gain 1; golden vectors.
-* ``energy_components``-- plane-wave closed forms; np.gradient is the NumPy
-                          library routine the reference calls.
-* ``chi_statistics`` / ``interior_mask`` -- counted by brute force.
+                          DC gain 1; inside the step_color golden vectors.
+* ``OracleState``      -- the A/B toggle through all the step vectors above;
+                          ``step(freeze_chi=True)`` by the alternating-chi
+                          mode recurrence (uniform A and B chi sets).
+* ``energy_components``-- real plane-wave and complex plane-wave closed forms
+                          (T, G with np.gradient's one-sided edges, V; the
+                          colour axis summed); the reference's metrics() of
+                          two reference-written checkpoints (COMPLEX and COLOR,
+                          1e-12); np.gradient is the NumPy routine the
+                          reference calls.
+* ``chi_statistics`` / ``interior_mask`` -- counted by brute force; the
+                          reference checkpoints' metrics (the reference takes
+                          mean and std in float32, so those agree to its
+                          rounding).
+* ``poisson_chi``      -- single-mode closed form, DC dropped.
 * ``relax_*``          -- a plane wave in uniform chi is a fixed shape of the
                           rescaled imaginary-time step and its eigenvalue is
                           sqrt(chi^2 - lambda(k)); a mode's amplitude against
