@@ -51,7 +51,10 @@ HBM_PEAK_GBS = 8000.0
 #                + mask + 3 S_a = 72 B, writes 6 Psi' + chi' = 28 B      -> 100 B
 #                (b_extra: the kernel also writes |Psi_a'|^2, 12 B, the next
 #                S_a transform's input, which the whole-step figure books to
-#                the transform)
+#                the transform; b_scv_saving: the Evolver's kernel reads the
+#                SCV term, 4 B, which the last transform pass forms from the
+#                three S_a values, instead of the 12 B of S_a, so it moves
+#                100 - 8 + 12 = 104 B per site itself)
 #   COLOR whole step: the kernel's 100 B + |Psi'|^2 written (12) + real-to-half-
 #                spectrum read and write (12 + 12) + half-spectrum-to-real read
 #                and write (12 + 12)                                     -> 160 B
@@ -59,7 +62,7 @@ HBM_PEAK_GBS = 8000.0
 #   REAL, mask: reads E, E_p, chi, chi_p, mask; writes E', chi'         -> 28 B
 WORKLOADS = {
     "color512": dict(level=2, N=512, preset="full_physics", mask=True, b_alg=100, b_extra=12,
-                     b_step=160, streams=(18, 10),
+                     b_scv_saving=8, b_step=160, streams=(18, 10), streams_scv=(16, 10),
                      desc="COLOR full_physics 512^3 fp32, mask on, S_a FFT every step"),
     # b_alg2 / streams2: per step of the two-step kernel (one launch reads the
     # fields once and writes both steps' Psi and chi)
@@ -611,8 +614,11 @@ def main():
     n3 = N ** 3
     b_alg = w["b_alg2"] if pairs else w["b_alg"]
     achieved = b_alg * n3 / (kms * 1e-3) / 1e9
-    # what the kernel must move including its extra outputs (COLOR: |Psi'|^2)
-    b_kernel = b_alg + w.get("b_extra", 0)
+    # what the kernel itself must move: its extra outputs (COLOR: |Psi'|^2) in,
+    # the S_a it no longer reads (COLOR on the SCV-term path) out
+    scv_path = bool(getattr(ev, "_sa_scv", False)) and not args.direct
+    b_kernel = b_alg + w.get("b_extra", 0) - (w.get("b_scv_saving", 0) if scv_path else 0)
+    streams = w["streams2"] if pairs else (w["streams_scv"] if scv_path else w["streams"])
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     lvl = ['REAL', 'COMPLEX', 'COLOR'][w['level']]
@@ -650,9 +656,12 @@ def main():
                  "tuned_ms": {tune_key(t, c): round(v, 4) for (t, c), v in tuning.items()} if tuning else None},
     }
     if b_kernel != b_alg:
-        # the kernel's rate counting the |Psi'|^2 it also writes
-        out["roofline"]["achieved_incl_extra_outputs"] = b_kernel * n3 / (kms * 1e-3) / 1e9
-        out["roofline"]["bytes_per_site_incl_extra_outputs"] = b_kernel
+        # the kernel's rate on the bytes it moves itself (COLOR: plus the
+        # |Psi'|^2 it writes, minus the 8 B of S_a it does not read when the
+        # transform hands it the SCV term)
+        out["roofline"]["achieved_kernel_bytes"] = b_kernel * n3 / (kms * 1e-3) / 1e9
+        out["roofline"]["kernel_bytes_per_site"] = b_kernel
+        out["roofline"]["sa_input"] = "SCV term (4 B)" if scv_path else "S_a fields (12 B)"
     if "b_step" in w:
         # the whole step (stencil kernel + the five S_a transform passes) on
         # its algorithmic bytes per site
@@ -661,7 +670,7 @@ def main():
                                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "ms_per_step": ms}
     if args.size is None and not args.direct:
         # the same access pattern with nothing but streaming: the achievable rate
-        ceil = stream_ceiling(*(w["streams2"] if pairs else w["streams"]), n3)
+        ceil = stream_ceiling(*streams, n3)
         out["roofline"]["stream_ceiling"] = ceil
         out["roofline"]["frac_of_stream_ceiling"] = (b_kernel * n3 / (kms * 1e-3) / 1e9) / ceil
     del ev

@@ -49,6 +49,12 @@ extern "C" {
 #define LFM_F_CHI_DELTA 0x10  /* every chi buffer (in and out) holds chi - chi0: the
                                  leapfrog accumulates the small deviation, which
                                  float32 resolves ~1e6 times finer than chi ~ 19 */
+#define LFM_F_SA_SCV 0x20     /* COLOR: `sa` is one N^3 field holding the SCV term
+                                 sum_a S_a^2 - 1/3 (sum_a S_a)^2 of the clipped S_a,
+                                 as lfm_scv_apply_scv_f32 writes it, instead of the
+                                 three S_a fields (same bits; 4 instead of 12 bytes
+                                 per site read); needs kappa_tube > 0 and no
+                                 sa_clip_out */
 
 /* Number of tile shapes of the tiled kernel (lfm_params_t.tile = 1..this). */
 #define LFM_TILE_VARIANTS 6
@@ -158,6 +164,15 @@ int lfm_scv_apply_psisq(void* plan, const float* e, float gamma, float D, float*
 /* e computed from pr, pi first (into the plan's own buffer). */
 int lfm_scv_apply(void* plan, const float* pr, const float* pi, float gamma, float D,
                   float* sa_out, void* stream);
+/* The SCV term of the COLOR step straight from the transform (for
+ * lfm_step_color_f32 with LFM_F_SA_SCV): scv_out[site] =
+ * sum_a S_a^2 - 1/3 (sum_a S_a)^2 with S_a = max(0, irfftn(gamma/(gamma + D k^2)
+ * rfftn(e_a))) (numpy_backend.py:338-358, the SCV of kernel_source.py:187-197),
+ * bit for bit what the step computes from the three S_a fields, written as
+ * one N^3 field.  e = |Psi_a|^2 (3 N^3) when given, else computed from pr, pi.
+ * Direct-method plans for 3 colours only. */
+int lfm_scv_apply_scv_f32(void* plan, const float* pr, const float* pi, const float* e,
+                          float gamma, float D, float* scv_out, void* stream);
 /* GOV-04 chi equilibration (lfm/fields/equilibrium.py:26-59,62-100, used by
  * Simulation.equilibrate, lfm/simulation.py:877-963): solves
  * lap(delta) = kappa (|Psi|^2 - e0_sq) spectrally with -1/k^2 (DC dropped),

@@ -20,6 +20,7 @@
 
 #include "lfm_internal.h"
 #include "lfm_fft.h"
+#include "lfm_stencil.h"
 
 namespace lfm {
 namespace {
@@ -398,6 +399,92 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_c2r(const float2* 
     }
 }
 
+// Half spectra -> the SCV term of the step kernel, for three colours:
+// fft_rows_c2r's inverse transform on the same row pair (i, j), (i, j+1) of
+// every colour in one workgroup (PJ row pairs per colour), then
+// scv[site] = scv_term(S_0, S_1, S_2) -- the clipped-S_a combination the
+// step kernel otherwise forms from the three S_a fields it reads, here
+// formed where S_a is produced, so one float per site is written instead of
+// three (and read by the step instead of three).  Colours 1 and 2 pass their
+// values to colour 0's threads through LDS (same positions t + e*TC).
+template <int N>
+constexpr int scv_pairs() { return N / 8 >= 64 ? 1 : 64 / (N / 8); }
+template <int N>
+constexpr int scv_threads() { return 3 * scv_pairs<N>() * (N / 8); }
+
+template <int N>
+__global__ __launch_bounds__(scv_threads<N>()) void fft_rows_c2r_scv(
+        const float2* __restrict__ spec, float* __restrict__ scv,
+        const float2* __restrict__ tw, int Hp) {
+    constexpr int TC = N / 8;
+    constexpr int PJ = scv_pairs<N>();
+    constexpr int PS = rows_pitch<N>();
+    constexpr long long NP = (long long)N * N / 2;   // row pairs per colour
+    __shared__ float2 lds[3 * PJ * PS];
+    __shared__ float2 twl[N];
+    const int q = threadIdx.x / TC, t = threadIdx.x % TC;
+    const int a = q / PJ, lp = q % PJ;               // colour, row pair in the workgroup
+    const long long pair = (long long)blockIdx.x * PJ + lp;
+    const float2* s0 = spec + (a * NP + pair) * 2 * Hp;
+    const float2* s1 = s0 + Hp;
+    load_twiddles<N>(twl, tw);
+    constexpr int H = N / 2 + 1;
+    static_assert(2 * H <= PS, "both half spectra fit the line's LDS slot");
+    float2* L = lds + q * PS;
+#pragma unroll
+    for (int e = 0; e < (H + TC - 1) / TC; ++e) {
+        const int k = t + e * TC;
+        if (k < H) {
+            L[k] = ld2nt<N>(s0 + k);
+            L[H + k] = ld2nt<N>(s1 + k);
+        }
+    }
+    __syncthreads();
+    float2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int k = t + e * TC;
+        const bool lo = k <= N / 2;
+        const int m = lo ? k : N - k;
+        float2 x = L[m], y = L[H + m];
+        if (m == 0 || m == N / 2) { x.y = 0.0f; y.y = 0.0f; }
+        if (!lo) { x.y = -x.y; y.y = -y.y; }
+        v[e] = make_float2(x.x - y.y, x.y + y.x);
+    }
+    fft_line<N>(v, L, t, 1, twl, [](int pos) { return pos + pos / 8; });
+    // v[e] = (S_a at row 2 pair, S_a at row 2 pair + 1), position t + e TC.
+    // The transform's last LDS reads precede this barrier; colours 1 and 2
+    // then park their results in their own slots for colour 0.
+    __syncthreads();
+    if (a > 0) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) L[t + e * TC] = v[e];
+    }
+    __syncthreads();
+    if (a == 0) {
+        const float2* L1 = lds + (PJ + lp) * PS;
+        const float2* L2 = lds + (2 * PJ + lp) * PS;
+        float* r0 = scv + pair * 2 * N;
+        float* r1 = r0 + N;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float2 w1 = L1[t + e * TC], w2 = L2[t + e * TC];
+            float clip[3];
+            st1nt<N>(r0 + t + e * TC, scv_term(v[e].x, w1.x, w2.x, clip));
+            st1nt<N>(r1 + t + e * TC, scv_term(v[e].y, w1.y, w2.y, clip));
+        }
+    }
+}
+
+template <int N>
+int launch_c2r_scv(const float2* spec, float* scv, const float2* tw, int Hp, hipStream_t st) {
+    const long long nb = (long long)N * N / 2 / scv_pairs<N>();
+    hipLaunchKernelGGL(fft_rows_c2r_scv<N>, dim3(unsigned(nb)), dim3(scv_threads<N>()), 0, st,
+                       spec, scv, tw, Hp);
+    LFM_LAUNCH_CHECK("fft_rows_c2r_scv");
+    return LFM_OK;
+}
+
 template <int N>
 int launch_columns(const ColArgs& A, hipStream_t st) {
     constexpr int NCOL = ncol_for<N>();
@@ -444,6 +531,15 @@ int rows(int N, int dir, const float* in, float2* spec, float* out, const float2
          int nc, hipStream_t st) {
     switch (N) {
 #define LFM_CASE(n) case n: return launch_rows<n>(dir, in, spec, out, tw, Hp, nc, st);
+        LFM_FFT_SIZES(LFM_CASE)
+#undef LFM_CASE
+        default: set_error("row FFT: N=%d unsupported", N); return LFM_E_ARG;
+    }
+}
+
+int rows_scv(int N, const float2* spec, float* scv, const float2* tw, int Hp, hipStream_t st) {
+    switch (N) {
+#define LFM_CASE(n) case n: return launch_c2r_scv<n>(spec, scv, tw, Hp, st);
         LFM_FFT_SIZES(LFM_CASE)
 #undef LFM_CASE
         default: set_error("row FFT: N=%d unsupported", N); return LFM_E_ARG;
@@ -548,6 +644,17 @@ int run_filter(ScvPlan* p, const float* e, int filter, float gamma, float D, flo
 
 int run(ScvPlan* p, const float* e, float gamma, float D, float* sa_out, hipStream_t st) {
     return run_filter(p, e, 0, gamma, D, sa_out, st, p->nc);
+}
+
+// Direct method, three colours: e -> spectrum -> Helmholtz filter -> the
+// SCV term (one field) instead of the three S_a fields.
+int run_scv(ScvPlan* p, const float* e, float gamma, float D, float* scv_out, hipStream_t st) {
+    const int N = p->N;
+    int rc = rows(N, 0, e, p->spec, nullptr, p->tw, p->Hp, 3, st);
+    if (rc) return rc;
+    rc = run_columns(p, 0, gamma, D, st, 3);
+    if (rc) return rc;
+    return rows_scv(N, p->spec, scv_out, p->tw, p->Hp, st);
 }
 
 }  // namespace
@@ -717,6 +824,29 @@ extern "C" int lfm_scv_columns_f32(void* plan, float gamma, float D, void* strea
         return LFM_E_ARG;
     }
     return scv_columns(plan, gamma, D, (hipStream_t)stream);
+}
+
+extern "C" int lfm_scv_apply_scv_f32(void* plan, const float* pr, const float* pi,
+                                     const float* e, float gamma, float D, float* scv_out,
+                                     void* stream) {
+    auto* p = static_cast<ScvPlan*>(plan);
+    if (!p || !scv_out || (!e && (!pr || !pi))) {
+        set_error("lfm_scv_apply_scv_f32: bad argument");
+        return LFM_E_ARG;
+    }
+    if (p->method != 2 || p->nc != 3) {
+        set_error("lfm_scv_apply_scv_f32: needs a direct-method plan for 3 colours "
+                  "(method %d, %d colours)", p->method, p->nc);
+        return LFM_E_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (!e) {
+        const size_t n = size_t(3) * p->N * p->N * p->N;
+        hipLaunchKernelGGL(psisq_kernel, dim3(grid_for(n)), dim3(256), 0, st, pr, pi, p->e, n);
+        LFM_LAUNCH_CHECK("psisq_kernel");
+        e = p->e;
+    }
+    return run_scv(p, e, gamma, D, scv_out, st);
 }
 
 extern "C" int lfm_clip_nonneg_f32(const float* x, float* y, int64_t n, void* stream) {

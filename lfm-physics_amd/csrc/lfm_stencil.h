@@ -22,4 +22,22 @@ __device__ __forceinline__ float lap_part_b(float ctr, float faces) {
     return fmaf(WF, ctr, __fmul_rn(WE, faces));
 }
 
+// The S_a colour-variance term of GOV-02 (kernel_source.py:187-197):
+// SCV = sum_a S_a^2 - 1/3 (sum_a S_a)^2 of the S_a clipped at 0
+// (numpy_backend.py:353), every rounding explicit.  The step kernels apply
+// it to S_a they read; the last transform pass evaluates the same function
+// on the S_a it produces, so either route gives the same bits.
+__device__ __forceinline__ float scv_term(float s0, float s1, float s2, float (&clip)[3]) {
+    const float s[3] = {s0, s1, s2};
+    float s_sum = 0.0f, s_sq = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const float sv = s[q] > 0.0f ? s[q] : 0.0f;
+        clip[q] = sv;
+        s_sum = __fadd_rn(s_sum, sv);
+        s_sq = fmaf(sv, sv, s_sq);
+    }
+    return fmaf(-1.0f / 3.0f, __fmul_rn(s_sum, s_sum), s_sq);
+}
+
 }  // namespace lfm

@@ -147,8 +147,12 @@ __device__ __forceinline__ PointData<L> load_point(const KParams& kp, const Ptrs
     d.chip = (kp.flags & LFM_F_FREEZE_CHI) ? 0.0f : idx.ldp(P.chip);
     if constexpr (L == COLOR && SA) {
         if (P.sa) {
+            if (kp.flags & LFM_F_SA_SCV) {
+                d.sa[0] = idx.ldp(P.sa);          // the SCV term itself
+            } else {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) d.sa[q] = idx.ldp(P.sa + q * P.n3);
+                for (int q = 0; q < 3; ++q) d.sa[q] = idx.ldp(P.sa + q * P.n3);
+            }
         }
     }
     return d;
@@ -159,7 +163,8 @@ __device__ __forceinline__ PointData<L> load_point(const KParams& kp, const Ptrs
 //  jsum     sum over colours and directions of j_{a,d}   (COMPLEX/COLOR)
 //  ccv      sum_d (sum_a j_{a,d}^2 - 1/3 (sum_a j_{a,d})^2)  (COLOR)
 //  chi_st, lap_chi  chi as stored and the Laplacian of the stored field
-//  scv_on   the S_a term is on (COLOR); pd.sa holds the raw S_a
+//  scv_on   the S_a term is on (COLOR); pd.sa holds the raw S_a, or with
+//           LFM_F_SA_SCV the SCV term in pd.sa[0]
 // Out: nw (new Psi), cn (new chi as stored; chi_st under LFM_F_FREEZE_CHI),
 // sclip (S_a clipped at 0, COLOR with scv_on).
 template <int L>
@@ -233,16 +238,9 @@ __device__ __forceinline__ void site_compute(const KParams& kp, const PointData<
                 }
             }
             if (scv_on) {
-                float s_sum = 0.0f, s_sq = 0.0f;
-#pragma unroll
-                for (int q = 0; q < C; ++q) {
-                    float sv = pd.sa[q];
-                    sv = sv > 0.0f ? sv : 0.0f;
-                    sclip[q] = sv;
-                    s_sum = __fadd_rn(s_sum, sv);
-                    s_sq = fmaf(sv, sv, s_sq);
-                }
-                scv = fmaf(-1.0f / 3.0f, __fmul_rn(s_sum, s_sum), s_sq);
+                // LFM_F_SA_SCV: pd.sa[0] already holds scv_term of the S_a
+                scv = (kp.flags & LFM_F_SA_SCV) ? pd.sa[0]
+                                                : scv_term(pd.sa[0], pd.sa[1], pd.sa[2], sclip);
             }
         }
     }
@@ -1559,9 +1557,15 @@ extern "C" int lfm_step_color_f32(const float* pr, const float* pr_prev, const f
         set_error("lfm_step_color_f32: sa_clip_out given without sa");
         return LFM_E_ARG;
     }
+    const bool sa_scv = (p->flags & LFM_F_SA_SCV) != 0;
+    if (sa_scv && (sa_clip_out || !(p->kappa_tube > 0.0f))) {
+        set_error("lfm_step_color_f32: LFM_F_SA_SCV needs kappa_tube > 0 and no sa_clip_out "
+                  "(the S_a fields themselves are not passed)");
+        return LFM_E_ARG;
+    }
     const size_t n3 = size_t(p->N) * p->N * p->N;
     const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask, sa};
-    const size_t nin[] = {3 * n3, 3 * n3, 3 * n3, 3 * n3, n3, n3, n3, 3 * n3};
+    const size_t nin[] = {3 * n3, 3 * n3, 3 * n3, 3 * n3, n3, n3, n3, sa_scv ? n3 : 3 * n3};
     float* outs[] = {pr_out, pr_prev_out, pi_out, pi_prev_out, chi_out, chi_prev_out,
                      sa_clip_out, psisq_out};
     const size_t nout[] = {3 * n3, 3 * n3, 3 * n3, 3 * n3, n3, n3, 3 * n3, 3 * n3};
@@ -1597,8 +1601,8 @@ extern "C" int lfm_step_color_fused_f32(const float* pr, const float* pr_prev, c
         set_error("lfm_step_color_fused_f32: needs kappa_tube > 0 (use lfm_step_color_f32)");
         return LFM_E_ARG;
     }
-    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI)) {
-        set_error("lfm_step_color_fused_f32: LFM_F_DIRECT / LFM_F_FREEZE_CHI not supported");
+    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI | LFM_F_SA_SCV)) {
+        set_error("lfm_step_color_fused_f32: LFM_F_DIRECT / LFM_F_FREEZE_CHI / LFM_F_SA_SCV not supported");
         return LFM_E_ARG;
     }
     ScvView v{};
@@ -1653,6 +1657,7 @@ extern "C" int lfm_batch_plan_create(int32_t level, int32_t nlat, const lfm_para
         L.ncomp = ncomp;
         L.CH = chunk > 0 ? std::min<int>(chunk, p.N) : std::min<int>(16, p.N);
         L.kp = to_k(&p);
+        L.kp.flags &= LFM_F_MASK;   // per-lattice flags: the mask only (the rest is per call)
         off += (long long)p.N * p.N * p.N;
     }
     // Largest lattices first (their tiles are the fullest), tiles of one

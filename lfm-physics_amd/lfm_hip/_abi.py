@@ -23,6 +23,7 @@ LFM_F_DAMP_PREV = 0x02
 LFM_F_FREEZE_CHI = 0x04
 LFM_F_DIRECT = 0x08
 LFM_F_CHI_DELTA = 0x10
+LFM_F_SA_SCV = 0x20
 LFM_ABI_VERSION = 3
 LFM_TILE_VARIANTS = 6
 LFM_RELAX_SLOT_WORDS = 2048
@@ -48,6 +49,7 @@ SIGNATURES = [
     ("lfm_scv_plan_destroy", ctypes.c_int, [_vp]),
     ("lfm_scv_apply_psisq", ctypes.c_int, [_vp, _vp, _f32, _f32, _vp, _vp]),
     ("lfm_scv_apply", ctypes.c_int, [_vp, _vp, _vp, _f32, _f32, _vp, _vp]),
+    ("lfm_scv_apply_scv_f32", ctypes.c_int, [_vp, _vp, _vp, _vp, _f32, _f32, _vp, _vp]),
     ("lfm_poisson_chi_f32", ctypes.c_int, [_vp, _vp, _vp, _i32, _f32, _f32, _f32, _f64, _vp, _vp]),
     ("lfm_clip_nonneg_f32", ctypes.c_int, [_vp, _vp, _i64, _vp]),
     ("lfm_batch_plan_create", ctypes.c_int, [_i32, _i32, _vp, _i32, _i32, ctypes.POINTER(_vp)]),

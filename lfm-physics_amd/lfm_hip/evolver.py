@@ -120,14 +120,24 @@ class Evolver:
         self.boundary_mask = self.backend.create_boundary_mask(self.N, config.boundary_fraction)
         self._absorb = 1.0 - self.boundary_mask
         self.sa_enabled = bool(config.sa_enabled and lvl == FieldLevel.COLOR)
-        if sa_transform not in ("auto", "fused", "separate"):
-            raise ValueError(f"sa_transform must be 'auto', 'fused' or 'separate', got {sa_transform!r}")
+        if sa_transform not in ("auto", "fused", "separate", "fields"):
+            raise ValueError("sa_transform must be 'auto', 'fused', 'separate' or 'fields', "
+                             f"got {sa_transform!r}")
         self._fused = False
+        self._sa_scv = False
         if self.sa_enabled:
             self._sa = torch.zeros(3 * self.total, dtype=f32, device=dev)
             self._psisq = torch.empty(3 * self.total, dtype=f32, device=dev)
             self._psisq_valid = False
             self._scv = ScvPlan(self.N, 3)
+            # "separate" / "auto" on the hand-written passes: the last pass
+            # writes the step's SCV term (one field) instead of the three S_a
+            # fields (lfm_scv_apply_scv_f32 + LFM_F_SA_SCV; same bits, 16 B
+            # per site less traffic between the two); "fields" keeps the
+            # three S_a fields between them, as the drop-in backend does.
+            # S_a itself is then recomputed on demand (get_sa_fields).
+            self._sa_scv = self._scv.method == 2 and sa_transform in ("auto", "separate")
+            self._scvf = torch.empty(self.total, dtype=f32, device=dev) if self._sa_scv else None
             # "fused": the k-axis transform passes folded into the step kernel
             # (power-of-two N up to 512).  Its full-row workgroups hold one
             # workgroup per CU at N = 512, so it is latency-bound there; measured
@@ -235,6 +245,9 @@ class Evolver:
             flags |= _abi.LFM_F_FREEZE_CHI
         if direct:
             flags |= _abi.LFM_F_DIRECT
+        separate_scv = self.level == 2 and self._sa_scv and not (self._fused and not (freeze_chi or direct))
+        if separate_scv:
+            flags |= _abi.LFM_F_SA_SCV
         p = ctypes.byref(self._p(flags))
         st = stream_handle()
         cc, cp = self._cc, self._cp
@@ -274,7 +287,17 @@ class Evolver:
             self._psisq_valid = False
         else:
             sa = psisq = None
-            if self.sa_enabled:
+            if separate_scv:
+                # the transform writes the step's SCV term; S_a is left stale
+                self._spec_valid = False
+                e = self._psisq.data_ptr() if self._psisq_valid else None
+                _abi.check(lib.lfm_scv_apply_scv_f32(self._scv.handle, pr[cur].data_ptr(),
+                                                     pi[cur].data_ptr(), e, self.config.sa_gamma,
+                                                     self.config.sa_d, self._scvf.data_ptr(), st),
+                           "lfm_scv_apply_scv_f32")
+                self._sa_stale = True
+                sa, psisq = self._scvf.data_ptr(), self._psisq.data_ptr()
+            elif self.sa_enabled:
                 self._spec_valid = False
                 self._sa_stale = False
                 if self._psisq_valid:
@@ -582,13 +605,18 @@ class Evolver:
         input Psi (numpy_backend.py:341), clipped at 0."""
         if self._sa is None:
             return None
-        if self._sa_stale:
-            # fused steps keep S_a only in k space: recompute it from the last
-            # step's input Psi, which the rotation keeps (undamped) in the prev slot
+        self._materialize_sa()
+        return torch.clamp_min(self._sa, 0.0).cpu().numpy().reshape(3, self.N, self.N, self.N)
+
+    def _materialize_sa(self) -> None:
+        """Fused steps keep S_a only in k space and SCV steps only as the SCV
+        term: recompute S_a of the last step from that step's input Psi,
+        which the rotation keeps (undamped) in the prev slot.  Called before
+        anything overwrites or damps that slot."""
+        if self.sa_enabled and self._sa_stale:
             self._scv.apply(self._pr[self._prev], self._pi[self._prev], self.config.sa_gamma,
                             self.config.sa_d, self._sa)
             self._sa_stale = False
-        return torch.clamp_min(self._sa, 0.0).cpu().numpy().reshape(3, self.N, self.N, self.N)
 
     # ------------------------------------------------------------- setters
     def _dev_chi(self, arr) -> torch.Tensor:
@@ -603,6 +631,7 @@ class Evolver:
 
     def _undamp(self) -> None:
         """Store the damped prev values explicitly (before a partial write)."""
+        self._materialize_sa()
         if self._damped and self.has_mask:
             a = self._absorb.repeat(self._n_psi) if self._n_psi > 1 else self._absorb
             self._pr[self._prev].mul_(a)

@@ -981,3 +981,39 @@ def test_tile_shapes_and_chunks_equal_direct_kernel(gpu, level, N):
                 assert err < 1e-5, (key, name, err)
             else:
                 np.testing.assert_array_equal(a, b, err_msg=f"{key} {name}")
+
+
+@pytest.mark.parametrize("N", [16, 64, 128])
+def test_scv_term_path_equals_three_field_path(gpu, N):
+    """COLOR full_physics through the separate transform passes: the default
+    path (the last pass writes the SCV term, the step reads it with
+    LFM_F_SA_SCV) against the path that keeps the three S_a fields between
+    them ("fields", as HipBackend does): the same bits for every field after
+    steps through graph replay, the same S_a read-out (recomputed from the
+    last step's input Psi), and the same bits again after a setter
+    overwrites Psi mid-run (S_a is materialised before the prev slot is
+    overwritten)."""
+    import torch
+    import lfm_hip as L
+    log(f"SCV-term path vs three-field path, COLOR {N}^3")
+    cfg = L.full_physics(N)
+    pr, pi = make_ics(2, N, seed=9)
+    evs = []
+    for mode in ("auto", "fields"):
+        ev = L.Evolver(cfg, backend="hip", sa_transform=mode)
+        assert ev._sa_scv == (mode == "auto")
+        ev.set_psi_real(pr)
+        ev.set_psi_imag(pi)
+        ev.evolve(25)
+        evs.append(ev)
+    torch.cuda.synchronize()
+    a, b = evs
+    for get in ("get_psi_real", "get_psi_imag", "get_chi", "get_chi_prev", "get_psi_real_prev",
+                "get_sa_fields"):
+        np.testing.assert_array_equal(getattr(a, get)(), getattr(b, get)(), err_msg=get)
+    for ev in evs:
+        ev.set_psi_imag(ev.get_psi_imag() * np.float32(0.5))
+        ev.evolve(7)
+    torch.cuda.synchronize()
+    for get in ("get_psi_real", "get_psi_imag", "get_chi", "get_sa_fields"):
+        np.testing.assert_array_equal(getattr(a, get)(), getattr(b, get)(), err_msg=get)
