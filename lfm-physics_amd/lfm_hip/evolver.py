@@ -117,7 +117,10 @@ class Evolver:
         self.mask_policy = mask_policy
         self.has_mask = not (mask_policy == "boundary_type"
                              and config.boundary_type == BoundaryType.PERIODIC)
-        self.boundary_mask = self.backend.create_boundary_mask(self.N, config.boundary_fraction)
+        # held as a plain tensor: the per-step pointer reads skip the
+        # DeviceArray subclass dispatch (the backend's buffers are for callers)
+        self.boundary_mask = self.backend.create_boundary_mask(
+            self.N, config.boundary_fraction).as_subclass(torch.Tensor)
         self._absorb = 1.0 - self.boundary_mask
         self.sa_enabled = bool(config.sa_enabled and lvl == FieldLevel.COLOR)
         if sa_transform not in ("auto", "fused", "separate", "fields"):
