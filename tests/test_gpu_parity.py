@@ -1017,3 +1017,47 @@ def test_scv_term_path_equals_three_field_path(gpu, N):
     torch.cuda.synchronize()
     for get in ("get_psi_real", "get_psi_imag", "get_chi", "get_sa_fields"):
         np.testing.assert_array_equal(getattr(a, get)(), getattr(b, get)(), err_msg=get)
+
+
+@pytest.mark.parametrize("N", [256, 1024])
+def test_scv_transform_at_large_n(gpu, N):
+    """lfm_scv_apply_scv_f32 against the three S_a fields lfm_scv_apply_psisq
+    writes from the same |Psi|^2 (the same transforms up to the last pass):
+    SCV = sum S^2 - 1/3 (sum S)^2 of the clipped S_a, formed in float64 from
+    those fields, within 4 float32 ulps of sum S^2 at every site.  N = 1024
+    runs the pass's 384-lane shape; every other size shares the 192-lane one."""
+    import ctypes
+    import torch
+    from lfm_hip import _abi
+    lib = _abi.load()
+    log(f"SCV-term transform vs three S_a fields at {N}^3")
+    h = ctypes.c_void_p()
+    _abi.check(lib.lfm_scv_plan_create_ex(N, 3, 2, ctypes.byref(h)), "plan")
+    try:
+        g = torch.Generator(device="cuda")
+        g.manual_seed(N)
+        n3 = N ** 3
+        x = torch.arange(N, device="cuda", dtype=torch.float32)
+        e = torch.rand(3 * n3, device="cuda", generator=g)
+        # smooth blobs on top of the noise so S_a spans many magnitudes
+        r2 = ((x.view(N, 1, 1) - N / 2) ** 2 + (x.view(1, N, 1) - N / 3) ** 2
+              + (x.view(1, 1, N) - N / 2) ** 2) / (N / 8) ** 2
+        e.view(3, N, N, N)[0] += 20 * torch.exp(-r2)
+        sa = torch.empty(3 * n3, device="cuda")
+        scv = torch.empty(n3, device="cuda")
+        st = torch.cuda.current_stream().cuda_stream
+        _abi.check(lib.lfm_scv_apply_psisq(h, e.data_ptr(), 0.1, 4.9, sa.data_ptr(), st), "psisq")
+        _abi.check(lib.lfm_scv_apply_scv_f32(h, None, None, e.data_ptr(), 0.1, 4.9, scv.data_ptr(),
+                                             st), "scv")
+        torch.cuda.synchronize()
+        del e
+        s = sa.view(3, n3).clamp_min(0.0).double()
+        ssq = (s * s).sum(0)
+        ref = ssq - (s.sum(0) ** 2) / 3.0
+        tol = 4 * 2.0 ** -23 * ssq + 1e-30
+        bad = int(((scv.double() - ref).abs() > tol).sum())
+        worst = float(((scv.double() - ref).abs() / (ssq + 1e-30)).max())
+        log(f"  sites beyond 4 ulps: {bad}; worst |diff| / sum S^2 {worst:.2e}")
+        assert bad == 0
+    finally:
+        lib.lfm_scv_plan_destroy(h)
