@@ -193,7 +193,9 @@ int lfm_clip_nonneg_f32(const float* x, float* y, int64_t n, void* stream);
  * at ncomp*off_b in the Psi arenas (ncomp = 3 for COLOR, else 1; colour-major
  * inside a lattice).  One lfm_batch_step_f32 call advances every lattice by one
  * step with a single launch (rotation-mode outputs only); `flags` adds
- * LFM_F_DAMP_PREV / LFM_F_FREEZE_CHI / LFM_F_CHI_DELTA to every lattice.
+ * LFM_F_DAMP_PREV / LFM_F_FREEZE_CHI / LFM_F_CHI_DELTA (and, COLOR only,
+ * LFM_F_SA_SCV: the `sa` arena then holds one SCV-term field per lattice at
+ * the lattice's scalar offset off_b) to every lattice.
  * COLOR lattices with kappa_tube > 0 read their raw S_a from the `sa` arena
  * and write |Psi'_a|^2 to `psisq_out` (both 3 fields per lattice, laid out as
  * the Psi arenas; the per-lattice S_a transforms run between steps, as

@@ -621,8 +621,10 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled_batch(BatchArenas A,
     P.mask = (kp.flags & LFM_F_MASK) ? A.mask + o : nullptr;
     P.pr_out = A.pr_out + oc; P.chi_out = A.chi_out + o;
     if (A.sa && kp.kt > 0.0f) {
-        // the S_a term of this lattice, and |Psi'|^2 for its next transform
-        P.sa = A.sa + oc;
+        // the S_a term of this lattice (three S_a fields, or under
+        // LFM_F_SA_SCV the SCV term, one field at the lattice's scalar
+        // offset), and |Psi'|^2 for its next transform
+        P.sa = A.sa + ((kp.flags & LFM_F_SA_SCV) ? o : oc);
         P.psisq = A.psisq ? A.psisq + oc : nullptr;
     }
     P.n3 = n3;
@@ -1732,11 +1734,16 @@ extern "C" int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_p
         set_error("lfm_batch_step_f32: lattices with kappa_tube > 0 need the S_a arena");
         return LFM_E_ARG;
     }
+    if ((flags & LFM_F_SA_SCV) && P->level != 2) {
+        set_error("lfm_batch_step_f32: LFM_F_SA_SCV is for COLOR batches");
+        return LFM_E_ARG;
+    }
     const size_t ns = size_t(P->total) * (P->level == 2 ? 3 : 1);
     const float* sa_in = P->scv ? sa : nullptr;
     float* sq_out = P->scv ? psisq_out : nullptr;
+    const size_t nsa = (flags & LFM_F_SA_SCV) ? size_t(P->total) : ns;
     const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask, sa_in};
-    const size_t nin[] = {ns, ns, ns, ns, size_t(P->total), size_t(P->total), size_t(P->total), ns};
+    const size_t nin[] = {ns, ns, ns, ns, size_t(P->total), size_t(P->total), size_t(P->total), nsa};
     float* outs[] = {pr_out, pi_out, chi_out, sq_out};
     const size_t nout[] = {ns, ns, size_t(P->total), ns};
     for (int a = 0; a < 4; ++a)
@@ -1747,7 +1754,7 @@ extern "C" int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_p
             }
     BatchArenas A{pr, pr_prev, P->level ? pi : nullptr, P->level ? pi_prev : nullptr, chi, chi_prev,
                   mask, pr_out, P->level ? pi_out : nullptr, chi_out, sa_in, sq_out};
-    const int fo = flags & (LFM_F_DAMP_PREV | LFM_F_FREEZE_CHI | LFM_F_CHI_DELTA);
+    const int fo = flags & (LFM_F_DAMP_PREV | LFM_F_FREEZE_CHI | LFM_F_CHI_DELTA | LFM_F_SA_SCV);
     hipStream_t st = (hipStream_t)stream;
     switch (P->level) {
         case 0: return launch_batch<REAL>(A, P->d_lat, P->d_desc, P->ndesc, P->tile, fo,

@@ -18,10 +18,12 @@ result bit.  One group (the default) measured fastest: the extra, smaller
 launches cost more than the cache saves.
 
 Lattices of one batch share a field level.  COLOR lattices with
-kappa_tube > 0 keep their S_a and |Psi'|^2 in two more arenas; before each
-step the S_a transform runs per lattice (one plan per lattice size, the
+kappa_tube > 0 keep their S_a input and |Psi'|^2 in two more arenas; before
+each step the S_a transform runs per lattice (one plan per lattice size, the
 |Psi'|^2 the previous step wrote as its input), then the one batched launch
-reads S_a and writes the next |Psi'|^2.
+reads the S_a input and writes the next |Psi'|^2.  With the hand-written
+transforms (power-of-two sizes) the S_a input is the SCV term the transform's
+last pass forms (one field per lattice), else the three S_a fields.
 """
 from __future__ import annotations
 
@@ -91,12 +93,18 @@ class LatticeBatch:
         self.scv = [self.level == 2 and c.kappa_tube > 0 for c in configs]
         self._sa = self._psisq = None
         self._scv_plans = {}
+        self.sa_scv = False
         if any(self.scv):
-            self._sa = torch.zeros(npsi, dtype=f32, device=dev)
             self._psisq = torch.empty(npsi, dtype=f32, device=dev)
             for b, on in enumerate(self.scv):
                 if on and self.sizes[b] not in self._scv_plans:
                     self._scv_plans[self.sizes[b]] = ScvPlan(self.sizes[b], 3)
+            # when every transform is the hand-written one, its last pass
+            # writes the step's SCV term (one field per lattice; the same
+            # bits as the three S_a fields, 16 B per site less traffic),
+            # else the three S_a fields travel between transform and step
+            self.sa_scv = all(p.method == 2 for p in self._scv_plans.values())
+            self._sa = torch.zeros(self.total if self.sa_scv else npsi, dtype=f32, device=dev)
         # per lattice: its |Psi'|^2 arena span matches its current Psi (it has
         # stepped since its Psi was last set), so S_a comes from it, as the
         # single-lattice Evolver does after its first step
@@ -280,6 +288,13 @@ class LatticeBatch:
             if not self.scv[b]:
                 continue
             c, plan = self.configs[b], self._scv_plans[self.sizes[b]]
+            if self.sa_scv:
+                e = self._view(self._psisq, b, True).data_ptr() if self._psisq_ok[b] else None
+                _abi.check(self._lib.lfm_scv_apply_scv_f32(
+                    plan.handle, self._view(self._pr[cur], b, True).data_ptr(),
+                    self._view(self._pi[cur], b, True).data_ptr(), e, c.sa_gamma, c.sa_d,
+                    self._view(self._sa, b).data_ptr(), stream_handle()), "lfm_scv_apply_scv_f32")
+                continue
             sa = self._view(self._sa, b, True)
             if self._psisq_ok[b]:
                 plan.apply_psisq(self._view(self._psisq, b, True), c.sa_gamma, c.sa_d, sa)
@@ -294,8 +309,10 @@ class LatticeBatch:
         o = int(self.offsets[self.groups[g][0]]) * 4
         op = o * self.ncomp
         pr, pi, chi = self._pr, self._pi, self._chi
-        sa = self._sa.data_ptr() + op if self._sa is not None else None
+        sa = self._sa.data_ptr() + (o if self.sa_scv else op) if self._sa is not None else None
         sq = self._psisq.data_ptr() + op if self._sa is not None else None
+        if self.sa_scv:
+            flags |= _abi.LFM_F_SA_SCV
         rc = self._lib.lfm_batch_step_f32(
             self._plans[g], pr[cur].data_ptr() + op, pr[prev].data_ptr() + op,
             pi[cur].data_ptr() + op if pi else None, pi[prev].data_ptr() + op if pi else None,
