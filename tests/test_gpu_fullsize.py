@@ -131,3 +131,49 @@ def test_two_step_equals_single_512(gpu):
         assert same, name
     del res, pr, pi, chi
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("n", [1023, 1024])
+def test_largest_sizes_real(gpu, n):
+    """The step kernels at the top of their size ranges (REAL gravity_only,
+    periodic, seeded random fields, 2 steps).  N = 1023 is the largest size the
+    tiled kernel's 32-bit byte offsets cover (ragged 64-wide tiles at the
+    edge): tiled against the untiled kernel within 1e-6 relative L2.  At
+    N = 1024 only the untiled kernel runs (64-bit indices): rolling the
+    initial fields by (5, 17, 33) and stepping gives the rolled result bit
+    for bit."""
+    import torch
+    import lfm_hip as L
+    log(f"largest sizes: REAL {n}^3")
+    cfg = L.gravity_only(n, boundary_type=L.BoundaryType.PERIODIC)
+    g = torch.Generator(device="cuda").manual_seed(n)
+    pr = torch.randn((n, n, n), generator=g, device="cuda")
+    chi = 19.0 + 0.1 * torch.randn((n, n, n), generator=g, device="cuda")
+    shift = (5, 17, 33)
+    runs = [(False, False), (True, False)] if n <= 1023 else [(False, False), (False, True)]
+    res = []
+    for direct, rolled in runs:
+        roll = (lambda t: torch.roll(t, shift, dims=(0, 1, 2))) if rolled else (lambda t: t)
+        ev = L.Evolver(cfg, backend="hip", mask_policy="boundary_type", graphs=False)
+        assert not ev.has_mask
+        ev.set_psi_real(roll(pr))
+        ev.set_chi(roll(chi))
+        ev._step(direct=direct)
+        ev._step(direct=direct)
+        torch.cuda.synchronize()
+        res.append(_state(ev))
+        del ev
+        torch.cuda.empty_cache()
+    shape = (n, n, n)
+    for a, b, name in zip(res[0], res[1], ("psi_r", "chi")):
+        assert bool(torch.isfinite(a).all()), name
+        if n <= 1023:
+            e = _rel_l2(b, a)
+            log(f"  {name}: direct vs tiled rel L2 {e:.2e}")
+            assert e <= 1e-6, (name, e)
+        else:
+            same = bool(torch.equal(torch.roll(a.view(shape), shift, dims=(0, 1, 2)), b.view(shape)))
+            log(f"  {name}: rolled-then-stepped equals stepped-then-rolled: {same}")
+            assert same, name
+    del res, pr, chi
+    torch.cuda.empty_cache()

@@ -63,6 +63,11 @@ def test_abi_version_and_error_string():
                                ctypes.byref(p), None)
     assert rc == -1
     assert b"out of range" in lib.lfm_last_error()
+    # and above the largest supported lattice (N = 1290, the largest N with N^3 < 2^31)
+    p.N = 1291
+    rc = lib.lfm_step_real_f32(*dummy, None, *[(2 << 20) + (q << 16) for q in range(4)],
+                               ctypes.byref(p), None)
+    assert rc == -1 and b"out of range" in lib.lfm_last_error()
 
 
 def test_abi_rejects_in_place_and_missing_inputs():
