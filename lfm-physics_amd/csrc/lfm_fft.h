@@ -118,15 +118,35 @@ __device__ __forceinline__ void stage_regs(float2 (&x)[8], int t, int Ns, int si
     }
 }
 
+// How a stage boundary orders its LDS writes and reads.
+//   Block: the line spans several wavefronts.  A workgroup barrier before the
+//          writes (every thread has read the previous boundary's values)
+//          and one between the writes and the reads.
+//   Wave:  the line lies in one wavefront (N/8 <= 64 threads per line).  A
+//          wavefront's LDS accesses complete in issue order, so only the
+//          compiler has to be kept from moving LDS accesses across the
+//          boundary (wave barrier, no instruction).
+//   Alt:   two LDS buffers taken in turn at successive boundaries.  The
+//          barrier between one boundary's writes and reads also orders every
+//          thread's reads of the other buffer (at the previous boundary)
+//          before the next boundary's writes to it: one barrier a boundary.
+enum class Sync { Block, Wave, Alt };
+
+template <Sync S>
+__device__ __forceinline__ void line_sync() {
+    if constexpr (S == Sync::Wave) __builtin_amdgcn_wave_barrier();
+    else __syncthreads();
+}
+
 // Stage boundary through LDS: results of the radix-R stage of span Ns go to
 // their Stockham positions (j/Ns)*Ns*R + j%Ns + r*Ns, then every thread
 // reads back positions t + e*TC.  `at(pos)` maps a line position to its LDS
-// slot.  The leading barrier protects the previous boundary's reads.
-template <int N, int R, class At>
+// slot.
+template <int N, int R, Sync S, class At>
 __device__ __forceinline__ void exchange(float2 (&x)[8], float2* lds, int t, int Ns, At at) {
     constexpr int TC = N / 8;
     constexpr int NB = 8 / R;
-    __syncthreads();
+    if constexpr (S != Sync::Alt) line_sync<S>();
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
         const int j = t + u * TC;
@@ -135,15 +155,18 @@ __device__ __forceinline__ void exchange(float2 (&x)[8], float2* lds, int t, int
 #pragma unroll
         for (int r = 0; r < R; ++r) lds[at(base + r * Ns)] = x[u + r * NB];
     }
-    __syncthreads();
+    line_sync<S>();
 #pragma unroll
     for (int e = 0; e < 8; ++e) x[e] = lds[at(t + e * TC)];
 }
 
 // Full length-N transform of the register line; result in the same slots.
-template <int N, class At>
+// With Sync::Alt, boundaries alternate between `lds` and `lds_alt`, and
+// `*par` counts the boundaries the workgroup has passed (kept across calls).
+template <int N, Sync S = Sync::Block, class At>
 __device__ __forceinline__ void fft_line(float2 (&x)[8], float2* lds, int t, int sign,
-                                         const float2* __restrict__ twl, At at) {
+                                         const float2* __restrict__ twl, At at,
+                                         float2* lds_alt = nullptr, int* par = nullptr) {
     constexpr int L = ilog2(N);
     constexpr int n8 = L / 3;
     constexpr int rem = L % 3;
@@ -151,7 +174,14 @@ __device__ __forceinline__ void fft_line(float2 (&x)[8], float2* lds, int t, int
 #pragma unroll
     for (int s = 0; s < n8; ++s) {
         stage_regs<N, 8>(x, t, Ns, sign, twl);
-        if (s + 1 < n8 || rem != 0) exchange<N, 8>(x, lds, t, Ns, at);
+        if (s + 1 < n8 || rem != 0) {
+            if constexpr (S == Sync::Alt) {
+                exchange<N, 8, S>(x, (*par & 1) ? lds_alt : lds, t, Ns, at);
+                ++*par;
+            } else {
+                exchange<N, 8, S>(x, lds, t, Ns, at);
+            }
+        }
         Ns *= 8;
     }
     if constexpr (rem == 1) stage_regs<N, 2>(x, t, Ns, sign, twl);

@@ -232,8 +232,9 @@ __global__ __launch_bounds__(ncol_for<N>() * (N / 8)) void fft_columns(ColArgs A
     constexpr int NCOL = ncol_for<N>();
     constexpr int TC = N / 8;
     constexpr int H = N / 2 + 1;
-    __shared__ float2 lds[N * NCOL];
+    __shared__ float2 lds[2][N * NCOL];             // stage boundaries alternate
     __shared__ float2 twl[N];
+    int par = 0;                                    // boundaries passed
     constexpr int NCB = (H + NCOL - 1) / NCOL;      // column groups per slab
     const int nbs = (NCB + A.rounds - 1) / A.rounds;  // workgroups per slab
     const int cb0 = (blockIdx.x % nbs) * A.rounds;
@@ -262,12 +263,12 @@ __global__ __launch_bounds__(ncol_for<N>() * (N / 8)) void fft_columns(ColArgs A
     // together, and a wait issued while stores are pending drains them too.
     auto transform_store = [&](int cb, float2 (*pending)[8]) {
         const int kk = cb * NCOL + col;
-        // the first stage needs no twiddles; the first boundary's barriers
-        // publish the table before any stage reads it
-        fft_line<N>(v, lds, t, MODE == 1 ? 1 : -1, twl, at);
+        // the first stage needs no twiddles; the first boundary's barrier
+        // publishes the table before any stage reads it
+        fft_line<N, Sync::Alt>(v, lds[0], t, MODE == 1 ? 1 : -1, twl, at, lds[1], &par);
         if constexpr (MODE == 2) {
             filter_line<N, FILTER>(v, t, kk, ky, A);
-            fft_line<N>(v, lds, t, 1, twl, at);
+            fft_line<N, Sync::Alt>(v, lds[0], t, 1, twl, at, lds[1], &par);
         }
         if (pending) {
 #pragma unroll
@@ -307,6 +308,10 @@ template <int N>
 constexpr int rows_pairs() { return rows_threads<N>() / (N / 8); }
 template <int N>
 constexpr int rows_pitch() { return N + N / 8; }
+// Rows of N <= 512 lie in one wavefront each (N/8 <= 64 threads), so their
+// stage boundaries need no workgroup barrier.
+template <int N>
+constexpr Sync rows_sync() { return N / 8 <= 64 ? Sync::Wave : Sync::Block; }
 
 template <int N>
 __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_r2c(const float* __restrict__ in,
@@ -329,11 +334,15 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_r2c(const float* _
         v[e] = make_float2(ld1nt<N>(r0 + t + e * TC), ld1nt<N>(r1 + t + e * TC));
     float2* L = lds + q * PS;
     auto at = [](int pos) { return pos + pos / 8; };
-    fft_line<N>(v, L, t, -1, twl, at);
-    __syncthreads();
+    constexpr Sync S = rows_sync<N>();
+    // with wavefront-level boundaries the twiddle table, which the whole
+    // workgroup fills, is published here
+    if constexpr (S == Sync::Wave) __syncthreads();
+    fft_line<N, S>(v, L, t, -1, twl, at);
+    line_sync<S>();
 #pragma unroll
     for (int e = 0; e < 8; ++e) L[at(t + e * TC)] = v[e];
-    __syncthreads();
+    line_sync<S>();
     float2* o0 = spec + pair * 2 * Hp;
     float2* o1 = o0 + Hp;
 #pragma unroll
@@ -387,9 +396,8 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_c2r(const float2* 
         if (!lo) { a.y = -a.y; b.y = -b.y; }
         v[e] = make_float2(a.x - b.y, a.y + b.x);
     }
-    // the transform's first exchange starts with a barrier, which orders
-    // these reads before its writes
-    fft_line<N>(v, L, t, 1, twl, [](int pos) { return pos + pos / 8; });
+    // the transform's first boundary orders these reads before its writes
+    fft_line<N, rows_sync<N>()>(v, L, t, 1, twl, [](int pos) { return pos + pos / 8; });
     float* r0 = out + pair * 2 * N;
     float* r1 = r0 + N;
 #pragma unroll
@@ -451,7 +459,7 @@ __global__ __launch_bounds__(scv_threads<N>()) void fft_rows_c2r_scv(
         if (!lo) { x.y = -x.y; y.y = -y.y; }
         v[e] = make_float2(x.x - y.y, x.y + y.x);
     }
-    fft_line<N>(v, L, t, 1, twl, [](int pos) { return pos + pos / 8; });
+    fft_line<N, rows_sync<N>()>(v, L, t, 1, twl, [](int pos) { return pos + pos / 8; });
     // v[e] = (S_a at row 2 pair, S_a at row 2 pair + 1), position t + e TC.
     // The transform's last LDS reads precede this barrier; colours 1 and 2
     // then park their results in their own slots for colour 0.
