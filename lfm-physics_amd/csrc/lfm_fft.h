@@ -1,7 +1,7 @@
 // Device-side building blocks of the length-N line FFTs (Stockham autosort,
-// radix-8 stages plus one radix-2/4 stage; thread t of a line's N/8 threads
-// holds positions t + e*N/8), shared by the transform passes and the fused
-// COLOR step.
+// radix-8 stages plus one radix-2/4 stage; thread t of a line's TC = N/V
+// threads holds the V positions t + e*TC, V = 8 or 16), shared by the
+// transform passes and the fused COLOR step.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -90,14 +90,14 @@ __device__ __forceinline__ void dft(float2 (&x)[R], int sign) {
     }
 }
 
-// One radix-R stage (span Ns) on the 8 register values of thread t.
-// Butterfly u (of NB = 8/R) takes positions j + r*(N/R), j = t + u*TC,
+// One radix-R stage (span Ns) on the V register values of thread t.
+// Butterfly u (of NB = V/R) takes positions j + r*(N/R), j = t + u*TC,
 // which are the register slots u + r*NB.
-template <int N, int R>
-__device__ __forceinline__ void stage_regs(float2 (&x)[8], int t, int Ns, int sign,
+template <int N, int R, int V = 8>
+__device__ __forceinline__ void stage_regs(float2 (&x)[V], int t, int Ns, int sign,
                                            const float2* __restrict__ twl) {
-    constexpr int TC = N / 8;
-    constexpr int NB = 8 / R;
+    constexpr int TC = N / V;
+    constexpr int NB = V / R;
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
         const int k = (t + u * TC) & (Ns - 1);
@@ -142,10 +142,10 @@ __device__ __forceinline__ void line_sync() {
 // their Stockham positions (j/Ns)*Ns*R + j%Ns + r*Ns, then every thread
 // reads back positions t + e*TC.  `at(pos)` maps a line position to its LDS
 // slot.
-template <int N, int R, Sync S, class At>
-__device__ __forceinline__ void exchange(float2 (&x)[8], float2* lds, int t, int Ns, At at) {
-    constexpr int TC = N / 8;
-    constexpr int NB = 8 / R;
+template <int N, int R, Sync S, int V = 8, class At>
+__device__ __forceinline__ void exchange(float2 (&x)[V], float2* lds, int t, int Ns, At at) {
+    constexpr int TC = N / V;
+    constexpr int NB = V / R;
     if constexpr (S != Sync::Alt) line_sync<S>();
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
@@ -157,14 +157,14 @@ __device__ __forceinline__ void exchange(float2 (&x)[8], float2* lds, int t, int
     }
     line_sync<S>();
 #pragma unroll
-    for (int e = 0; e < 8; ++e) x[e] = lds[at(t + e * TC)];
+    for (int e = 0; e < V; ++e) x[e] = lds[at(t + e * TC)];
 }
 
 // Full length-N transform of the register line; result in the same slots.
 // With Sync::Alt, boundaries alternate between `lds` and `lds_alt`, and
 // `*par` counts the boundaries the workgroup has passed (kept across calls).
-template <int N, Sync S = Sync::Block, class At>
-__device__ __forceinline__ void fft_line(float2 (&x)[8], float2* lds, int t, int sign,
+template <int N, Sync S = Sync::Block, int V = 8, class At>
+__device__ __forceinline__ void fft_line(float2 (&x)[V], float2* lds, int t, int sign,
                                          const float2* __restrict__ twl, At at,
                                          float2* lds_alt = nullptr, int* par = nullptr) {
     constexpr int L = ilog2(N);
@@ -173,19 +173,19 @@ __device__ __forceinline__ void fft_line(float2 (&x)[8], float2* lds, int t, int
     int Ns = 1;
 #pragma unroll
     for (int s = 0; s < n8; ++s) {
-        stage_regs<N, 8>(x, t, Ns, sign, twl);
+        stage_regs<N, 8, V>(x, t, Ns, sign, twl);
         if (s + 1 < n8 || rem != 0) {
             if constexpr (S == Sync::Alt) {
-                exchange<N, 8, S>(x, (*par & 1) ? lds_alt : lds, t, Ns, at);
+                exchange<N, 8, S, V>(x, (*par & 1) ? lds_alt : lds, t, Ns, at);
                 ++*par;
             } else {
-                exchange<N, 8, S>(x, lds, t, Ns, at);
+                exchange<N, 8, S, V>(x, lds, t, Ns, at);
             }
         }
         Ns *= 8;
     }
-    if constexpr (rem == 1) stage_regs<N, 2>(x, t, Ns, sign, twl);
-    if constexpr (rem == 2) stage_regs<N, 4>(x, t, Ns, sign, twl);
+    if constexpr (rem == 1) stage_regs<N, 2, V>(x, t, Ns, sign, twl);
+    if constexpr (rem == 2) stage_regs<N, 4, V>(x, t, Ns, sign, twl);
 }
 
 template <int N>

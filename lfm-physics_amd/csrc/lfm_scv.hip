@@ -150,11 +150,12 @@ __global__ __launch_bounds__(256) void poisson_finish(float* __restrict__ chi, i
 
 // ---------------------------------------------------------------------------
 // Transform passes of the direct path (method 2): a length-N complex FFT per
-// line, every thread holding 8 values of its line.
+// line, every thread holding V = 8 values of its line (16 in the column
+// passes at N = 1024).
 //
 // Stockham autosort with radix-8 stages and one radix-2/4 stage when log2 N
-// is not a multiple of 3.  Thread t of the TC = N/8 threads of a line holds
-// the values at positions t + e*TC, e = 0..7.  With that assignment every
+// is not a multiple of 3.  Thread t of the TC = N/V threads of a line holds
+// the values at positions t + e*TC, e = 0..V-1.  With that assignment every
 // stage reads exactly those positions, and the last stage (Ns * R = N) writes
 // exactly those positions, so the first stage starts from the values the
 // thread loaded from HBM, the last stage's results go straight back to HBM
@@ -185,24 +186,32 @@ struct ColArgs {
     int rounds;              // column groups of NCOL one workgroup takes in turn
 };
 
+// Column passes: 16 adjacent kk columns per workgroup (128-byte row segments)
+// and V values per thread, so a workgroup has 16 N / V <= 1024 lanes: V = 8
+// up to N = 512, 16 at N = 1024.  With V = 8 the stage boundaries alternate
+// between two LDS buffers; at N = 1024 one buffer (128 KiB) is all that fits.
 template <int N>
-constexpr int ncol_for() { return N >= 1024 ? 8 : 16; }
+constexpr int ncol_for() { return 16; }
+template <int N>
+constexpr int col_vals() { return N >= 1024 ? 16 : 8; }
+template <int N>
+constexpr Sync col_sync() { return N >= 1024 ? Sync::Block : Sync::Alt; }
 
-// The i-axis filter on a line held as positions t + e*N/8: multiply by
+// The i-axis filter on a line held as positions t + e*N/V: multiply by
 // gamma/(gamma + D k^2) (FILTER 0) or -1/k^2 (FILTER 1, zero at k = 0), times
 // the 1/N^3 normalisation, at (k_i = position, k_j = ky, k_kk = kk).  The
 // quotient is the hardware reciprocal refined by one Newton step (within
 // 1 ulp of the correctly rounded one).
-template <int N, int FILTER>
-__device__ __forceinline__ void filter_line(float2 (&v)[8], int t, int kk, float ky,
+template <int N, int FILTER, int V>
+__device__ __forceinline__ void filter_line(float2 (&v)[V], int t, int kk, float ky,
                                             const ColArgs& A) {
-    constexpr int TC = N / 8;
+    constexpr int TC = N / V;
     const float two_pi_over_n = 6.283185307179586f / float(N);
     const float kz = two_pi_over_n * float(kk);
     const float k2jk = ky * ky + kz * kz;
     const float num = FILTER == 0 ? A.gamma * A.scale : -A.scale;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
+    for (int e = 0; e < V; ++e) {
         const int i = t + e * TC;
         const int fi = i < (N + 1) / 2 ? i : i - N;
         const float kx = two_pi_over_n * float(fi);
@@ -220,21 +229,26 @@ __device__ __forceinline__ void filter_line(float2 (&v)[8], int t, int kk, float
 // Workgroup = NCOL adjacent kk columns x TC threads per column; lane order
 // is column-fastest so each load/store instruction moves whole 128-byte rows
 // and LDS rows ([pos][col]) are read and written contiguously.  A workgroup
-// takes `rounds` consecutive column groups of one slab in turn (the next
-// group's loads are issued before the current group is transformed), so
-// the N rows of a slab, which for the i axis lie 8 N Hp bytes apart, are
-// opened once per `rounds` groups.
+// takes `rounds` consecutive column groups of one slab in turn, so the N rows
+// of a slab, which for the i axis lie 8 N Hp bytes apart, are opened once per
+// `rounds` groups.  With V = 8 the next group's loads are issued before the
+// current group is transformed; with V = 16 the next group's 32 registers do
+// not fit beside the transform's in 128 VGPRs (1024 lanes per CU), so the
+// groups are taken one after the other.
 // MODE (0 forward, 1 inverse, 2 forward + filter + inverse) and FILTER are
 // template parameters so the butterflies' rotations and twiddle conjugates
 // fold at compile time.
 template <int N, int MODE, int FILTER>
-__global__ __launch_bounds__(ncol_for<N>() * (N / 8)) void fft_columns(ColArgs A) {
+__global__ __launch_bounds__(ncol_for<N>() * (N / col_vals<N>())) void fft_columns(ColArgs A) {
     constexpr int NCOL = ncol_for<N>();
-    constexpr int TC = N / 8;
+    constexpr int V = col_vals<N>();
+    constexpr int TC = N / V;
     constexpr int H = N / 2 + 1;
-    __shared__ float2 lds[2][N * NCOL];             // stage boundaries alternate
+    constexpr Sync S = col_sync<N>();
+    constexpr int NBUF = S == Sync::Alt ? 2 : 1;
+    __shared__ float2 lds[NBUF][N * NCOL];          // Alt: stage boundaries alternate
     __shared__ float2 twl[N];
-    int par = 0;                                    // boundaries passed
+    int par = 0;                                    // boundaries passed (Alt)
     constexpr int NCB = (H + NCOL - 1) / NCOL;      // column groups per slab
     const int nbs = (NCB + A.rounds - 1) / A.rounds;  // workgroups per slab
     const int cb0 = (blockIdx.x % nbs) * A.rounds;
@@ -248,50 +262,57 @@ __global__ __launch_bounds__(ncol_for<N>() * (N / 8)) void fft_columns(ColArgs A
     const float two_pi_over_n = 6.283185307179586f / float(N);
     const int fj = x < (N + 1) / 2 ? x : x - N;
     const float ky = two_pi_over_n * float(fj);
-    float2 v[8], vn[8];
+    float2 v[V], vn[V];
     // Loads carry no predicate: lanes past the last live column read column
     // H - 1, a line the live lanes fetch anyway.  Branch-free loads let the
     // compiler wait only for the current group's values, so the next
     // group's loads stay in flight while this one is transformed.
-    auto load = [&](float2 (&dst)[8], int cb) {
+    auto load = [&](float2 (&dst)[V], int cb) {
         const float2* b = sbase + min(cb * NCOL + col, H - 1);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) dst[e] = ld2nt<N>(b + (long long)(t + e * TC) * A.row_stride);
+        for (int e = 0; e < V; ++e) dst[e] = ld2nt<N>(b + (long long)(t + e * TC) * A.row_stride);
     };
     // `pending` (the prefetched group, or none) is waited for before this
     // group's stores issue: the wait counter covers loads and stores
     // together, and a wait issued while stores are pending drains them too.
-    auto transform_store = [&](int cb, float2 (*pending)[8]) {
+    auto transform_store = [&](int cb, float2 (*pending)[V]) {
         const int kk = cb * NCOL + col;
         // the first stage needs no twiddles; the first boundary's barrier
         // publishes the table before any stage reads it
-        fft_line<N, Sync::Alt>(v, lds[0], t, MODE == 1 ? 1 : -1, twl, at, lds[1], &par);
+        fft_line<N, S, V>(v, lds[0], t, MODE == 1 ? 1 : -1, twl, at, lds[NBUF - 1], &par);
         if constexpr (MODE == 2) {
-            filter_line<N, FILTER>(v, t, kk, ky, A);
-            fft_line<N, Sync::Alt>(v, lds[0], t, 1, twl, at, lds[1], &par);
+            filter_line<N, FILTER, V>(v, t, kk, ky, A);
+            fft_line<N, S, V>(v, lds[0], t, 1, twl, at, lds[NBUF - 1], &par);
         }
         if (pending) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) asm volatile("" ::"v"((*pending)[e].x), "v"((*pending)[e].y));
+            for (int e = 0; e < V; ++e) asm volatile("" ::"v"((*pending)[e].x), "v"((*pending)[e].y));
         }
         if (kk < H) {
             float2* b = sbase + kk;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) st2nt<N>(b + (long long)(t + e * TC) * A.row_stride, v[e]);
+            for (int e = 0; e < V; ++e) st2nt<N>(b + (long long)(t + e * TC) * A.row_stride, v[e]);
         }
     };
-    load(v, cb0);
-    // wait for the first group here, so no load is pending where the loop
-    // is entered and the loop's waits need not cover earlier stores
+    if constexpr (V == 8) {
+        load(v, cb0);
+        // wait for the first group here, so no load is pending where the loop
+        // is entered and the loop's waits need not cover earlier stores
 #pragma unroll
-    for (int e = 0; e < 8; ++e) asm volatile("" ::"v"(v[e].x), "v"(v[e].y));
-    for (int r = 0; r + 1 < ng; ++r) {
-        load(vn, cb0 + r + 1);
-        transform_store(cb0 + r, &vn);
+        for (int e = 0; e < V; ++e) asm volatile("" ::"v"(v[e].x), "v"(v[e].y));
+        for (int r = 0; r + 1 < ng; ++r) {
+            load(vn, cb0 + r + 1);
+            transform_store(cb0 + r, &vn);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = vn[e];
+            for (int e = 0; e < V; ++e) v[e] = vn[e];
+        }
+        transform_store(cb0 + ng - 1, nullptr);
+    } else {
+        for (int r = 0; r < ng; ++r) {
+            load(v, cb0 + r);
+            transform_store(cb0 + r, nullptr);
+        }
     }
-    transform_store(cb0 + ng - 1, nullptr);
 }
 
 // Lines along the contiguous k axis.  Two real lines p0 = 2q, p1 = 2q+1 form
@@ -499,7 +520,7 @@ int launch_columns(const ColArgs& A, hipStream_t st) {
     constexpr int H = N / 2 + 1;
     constexpr int NCB = (H + NCOL - 1) / NCOL;
     const long long nb = (long long)A.nc * N * ((NCB + A.rounds - 1) / A.rounds);
-    const dim3 g{unsigned(nb)}, b{unsigned(NCOL * (N / 8))};
+    const dim3 g{unsigned(nb)}, b{unsigned(NCOL * (N / col_vals<N>()))};
     if (A.mode == 0) hipLaunchKernelGGL((fft_columns<N, 0, 0>), g, b, 0, st, A);
     else if (A.mode == 1) hipLaunchKernelGGL((fft_columns<N, 1, 0>), g, b, 0, st, A);
     else if (A.filter == 0) hipLaunchKernelGGL((fft_columns<N, 2, 0>), g, b, 0, st, A);
@@ -585,15 +606,17 @@ void destroy(ScvPlan* p) {
     delete p;
 }
 
-// Column groups per workgroup: along j 4 from N = 512 on, 2 below; along i
-// a workgroup takes its whole slab (N rows of N/2+1 columns), which opens
-// each row of the slab once.  Measured (three passes), j/i rounds:
+// Column groups per workgroup: along j 1 at N = 1024, 4 at N = 512, 2 below;
+// along i a workgroup takes its whole slab (N rows of N/2+1 columns), which
+// opens each row of the slab once.  Measured (three passes), j/i rounds:
+// 1024^3 (16 values per thread): 4/whole 20.5 ms, 1/1 20.3, 1/4 20.1, 1/8 19.9,
+// 1/whole 19.4;
 // 512^3: 2/4 2.10 ms, 4/4 2.08, 2/whole 2.05, 4/whole 2.04, 6/whole 2.09;
 // 256^3: 4/whole 0.218, 2/whole 0.212, 2/4 0.221;
 // 128^3: 4/whole 0.045, 2/whole 0.037, 2/4 0.039.
-int rounds_j(int N) { return N >= 512 ? 4 : 2; }
+int rounds_j(int N) { return N >= 1024 ? 1 : N >= 512 ? 4 : 2; }
 int whole_slab_rounds(int N) {
-    const int ncol = N >= 1024 ? 8 : 16;   // ncol_for<N>()
+    const int ncol = 16;   // ncol_for<N>()
     return (N / 2 + 1 + ncol - 1) / ncol;
 }
 

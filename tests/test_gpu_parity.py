@@ -470,7 +470,8 @@ def test_sa_transform_methods_agree_with_oracle(gpu, N):
 def test_sa_direct_path_matches_rocfft_at_large_n(gpu, N, nc):
     """At the production sizes the direct passes agree with rocFFT 3-D plans
     (no FP64 oracle at this size: the small-N test pins both to it).  1024
-    runs the 8-column groups of the column passes."""
+    runs the column passes with 16 values per thread (the only size that
+    does)."""
     import torch
     from lfm_hip.backend import ScvPlan
     log(f"S_a direct vs rocFFT N={N} colours={nc}")
