@@ -568,7 +568,7 @@ def main():
     elif not args.direct:
         ev.tile, ev.chunk = int(args.tile), args.chunk
     # REAL / COMPLEX advance two steps per launch (as Evolver.evolve does)
-    pairs = getattr(ev, "two_step", False) and not args.direct and N <= 1023
+    pairs = getattr(ev, "two_step", False) and not args.direct and N <= 1024
     if args.direct:
         step = lambda: ev._step(direct=True)
     else:

@@ -104,7 +104,7 @@ int lfm_step_complex_f32(const float* pr, const float* pr_prev, const float* pi,
  * them; the second reads Psi_n damped).  The next pair then takes
  * (out2, out1) as (current, prev) with LFM_F_DAMP_PREV.  Outputs must not
  * alias any input (four Psi and four chi buffers in a rotation).
- * LFM_F_FREEZE_CHI and LFM_F_DIRECT are not supported; N <= 1023. */
+ * LFM_F_FREEZE_CHI and LFM_F_DIRECT are not supported; N <= 1024. */
 int lfm_step2_real_f32(const float* E, const float* E_prev, const float* chi,
                        const float* chi_prev, const float* mask, float* E_out1,
                        float* chi_out1, float* E_out2, float* chi_out2,

@@ -94,7 +94,8 @@ __device__ __forceinline__ float* field_out(float* pr, float* pi, int f, size_t 
 // Site addressing.  Off32 is a byte offset shared by every field (all fields
 // have the same layout), so each access is one uniform 64-bit base held in
 // SGPRs plus one 32-bit lane offset (the saddr form of global loads); it
-// covers N <= 1023.  Off64 is a plain element index for any N.
+// covers N <= 1024 (the largest offset, 4 N^3 - 4, is below 2^32).  Off64 is
+// a plain element index for any N.
 // With NT the pointwise inputs (ldp) and every output (st) are streaming
 // (non-temporal) accesses: on lattices far larger than the last-level cache
 // they stop data touched once per step from pushing the stencil fields' halo
@@ -668,7 +669,7 @@ bool streaming(size_t sites) {
 
 template <int L>
 int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
-    if ((kp.flags & LFM_F_DIRECT) || kp.N > 1023) {
+    if ((kp.flags & LFM_F_DIRECT) || kp.N > 1024) {
         const dim3 grid(unsigned((kp.N + 63) / 64), unsigned((kp.N + 3) / 4), unsigned(kp.N));
         hipLaunchKernelGGL((step_direct<L>), grid, dim3(64, 4), 0, st, P, kp);
         LFM_LAUNCH_CHECK("step_direct");
@@ -1462,8 +1463,8 @@ static int step2_check(const lfm_params_t* p, const char* fn) {
         set_error("%s: LFM_F_DIRECT / LFM_F_FREEZE_CHI not supported", fn);
         return LFM_E_ARG;
     }
-    if (p->N > 1023) {
-        set_error("%s: N=%d above 1023", fn, p->N);
+    if (p->N > 1024) {
+        set_error("%s: N=%d above 1024", fn, p->N);
         return LFM_E_ARG;
     }
     return LFM_OK;

@@ -193,7 +193,7 @@ class Evolver:
         results = {}
         old = (self.tile, self.chunk, self.kernel_events)
         N = self.N
-        pairs = self.two_step and N <= 1023
+        pairs = self.two_step and N <= 1024
         if self._fused:
             # the fused COLOR kernel has one tile shape (a full k-row); tune its chunk
             combos = [(0, ch) for ch in dict.fromkeys((0, N, N // 2, N // 4, N // 8)) if ch == 0 or ch >= 4]
@@ -421,7 +421,7 @@ class Evolver:
             return None
         if self.level == 2 and self.sa_enabled and self._scv.method != 2:
             return None              # rocFFT launches stay out of graphs
-        pair = self.two_step and self.N <= 1023 and hook is None
+        pair = self.two_step and self.N <= 1024 and hook is None
         key0 = self._key()
         k, m = key0, 0
         while True:
@@ -478,7 +478,7 @@ class Evolver:
         report = self.config.report_interval
         left = int(steps)
         hook = device_hook
-        pair_ok = self.two_step and not freeze_chi and self.N <= 1023 and hook is None
+        pair_ok = self.two_step and not freeze_chi and self.N <= 1024 and hook is None
         if hook is not None:
             if freeze_chi:
                 raise ValueError("device_hook with freeze_chi is not supported")

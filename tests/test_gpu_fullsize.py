@@ -9,7 +9,12 @@ need the CPU oracle to run at that size.
   for COLOR, the S_a transform on: relative L2 <= 1e-6 after 2 steps (the two
   kernels sum the Laplacian in different orders).
 * The two-step kernel against single steps at 512^3 COMPLEX: bit-identical.
+* The top of the size ranges: REAL at 1023^3 and 1024^3 (tiled against
+  untiled, and translation equivariance of the tiled kernel at 1024), 1025^3
+  on the untiled kernel; the two-step kernel and the COLOR step with S_a at
+  1024^3.
 """
+import gc
 import sys
 
 import numpy as np
@@ -133,15 +138,16 @@ def test_two_step_equals_single_512(gpu):
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("n", [1023, 1024])
+@pytest.mark.parametrize("n", [1023, 1024, 1025])
 def test_largest_sizes_real(gpu, n):
     """The step kernels at the top of their size ranges (REAL gravity_only,
-    periodic, seeded random fields, 2 steps).  N = 1023 is the largest size the
-    tiled kernel's 32-bit byte offsets cover (ragged 64-wide tiles at the
-    edge): tiled against the untiled kernel within 1e-6 relative L2.  At
-    N = 1024 only the untiled kernel runs (64-bit indices): rolling the
-    initial fields by (5, 17, 33) and stepping gives the rolled result bit
-    for bit."""
+    periodic, seeded random fields, 2 steps).  The tiled kernel's 32-bit byte
+    offsets cover N <= 1024 (the largest, 4 N^3 - 4, is 2^32 - 4 at 1024):
+    at N = 1023 (ragged 64-wide tiles at the edge) and 1024 it agrees with the
+    untiled kernel within 1e-6 relative L2, and at 1024 rolling the initial
+    fields by (5, 17, 33) and stepping gives the rolled result bit for bit.
+    N = 1025 runs the untiled kernel (64-bit indices): rolled-then-stepped
+    equals stepped-then-rolled bit for bit."""
     import torch
     import lfm_hip as L
     log(f"largest sizes: REAL {n}^3")
@@ -150,7 +156,11 @@ def test_largest_sizes_real(gpu, n):
     pr = torch.randn((n, n, n), generator=g, device="cuda")
     chi = 19.0 + 0.1 * torch.randn((n, n, n), generator=g, device="cuda")
     shift = (5, 17, 33)
-    runs = [(False, False), (True, False)] if n <= 1023 else [(False, False), (False, True)]
+    tiled = n <= 1024
+    # (direct kernel, rolled initial fields)
+    runs = [(False, False), (True, False)] if tiled else [(True, False)]
+    if n >= 1024:
+        runs.append((not tiled, True))
     res = []
     for direct, rolled in runs:
         roll = (lambda t: torch.roll(t, shift, dims=(0, 1, 2))) if rolled else (lambda t: t)
@@ -163,17 +173,88 @@ def test_largest_sizes_real(gpu, n):
         torch.cuda.synchronize()
         res.append(_state(ev))
         del ev
+        gc.collect()   # the Evolver's transform plan frees its device memory
         torch.cuda.empty_cache()
     shape = (n, n, n)
-    for a, b, name in zip(res[0], res[1], ("psi_r", "chi")):
+    for q, name in enumerate(("psi_r", "chi")):
+        a = res[0][q]
         assert bool(torch.isfinite(a).all()), name
-        if n <= 1023:
-            e = _rel_l2(b, a)
+        if tiled:
+            e = _rel_l2(res[1][q], a)
             log(f"  {name}: direct vs tiled rel L2 {e:.2e}")
             assert e <= 1e-6, (name, e)
-        else:
-            same = bool(torch.equal(torch.roll(a.view(shape), shift, dims=(0, 1, 2)), b.view(shape)))
-            log(f"  {name}: rolled-then-stepped equals stepped-then-rolled: {same}")
+        if n >= 1024:
+            same = bool(torch.equal(torch.roll(a.view(shape), shift, dims=(0, 1, 2)),
+                                    res[-1][q].view(shape)))
+            log(f"  {name}: rolled-then-stepped equals stepped-then-rolled "
+                f"({'tiled' if tiled else 'untiled'}): {same}")
             assert same, name
     del res, pr, chi
+    torch.cuda.empty_cache()
+
+
+def test_two_step_equals_single_1024(gpu):
+    """The two-step kernel at N = 1024 (COMPLEX gravity_em, mask on, 5 steps:
+    two pairs and one single step) against single steps: bit-identical."""
+    import torch
+    import lfm_hip as L
+    n = 1024
+    log(f"two-step vs single {n}^3 COMPLEX")
+    cfg = L.gravity_em(n)
+    g = torch.Generator(device="cuda").manual_seed(17)
+    pr = torch.randn((n, n, n), generator=g, device="cuda")
+    pi = torch.randn((n, n, n), generator=g, device="cuda")
+    chi = 19.0 + 0.1 * torch.randn((n, n, n), generator=g, device="cuda")
+    res = []
+    for two in (True, False):
+        ev = L.Evolver(cfg, backend="hip", two_step=two)
+        ev.set_psi_real(pr)
+        ev.set_psi_imag(pi)
+        ev.set_chi(chi)
+        ev.evolve(5)
+        torch.cuda.synchronize()
+        res.append(_state(ev))
+        del ev
+        gc.collect()   # the Evolver's transform plan frees its device memory
+        torch.cuda.empty_cache()
+    for a, b, name in zip(res[0], res[1], ("psi_r", "chi", "psi_i")):
+        same = bool(torch.equal(a, b))
+        log(f"  {name}: bit-identical {same}")
+        assert same, name
+    del res, pr, pi, chi
+    torch.cuda.empty_cache()
+
+
+def test_color_tiled_vs_direct_1024(gpu):
+    """COLOR full_physics at 1024^3 (mask on, S_a through the five transform
+    passes, whose column passes run 16 values per thread only at this size):
+    the tiled step against the untiled one, relative L2 <= 1e-6 after 2
+    steps."""
+    import torch
+    import lfm_hip as L
+    n = 1024
+    log(f"tiled vs direct {n}^3 COLOR")
+    cfg = L.full_physics(n)
+    g = torch.Generator(device="cuda").manual_seed(19)
+    pr = 0.5 * torch.randn((3, n, n, n), generator=g, device="cuda")
+    pi = 0.5 * torch.randn((3, n, n, n), generator=g, device="cuda")
+    chi = 19.0 + 0.1 * torch.randn((n, n, n), generator=g, device="cuda")
+    res = []
+    for direct in (False, True):
+        ev = L.Evolver(cfg, backend="hip")
+        ev.set_psi_real(pr)
+        ev.set_psi_imag(pi)
+        ev.set_chi(chi)
+        ev._step(direct=direct)
+        ev._step(direct=direct)
+        torch.cuda.synchronize()
+        res.append(_state(ev))
+        del ev
+        gc.collect()   # the Evolver's transform plan frees its device memory
+        torch.cuda.empty_cache()
+    for a, b, name in zip(res[0], res[1], ("psi_r", "chi", "psi_i")):
+        e = _rel_l2(a, b)
+        log(f"  {name}: rel L2 {e:.2e}")
+        assert bool(torch.isfinite(a).all()) and e <= 1e-6, (name, e)
+    del res, pr, pi, chi
     torch.cuda.empty_cache()
