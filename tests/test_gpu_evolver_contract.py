@@ -1,4 +1,4 @@
-"""The reference's Evolver and S_a contract tests, restated on the HIP path.
+"""The reference's Evolver, backend and S_a contract tests, restated on the HIP path.
 
 Each case follows one of the reference's own tests (cited per test) with the
 same configuration and the same assertion, through ``lfm_hip.Evolver`` /
@@ -148,3 +148,72 @@ def test_sa_fields_grow_and_survive_checkpoint(gpu, tmp_path):
     sa_after = L.Simulation.load_checkpoint(str(path), backend="hip").sa_fields
     assert sa_after is not None
     np.testing.assert_allclose(sa_after, sa, atol=1e-6)
+
+
+# --- tests/test_backends.py:27-290 (the backend object itself) --------------
+
+def test_backend_mask_and_conversions(gpu):
+    import lfm_hip as L
+    b = L.HipBackend()
+    m = b.create_boundary_mask(N, 0.15)
+    h = b.to_numpy(m)
+    assert h.shape == (N ** 3,) and h.dtype == np.float32
+    c = N // 2
+    assert h[c * N * N + c * N + c] == 0.0, "centre must not be absorbing"
+    assert h[0] == 1.0, "corner must be absorbing"
+    arr = np.array([1.0, 2.0, 3.0], dtype=np.float32)
+    np.testing.assert_array_equal(b.to_numpy(b.from_numpy(arr)), arr)
+    assert b.to_numpy(b.from_numpy(np.array([1.0, 2.0]))).dtype == np.float32
+
+
+def _ab_buffers(b, ncomp, energy=0.0):
+    """The reference tests' hand-made A/B buffers: Psi = 0 (or an energy blob
+    at the centre of component 0), chi = 19, as flat float32 device arrays."""
+    total = N ** 3
+    psi = np.zeros(ncomp * total, dtype=np.float32)
+    c = N // 2
+    psi[c * N * N + c * N + c] = energy
+    chi = np.full(total, 19.0, dtype=np.float32)
+    return {k: b.from_numpy(v) for k, v in (
+        ("pr", psi), ("prp", psi), ("pi", psi * 0), ("pip", psi * 0), ("chi", chi), ("chip", chi))}
+
+
+@pytest.mark.parametrize("level", ["REAL", "COMPLEX", "COLOR"])
+def test_backend_empty_universe_stable(gpu, level):
+    """10 A/B-toggled steps of the backend's own step function from Psi = 0,
+    chi = 19 with the 0.15 mask: Psi stays 0 (atol 1e-10), chi stays 19
+    (atol 1e-5), the reference's bars on its absolute-chi buffers."""
+    import lfm_hip as L
+    b = L.HipBackend()
+    mask = b.create_boundary_mask(N, 0.15)
+    nc = 3 if level == "COLOR" else 1
+    A, B = _ab_buffers(b, nc), _ab_buffers(b, nc)
+    dt2 = 0.02 ** 2
+    for _ in range(10):
+        if level == "REAL":
+            b.step_real(A["pr"], A["prp"], A["chi"], A["chip"], mask, B["pr"], B["prp"],
+                        B["chi"], B["chip"], N, dt2, 1 / 63, 0.0, 19.0, 0.0)
+        else:
+            fn = b.step_complex if level == "COMPLEX" else b.step_color
+            fn(A["pr"], A["prp"], A["pi"], A["pip"], A["chi"], A["chip"], mask, B["pr"], B["prp"],
+               B["pi"], B["pip"], B["chi"], B["chip"], N, dt2, 1 / 63, 0.0, 19.0, 0.0, 0.1)
+        A, B = B, A
+    np.testing.assert_allclose(b.to_numpy(A["pr"]), 0.0, atol=1e-10)
+    if level != "REAL":
+        np.testing.assert_allclose(b.to_numpy(A["pi"]), 0.0, atol=1e-10)
+    np.testing.assert_allclose(b.to_numpy(A["chi"]), 19.0, atol=1e-5)
+
+
+def test_backend_energy_reduces_chi(gpu):
+    """A blob of E = 5 at the centre (E_prev too), 50 A/B-toggled REAL steps:
+    chi at the centre drops below chi0."""
+    import lfm_hip as L
+    b = L.HipBackend()
+    mask = b.create_boundary_mask(N, 0.15)
+    A, B = _ab_buffers(b, 1, energy=5.0), _ab_buffers(b, 1)
+    for _ in range(50):
+        b.step_real(A["pr"], A["prp"], A["chi"], A["chip"], mask, B["pr"], B["prp"], B["chi"],
+                    B["chip"], N, 0.02 ** 2, 1 / 63, 0.0, 19.0, 0.0)
+        A, B = B, A
+    c = N // 2
+    assert b.to_numpy(A["chi"])[c * N * N + c * N + c] < 19.0
