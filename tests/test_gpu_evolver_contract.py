@@ -217,3 +217,24 @@ def test_backend_energy_reduces_chi(gpu):
         A, B = B, A
     c = N // 2
     assert b.to_numpy(A["chi"])[c * N * N + c * N + c] < 19.0
+
+
+# --- tests/test_gpu_parity.py:428-443 (allocate) -----------------------------
+
+def test_backend_allocate(gpu):
+    """allocate(N, n_psi_arrays, chi0): the eight A/B buffers, Psi zeros of
+    n_psi_arrays * N^3 and chi filled with chi0 (numpy_backend.py:28-47),
+    each its own storage."""
+    import lfm_hip as L
+    b = L.HipBackend()
+    bufs = b.allocate(N, 2, 19.0)
+    assert sorted(bufs) == sorted(["psi_A", "psi_prev_A", "chi_A", "chi_prev_A",
+                                   "psi_B", "psi_prev_B", "chi_B", "chi_prev_B"])
+    for k, v in bufs.items():
+        h = b.to_numpy(v)
+        assert h.dtype == np.float32
+        if k.startswith("psi"):
+            assert h.shape == (2 * N ** 3,) and np.all(h == 0.0), k
+        else:
+            assert h.shape == (N ** 3,) and np.all(h == np.float32(19.0)), k
+    assert len({v.data_ptr() for v in bufs.values()}) == 8
