@@ -7,13 +7,20 @@ in include/lfm_hip.h) behind the reference's own interfaces:
 """
 from .config import (BoundaryType, ChiMode, FieldLevel, PhysicsScale, SimulationConfig,
                      full_physics, gravity_em, gravity_only, spinor_field)
+# the step's constants, under the names lfm exports them (lfm/__init__.py:98-125)
+from .config import (ALPHA_S, BETA_0, CHI0, D_ST, DT_DEFAULT, E_AMPLITUDE_BY_GRID, EPSILON_CC,
+                     EPSILON_W, KAPPA, KAPPA_C, KAPPA_STRING, KAPPA_TUBE, LAMBDA_H, N_COLORS,
+                     RANK_SU3, SA_D, SA_GAMMA)
 from .backend import HipBackend
 from .evolver import Evolver, get_backend
 from .simulation import Simulation
-from .batch import LatticeBatch
+from .batch import LatticeBatch, sweep, sweep_2d
 from . import fields
 
 __all__ = ["BoundaryType", "ChiMode", "FieldLevel", "PhysicsScale", "SimulationConfig",
            "full_physics", "gravity_em", "gravity_only", "spinor_field", "HipBackend",
-           "Evolver", "get_backend", "Simulation", "LatticeBatch", "fields"]
+           "Evolver", "get_backend", "Simulation", "LatticeBatch", "sweep", "sweep_2d", "fields",
+           "ALPHA_S", "BETA_0", "CHI0", "D_ST", "DT_DEFAULT", "E_AMPLITUDE_BY_GRID", "EPSILON_CC",
+           "EPSILON_W", "KAPPA", "KAPPA_C", "KAPPA_STRING", "KAPPA_TUBE", "LAMBDA_H", "N_COLORS",
+           "RANK_SU3", "SA_D", "SA_GAMMA"]
 __version__ = "0.1.0"

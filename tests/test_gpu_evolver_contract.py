@@ -238,3 +238,25 @@ def test_backend_allocate(gpu):
         else:
             assert h.shape == (N ** 3,) and np.all(h == np.float32(19.0)), k
     assert len({v.data_ptr() for v in bufs.values()}) == 8
+
+
+# --- tests/test_integration.py:29-106 (sweep API) ---------------------------
+
+def test_sweep_api(gpu):
+    import lfm_hip as L
+    cfg = L.SimulationConfig(grid_size=8)
+    rows = L.sweep(cfg, param="kappa", values=[0.01, 0.02], steps=5, equilibrate=False,
+                   backend="hip")
+    assert isinstance(rows, list) and len(rows) == 2
+    assert rows[0]["kappa"] == 0.01 and rows[1]["kappa"] == 0.02
+    assert "chi_min" in rows[0] and "energy_total" in rows[0]
+    only = L.sweep(cfg, param="kappa", values=[0.016], steps=5, metric_names=["chi_min"],
+                   equilibrate=False, backend="hip")
+    assert "chi_min" in only[0] and "total_energy" not in only[0]
+    cfg16 = L.SimulationConfig(grid_size=16)
+    for eq in (False, True):
+        r = L.sweep(cfg16, param="kappa", values=[1 / 63], steps=5, soliton={"amplitude": 3.0},
+                    equilibrate=eq, backend="hip")
+        assert len(r) == 1
+        if not eq:
+            assert r[0]["chi_min"] < L.CHI0
