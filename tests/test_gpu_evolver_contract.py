@@ -260,3 +260,54 @@ def test_sweep_api(gpu):
         assert len(r) == 1
         if not eq:
             assert r[0]["chi_min"] < L.CHI0
+
+
+# --- tests/test_simulation.py:298-389 (checkpoint round trips) ---------------
+
+def _small(**kw):
+    import lfm_hip as L
+    d = dict(grid_size=N, field_level=L.FieldLevel.REAL, boundary_type=L.BoundaryType.FROZEN,
+             report_interval=100, dt=0.02)
+    d.update(kw)
+    return L.SimulationConfig(**d)
+
+
+def test_checkpoint_round_trips(gpu, tmp_path):
+    import lfm_hip as L
+    c = (N // 2, N // 2, N // 2)
+    sim = L.Simulation(_small(), backend="hip")
+    sim.place_soliton(c, amplitude=3.0, sigma=2.0)
+    sim.equilibrate()
+    sim.run(10, record_metrics=False)
+    path = tmp_path / "ckpt.npz"
+    sim.save_checkpoint(path)
+    assert path.exists() and path.stat().st_size > 0
+    loaded = L.Simulation.load_checkpoint(path, backend="hip")
+    assert loaded.step == sim.step
+    np.testing.assert_allclose(loaded.get_chi(), sim.get_chi(), atol=1e-6)
+    np.testing.assert_allclose(loaded.get_psi_real(), sim.get_psi_real(), atol=1e-6)
+
+    cfg = _small(chi0=18.5, kappa=0.02)
+    s2 = L.Simulation(cfg, backend="hip")
+    s2.run(10, record_metrics=False)
+    s2.save_checkpoint(tmp_path / "c2.npz")
+    l2 = L.Simulation.load_checkpoint(tmp_path / "c2.npz", backend="hip")
+    assert l2.config.grid_size == cfg.grid_size
+    assert l2.config.chi0 == pytest.approx(cfg.chi0)
+    assert l2.config.kappa == pytest.approx(cfg.kappa)
+
+    s3 = L.Simulation(_small(report_interval=5), backend="hip")
+    s3.place_soliton(c, amplitude=3.0, sigma=2.0)
+    s3.run(10)
+    s3.save_checkpoint(tmp_path / "c3.npz")
+    assert len(L.Simulation.load_checkpoint(tmp_path / "c3.npz", backend="hip").history) == \
+        len(s3.history)
+
+    s4 = L.Simulation(_small(field_level=L.FieldLevel.COMPLEX), backend="hip")
+    s4.place_soliton(c, amplitude=3.0, sigma=2.0, phase=0.5)
+    s4.equilibrate()
+    s4.run(10, record_metrics=False)
+    s4.save_checkpoint(tmp_path / "c4.npz")
+    l4 = L.Simulation.load_checkpoint(tmp_path / "c4.npz", backend="hip")
+    assert l4.config.field_level == L.FieldLevel.COMPLEX
+    np.testing.assert_allclose(l4.get_psi_imag(), s4.get_psi_imag(), atol=1e-6)
