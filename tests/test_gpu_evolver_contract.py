@@ -311,3 +311,30 @@ def test_checkpoint_round_trips(gpu, tmp_path):
     l4 = L.Simulation.load_checkpoint(tmp_path / "c4.npz", backend="hip")
     assert l4.config.field_level == L.FieldLevel.COMPLEX
     np.testing.assert_allclose(l4.get_psi_imag(), s4.get_psi_imag(), atol=1e-6)
+
+
+# --- tests/test_integration.py:210-229, tests/test_edge_cases.py:55-82 ------
+
+def test_no_nan_runs_and_metric_keys(gpu):
+    """COLOR with kappa_c (amplitude 6, 20 steps), an empty REAL run, COMPLEX
+    and COLOR with a soliton (10 steps): no NaN; metrics() carries chi_min,
+    chi_max and energy_total."""
+    import lfm_hip as L
+    sim = L.Simulation(L.SimulationConfig(grid_size=16, field_level=L.FieldLevel.COLOR,
+                                          kappa_c=1 / 189), backend="hip")
+    sim.place_soliton((8, 8, 8), amplitude=6.0)
+    sim.run(steps=20)
+    assert not np.any(np.isnan(sim.chi)) and not np.any(np.isnan(sim.psi_real))
+    sim = L.Simulation(L.SimulationConfig(grid_size=16, e_amplitude=0.0), backend="hip")
+    sim.run(steps=10)
+    assert not np.any(np.isnan(sim.chi))
+    for lvl in (L.FieldLevel.COMPLEX, L.FieldLevel.COLOR):
+        sim = L.Simulation(L.SimulationConfig(grid_size=16, field_level=lvl), backend="hip")
+        sim.place_soliton((8, 8, 8), amplitude=2.0)
+        sim.run(steps=10)
+        assert not np.any(np.isnan(sim.chi))
+    sim = L.Simulation(L.SimulationConfig(grid_size=16), backend="hip")
+    sim.place_soliton((8, 8, 8), amplitude=2.0)
+    sim.run(steps=5)
+    m = sim.metrics()
+    assert "chi_min" in m and "chi_max" in m and "energy_total" in m
