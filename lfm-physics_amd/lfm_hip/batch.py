@@ -36,7 +36,7 @@ import torch
 
 from . import _abi
 from .backend import ScvPlan, params, stream_handle
-from .config import BoundaryType, FieldLevel, SimulationConfig
+from .config import BoundaryType, SimulationConfig
 from .evolver import get_backend
 from .fields import boundary_mask_host, gaussian_soliton
 

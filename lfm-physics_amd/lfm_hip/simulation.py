@@ -288,7 +288,6 @@ class Simulation:
     def equilibrate(self) -> None:
         """Poisson-equilibrate chi from |Psi|^2 (GOV-04) on the GPU
         (lfm/simulation.py:877-963, lfm/fields/equilibrium.py:26-100)."""
-        import ctypes
         from . import _abi
         from .backend import ScvPlan, stream_handle
         if not self._solitons_placed:
