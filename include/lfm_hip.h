@@ -95,8 +95,9 @@ int lfm_step_complex_f32(const float* pr, const float* pr_prev, const float* pi,
  * sa:          NULL, or 3*N^3 S_a values (raw Helmholtz output; the kernel
  *              clips at 0 on read) -- needed when kappa_tube > 0.
  * sa_clip_out: NULL, or receives max(sa, 0) (the reference's sa_fields_out).
- * psisq_out:   NULL, or receives |Psi_a'|^2 of the NEW fields (3*N^3), the
- *              input of the next step's S_a transform. */
+ * psisq_out:   NULL, or receives |Psi_a'|^2 = fmaf(pr', pr', pi'*pi') of the
+ *              NEW fields (3*N^3), the input of the next step's S_a transform
+ *              (the rounding the transforms use when they form it from Psi). */
 /* Two leapfrog steps per call (REAL / COMPLEX), rotation mode: from
  * (Psi_n = pr/pi, Psi_{n-1} = pr_prev/pi_prev, chi_n, chi_{n-1}) it writes
  * step n+1 into *_out1 and step n+2 into *_out2, exactly as two calls of
@@ -161,7 +162,9 @@ int lfm_scv_plan_destroy(void* plan);
 /* e = |Psi_a|^2 given directly (ncolors*N^3). */
 int lfm_scv_apply_psisq(void* plan, const float* e, float gamma, float D, float* sa_out,
                         void* stream);
-/* e computed from pr, pi first (into the plan's own buffer). */
+/* e = |Psi_a|^2 = fmaf(pr, pr, pi*pi) formed from pr, pi: by the direct
+ * method in its first pass (no e field is stored), by the rocFFT method in
+ * the plan's own buffer first. */
 int lfm_scv_apply(void* plan, const float* pr, const float* pi, float gamma, float D,
                   float* sa_out, void* stream);
 /* The SCV term of the COLOR step straight from the transform (for
@@ -169,8 +172,9 @@ int lfm_scv_apply(void* plan, const float* pr, const float* pi, float gamma, flo
  * sum_a S_a^2 - 1/3 (sum_a S_a)^2 with S_a = max(0, irfftn(gamma/(gamma + D k^2)
  * rfftn(e_a))) (numpy_backend.py:338-358, the SCV of kernel_source.py:187-197),
  * bit for bit what the step computes from the three S_a fields, written as
- * one N^3 field.  e = |Psi_a|^2 (3 N^3) when given, else computed from pr, pi.
- * Direct-method plans for 3 colours only. */
+ * one N^3 field.  e = |Psi_a|^2 (3 N^3) when given, else formed from pr, pi
+ * in the first pass as lfm_scv_apply does.  Direct-method plans for 3 colours
+ * only. */
 int lfm_scv_apply_scv_f32(void* plan, const float* pr, const float* pi, const float* e,
                           float gamma, float D, float* scv_out, void* stream);
 /* GOV-04 chi equilibration (lfm/fields/equilibrium.py:26-59,62-100, used by

@@ -49,6 +49,7 @@ void launch_stream(const StreamArgs& A, int grid, hipStream_t st) {
     const int key = A.nin * 100 + A.nout;
     switch (key) {
         case 1810: hipLaunchKernelGGL((stream_kernel<18, 10, V, NT>), g, b, 0, st, A); break;
+        case 1610: hipLaunchKernelGGL((stream_kernel<16, 10, V, NT>), g, b, 0, st, A); break;
         case 603: hipLaunchKernelGGL((stream_kernel<6, 3, V, NT>), g, b, 0, st, A); break;
         case 703: hipLaunchKernelGGL((stream_kernel<7, 3, V, NT>), g, b, 0, st, A); break;
         case 502: hipLaunchKernelGGL((stream_kernel<5, 2, V, NT>), g, b, 0, st, A); break;

@@ -55,8 +55,7 @@ __global__ __launch_bounds__(256) void psisq_kernel(const float* __restrict__ pr
                                                     float* __restrict__ e, size_t n) {
     for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
          t += size_t(gridDim.x) * blockDim.x) {
-        const float r = pr[t], m = pi[t];
-        e[t] = r * r + m * m;
+        e[t] = psi_sq(pr[t], pi[t]);
     }
 }
 
@@ -334,8 +333,11 @@ constexpr int rows_pitch() { return N + N / 8; }
 template <int N>
 constexpr Sync rows_sync() { return N / 8 <= 64 ? Sync::Wave : Sync::Block; }
 
-template <int N>
+// PSI: the real lines are |Psi_a|^2, formed here from the Pr lines of `in`
+// and the Pi lines of `in_i` (same layout), so no |Psi|^2 field is stored.
+template <int N, bool PSI>
 __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_r2c(const float* __restrict__ in,
+                                                                 const float* __restrict__ in_i,
                                                                  float2* __restrict__ spec,
                                                                  const float2* __restrict__ tw,
                                                                  int Hp) {
@@ -350,9 +352,20 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_r2c(const float* _
     const float* r1 = r0 + N;
     load_twiddles<N>(twl, tw);
     float2 v[8];
+    if constexpr (PSI) {
+        const float* m0 = in_i + pair * 2 * N;
+        const float* m1 = m0 + N;
 #pragma unroll
-    for (int e = 0; e < 8; ++e)
-        v[e] = make_float2(ld1nt<N>(r0 + t + e * TC), ld1nt<N>(r1 + t + e * TC));
+        for (int e = 0; e < 8; ++e) {
+            const int k = t + e * TC;
+            v[e] = make_float2(psi_sq(ld1nt<N>(r0 + k), ld1nt<N>(m0 + k)),
+                               psi_sq(ld1nt<N>(r1 + k), ld1nt<N>(m1 + k)));
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            v[e] = make_float2(ld1nt<N>(r0 + t + e * TC), ld1nt<N>(r1 + t + e * TC));
+    }
     float2* L = lds + q * PS;
     auto at = [](int pos) { return pos + pos / 8; };
     constexpr Sync S = rows_sync<N>();
@@ -529,15 +542,19 @@ int launch_columns(const ColArgs& A, hipStream_t st) {
     return LFM_OK;
 }
 
-// dir 0: real lines -> half spectra; 1: half spectra -> real lines
+// dir 0: real lines -> half spectra (of |Psi|^2 when in_i, the Pi lines, is
+// given); 1: half spectra -> real lines
 template <int N>
-int launch_rows(int dir, const float* in, float2* spec, float* out, const float2* tw, int Hp,
-                int nc, hipStream_t st) {
+int launch_rows(int dir, const float* in, const float* in_i, float2* spec, float* out,
+                const float2* tw, int Hp, int nc, hipStream_t st) {
     const long long pairs = (long long)nc * N * N / 2;
     const long long nb = pairs / rows_pairs<N>();
-    if (dir == 0)
-        hipLaunchKernelGGL(fft_rows_r2c<N>, dim3(unsigned(nb)), dim3(rows_threads<N>()), 0, st, in,
-                           spec, tw, Hp);
+    if (dir == 0 && in_i)
+        hipLaunchKernelGGL((fft_rows_r2c<N, true>), dim3(unsigned(nb)), dim3(rows_threads<N>()), 0,
+                           st, in, in_i, spec, tw, Hp);
+    else if (dir == 0)
+        hipLaunchKernelGGL((fft_rows_r2c<N, false>), dim3(unsigned(nb)), dim3(rows_threads<N>()), 0,
+                           st, in, nullptr, spec, tw, Hp);
     else
         hipLaunchKernelGGL(fft_rows_c2r<N>, dim3(unsigned(nb)), dim3(rows_threads<N>()), 0, st,
                            spec, out, tw, Hp);
@@ -556,10 +573,10 @@ int columns(int N, const ColArgs& A, hipStream_t st) {
     }
 }
 
-int rows(int N, int dir, const float* in, float2* spec, float* out, const float2* tw, int Hp,
-         int nc, hipStream_t st) {
+int rows(int N, int dir, const float* in, const float* in_i, float2* spec, float* out,
+         const float2* tw, int Hp, int nc, hipStream_t st) {
     switch (N) {
-#define LFM_CASE(n) case n: return launch_rows<n>(dir, in, spec, out, tw, Hp, nc, st);
+#define LFM_CASE(n) case n: return launch_rows<n>(dir, in, in_i, spec, out, tw, Hp, nc, st);
         LFM_FFT_SIZES(LFM_CASE)
 #undef LFM_CASE
         default: set_error("row FFT: N=%d unsupported", N); return LFM_E_ARG;
@@ -642,16 +659,25 @@ int run_columns(ScvPlan* p, int filter, float gamma, float D, hipStream_t st, in
 }
 
 // Transform e -> spectrum -> filter -> out.  filter 0: Helmholtz; 1: Poisson.
-int run_filter(ScvPlan* p, const float* e, int filter, float gamma, float D, float* out,
-               hipStream_t st, int ncomp_used) {
+// With e_i given, e and e_i are the Pr and Pi fields and the transformed lines
+// are |Psi|^2: the direct method forms it in its first pass, the rocFFT
+// method in the plan's real-space buffer first.
+int run_filter(ScvPlan* p, const float* e, const float* e_i, int filter, float gamma, float D,
+               float* out, hipStream_t st, int ncomp_used) {
     const int N = p->N;
     const double n3 = double(N) * N * N;
     if (p->method == 2) {
-        int rc = rows(N, 0, e, p->spec, nullptr, p->tw, p->Hp, ncomp_used, st);
+        int rc = rows(N, 0, e, e_i, p->spec, nullptr, p->tw, p->Hp, ncomp_used, st);
         if (rc) return rc;
         rc = run_columns(p, filter, gamma, D, st, ncomp_used);
         if (rc) return rc;
-        return rows(N, 1, nullptr, p->spec, out, p->tw, p->Hp, ncomp_used, st);
+        return rows(N, 1, nullptr, nullptr, p->spec, out, p->tw, p->Hp, ncomp_used, st);
+    }
+    if (e_i) {
+        const size_t n = size_t(ncomp_used) * N * N * N;
+        hipLaunchKernelGGL(psisq_kernel, dim3(grid_for(n)), dim3(256), 0, st, e, e_i, p->e, n);
+        LFM_LAUNCH_CHECK("psisq_kernel");
+        e = p->e;
     }
     LFM_FFT_TRY(rocfft_execution_info_set_stream(p->info, st));
     void* in_f[1] = {const_cast<float*>(e)};
@@ -673,15 +699,18 @@ int run_filter(ScvPlan* p, const float* e, int filter, float gamma, float D, flo
     return LFM_OK;
 }
 
-int run(ScvPlan* p, const float* e, float gamma, float D, float* sa_out, hipStream_t st) {
-    return run_filter(p, e, 0, gamma, D, sa_out, st, p->nc);
+int run(ScvPlan* p, const float* e, const float* e_i, float gamma, float D, float* sa_out,
+        hipStream_t st) {
+    return run_filter(p, e, e_i, 0, gamma, D, sa_out, st, p->nc);
 }
 
-// Direct method, three colours: e -> spectrum -> Helmholtz filter -> the
-// SCV term (one field) instead of the three S_a fields.
-int run_scv(ScvPlan* p, const float* e, float gamma, float D, float* scv_out, hipStream_t st) {
+// Direct method, three colours: e (or |Psi|^2 of Pr = e, Pi = e_i) ->
+// spectrum -> Helmholtz filter -> the SCV term (one field) instead of the
+// three S_a fields.
+int run_scv(ScvPlan* p, const float* e, const float* e_i, float gamma, float D, float* scv_out,
+            hipStream_t st) {
     const int N = p->N;
-    int rc = rows(N, 0, e, p->spec, nullptr, p->tw, p->Hp, 3, st);
+    int rc = rows(N, 0, e, e_i, p->spec, nullptr, p->tw, p->Hp, 3, st);
     if (rc) return rc;
     rc = run_columns(p, 0, gamma, D, st, 3);
     if (rc) return rc;
@@ -701,10 +730,7 @@ int scv_prepare(void* plan, const float* pr, const float* pi, float gamma, float
                 hipStream_t st) {
     auto* p = static_cast<ScvPlan*>(plan);
     if (!p || p->method != 2) { set_error("scv_prepare: needs a direct-method plan"); return LFM_E_ARG; }
-    const size_t n = size_t(p->nc) * p->N * p->N * p->N;
-    hipLaunchKernelGGL(psisq_kernel, dim3(grid_for(n)), dim3(256), 0, st, pr, pi, p->e, n);
-    LFM_LAUNCH_CHECK("psisq_kernel");
-    int rc = rows(p->N, 0, p->e, p->spec, nullptr, p->tw, p->Hp, p->nc, st);
+    int rc = rows(p->N, 0, pr, pi, p->spec, nullptr, p->tw, p->Hp, p->nc, st);
     if (rc) return rc;
     return run_columns(p, 0, gamma, D, st, p->nc);
 }
@@ -768,7 +794,11 @@ extern "C" int lfm_scv_plan_create_ex(int32_t N, int32_t ncolors, int32_t method
     if (p->work_bytes) he = hipMalloc(&p->work, p->work_bytes);
     const size_t ns = size_t(ncolors) * N * N * p->Hp;
     if (he == hipSuccess) he = hipMalloc((void**)&p->spec, ns * sizeof(float2));
-    if (he == hipSuccess) he = hipMalloc((void**)&p->e, size_t(ncolors) * N * N * N * sizeof(float));
+    // the real-space buffer: the Poisson solve's right-hand side (one-colour
+    // plans) and the rocFFT method's |Psi|^2 input; the direct method forms
+    // |Psi|^2 in its first pass and needs none for several colours
+    if (he == hipSuccess && (method == 1 || ncolors == 1))
+        he = hipMalloc((void**)&p->e, size_t(ncolors) * N * N * N * sizeof(float));
     if (he == hipSuccess && method == 2) {
         // twiddles exp(-2 pi i m / N), computed in double on the host
         float2* h = new float2[N];
@@ -818,7 +848,7 @@ extern "C" int lfm_scv_apply_psisq(void* plan, const float* e, float gamma, floa
         set_error("lfm_scv_apply_psisq: need gamma > 0 and D >= 0 (got %g, %g)", gamma, D);
         return LFM_E_ARG;
     }
-    return run(p, e, gamma, D, sa_out, (hipStream_t)stream);
+    return run(p, e, nullptr, gamma, D, sa_out, (hipStream_t)stream);
 }
 
 extern "C" int lfm_scv_apply(void* plan, const float* pr, const float* pi, float gamma, float D,
@@ -829,11 +859,7 @@ extern "C" int lfm_scv_apply(void* plan, const float* pr, const float* pi, float
         set_error("lfm_scv_apply: need gamma > 0 and D >= 0 (got %g, %g)", gamma, D);
         return LFM_E_ARG;
     }
-    const size_t n = size_t(p->nc) * p->N * p->N * p->N;
-    hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(psisq_kernel, dim3(grid_for(n)), dim3(256), 0, st, pr, pi, p->e, n);
-    LFM_LAUNCH_CHECK("psisq_kernel");
-    return run(p, p->e, gamma, D, sa_out, st);
+    return run(p, pr, pi, gamma, D, sa_out, (hipStream_t)stream);
 }
 
 extern "C" int lfm_scv_prepare_f32(void* plan, const float* pr, const float* pi, float gamma,
@@ -871,13 +897,7 @@ extern "C" int lfm_scv_apply_scv_f32(void* plan, const float* pr, const float* p
         return LFM_E_ARG;
     }
     hipStream_t st = (hipStream_t)stream;
-    if (!e) {
-        const size_t n = size_t(3) * p->N * p->N * p->N;
-        hipLaunchKernelGGL(psisq_kernel, dim3(grid_for(n)), dim3(256), 0, st, pr, pi, p->e, n);
-        LFM_LAUNCH_CHECK("psisq_kernel");
-        e = p->e;
-    }
-    return run_scv(p, e, gamma, D, scv_out, st);
+    return e ? run_scv(p, e, nullptr, gamma, D, scv_out, st) : run_scv(p, pr, pi, gamma, D, scv_out, st);
 }
 
 extern "C" int lfm_clip_nonneg_f32(const float* x, float* y, int64_t n, void* stream) {
@@ -906,7 +926,7 @@ extern "C" int lfm_poisson_chi_f32(void* plan, const float* pr, const float* pi,
     hipLaunchKernelGGL(poisson_rhs, dim3(grid_for(n3)), dim3(256), 0, st, pr, pi, ncomp, kappa,
                        e0_sq, p->e, n3);
     LFM_LAUNCH_CHECK("poisson_rhs");
-    const int rc = run_filter(p, p->e, 1, 0.0f, 0.0f, chi_out, st, 1);
+    const int rc = run_filter(p, p->e, nullptr, 1, 0.0f, 0.0f, chi_out, st, 1);
     if (rc) return rc;
     hipLaunchKernelGGL(poisson_finish, dim3(grid_for(n3)), dim3(256), 0, st, chi_out, p->N, chi0,
                        boundary_fraction);

@@ -22,6 +22,13 @@ __device__ __forceinline__ float lap_part_b(float ctr, float faces) {
     return fmaf(WF, ctr, __fmul_rn(WE, faces));
 }
 
+// |Psi_a|^2 = Pr^2 + Pi^2 (numpy_backend.py:340-341, the S_a transform's
+// input) with its roundings fixed, so the transform pass that forms it from
+// Psi and any kernel writing it for later use give the same bits.
+__device__ __forceinline__ float psi_sq(float r, float m) {
+    return fmaf(r, r, __fmul_rn(m, m));
+}
+
 // The S_a colour-variance term of GOV-02 (kernel_source.py:187-197):
 // SCV = sum_a S_a^2 - 1/3 (sum_a S_a)^2 of the S_a clipped at 0
 // (numpy_backend.py:353), every rounding explicit.  The step kernels apply

@@ -281,11 +281,11 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
     if constexpr (L == COLOR) {
         if (P.psisq) {
 #pragma unroll
-            for (int q = 0; q < C; ++q) idx.st(P.psisq + q * n3, nw[q] * nw[q] + nw[q + C] * nw[q + C]);
+            for (int q = 0; q < C; ++q) idx.st(P.psisq + q * n3, psi_sq(nw[q], nw[q + C]));
         }
         if (ret3) {
 #pragma unroll
-            for (int q = 0; q < C; ++q) ret3[q] = nw[q] * nw[q] + nw[q + C] * nw[q + C];
+            for (int q = 0; q < C; ++q) ret3[q] = psi_sq(nw[q], nw[q + C]);
         }
     }
     if (kp.flags & LFM_F_FREEZE_CHI) return;
