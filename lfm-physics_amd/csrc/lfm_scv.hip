@@ -322,16 +322,19 @@ __global__ __launch_bounds__(ncol_for<N>() * (N / col_vals<N>())) void fft_colum
 // dropped, as a C2R transform does) and takes x_p0 = Re z, x_p1 = Im z.
 // LDS slot of position pos: pos + pos/8, which spreads the radix-8
 // boundaries' stride-8 writes over all banks.
+// Values per thread: 8, and 16 at N = 1024, so a line of N/V <= 64 threads
+// lies in one wavefront at every size and its stage boundaries need no
+// workgroup barrier.
+template <int N>
+constexpr int row_vals() { return N >= 1024 ? 16 : 8; }
 template <int N>
 constexpr int rows_threads() { return 256; }
 template <int N>
-constexpr int rows_pairs() { return rows_threads<N>() / (N / 8); }
+constexpr int rows_pairs() { return rows_threads<N>() / (N / row_vals<N>()); }
 template <int N>
 constexpr int rows_pitch() { return N + N / 8; }
-// Rows of N <= 512 lie in one wavefront each (N/8 <= 64 threads), so their
-// stage boundaries need no workgroup barrier.
 template <int N>
-constexpr Sync rows_sync() { return N / 8 <= 64 ? Sync::Wave : Sync::Block; }
+constexpr Sync rows_sync() { return N / row_vals<N>() <= 64 ? Sync::Wave : Sync::Block; }
 
 // PSI: the real lines are |Psi_a|^2, formed here from the Pr lines of `in`
 // and the Pi lines of `in_i` (same layout), so no |Psi|^2 field is stored.
@@ -341,7 +344,8 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_r2c(const float* _
                                                                  float2* __restrict__ spec,
                                                                  const float2* __restrict__ tw,
                                                                  int Hp) {
-    constexpr int TC = N / 8;
+    constexpr int V = row_vals<N>();
+    constexpr int TC = N / V;
     constexpr int PB = rows_pairs<N>();
     constexpr int PS = rows_pitch<N>();
     __shared__ float2 lds[PB * PS];
@@ -351,19 +355,19 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_r2c(const float* _
     const float* r0 = in + pair * 2 * N;
     const float* r1 = r0 + N;
     load_twiddles<N>(twl, tw);
-    float2 v[8];
+    float2 v[V];
     if constexpr (PSI) {
         const float* m0 = in_i + pair * 2 * N;
         const float* m1 = m0 + N;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
+        for (int e = 0; e < V; ++e) {
             const int k = t + e * TC;
             v[e] = make_float2(psi_sq(ld1nt<N>(r0 + k), ld1nt<N>(m0 + k)),
                                psi_sq(ld1nt<N>(r1 + k), ld1nt<N>(m1 + k)));
         }
     } else {
 #pragma unroll
-        for (int e = 0; e < 8; ++e)
+        for (int e = 0; e < V; ++e)
             v[e] = make_float2(ld1nt<N>(r0 + t + e * TC), ld1nt<N>(r1 + t + e * TC));
     }
     float2* L = lds + q * PS;
@@ -372,17 +376,17 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_r2c(const float* _
     // with wavefront-level boundaries the twiddle table, which the whole
     // workgroup fills, is published here
     if constexpr (S == Sync::Wave) __syncthreads();
-    fft_line<N, S>(v, L, t, -1, twl, at);
+    fft_line<N, S, V>(v, L, t, -1, twl, at);
     line_sync<S>();
 #pragma unroll
-    for (int e = 0; e < 8; ++e) L[at(t + e * TC)] = v[e];
+    for (int e = 0; e < V; ++e) L[at(t + e * TC)] = v[e];
     line_sync<S>();
     float2* o0 = spec + pair * 2 * Hp;
     float2* o1 = o0 + Hp;
 #pragma unroll
-    for (int e = 0; e < 5; ++e) {
+    for (int e = 0; e <= V / 2; ++e) {
         const int k = t + e * TC;
-        if (e == 4 && t != 0) break;  // k = N/2 once
+        if (e == V / 2 && t != 0) break;  // k = N/2 once
         const float2 zk = L[at(k)];
         const float2 zm = L[at((N - k) & (N - 1))];
         st2nt<N>(o0 + k, make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y)));
@@ -395,7 +399,8 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_c2r(const float2* 
                                                                  float* __restrict__ out,
                                                                  const float2* __restrict__ tw,
                                                                  int Hp) {
-    constexpr int TC = N / 8;
+    constexpr int V = row_vals<N>();
+    constexpr int TC = N / V;
     constexpr int PB = rows_pairs<N>();
     constexpr int PS = rows_pitch<N>();
     __shared__ float2 lds[PB * PS];
@@ -419,9 +424,9 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_c2r(const float2* 
         }
     }
     __syncthreads();
-    float2 v[8];
+    float2 v[V];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
+    for (int e = 0; e < V; ++e) {
         const int k = t + e * TC;
         const bool lo = k <= N / 2;
         const int m = lo ? k : N - k;
@@ -431,11 +436,11 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_c2r(const float2* 
         v[e] = make_float2(a.x - b.y, a.y + b.x);
     }
     // the transform's first boundary orders these reads before its writes
-    fft_line<N, rows_sync<N>()>(v, L, t, 1, twl, [](int pos) { return pos + pos / 8; });
+    fft_line<N, rows_sync<N>(), V>(v, L, t, 1, twl, [](int pos) { return pos + pos / 8; });
     float* r0 = out + pair * 2 * N;
     float* r1 = r0 + N;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
+    for (int e = 0; e < V; ++e) {
         st1nt<N>(r0 + t + e * TC, v[e].x);
         st1nt<N>(r1 + t + e * TC, v[e].y);
     }
@@ -450,15 +455,16 @@ __global__ __launch_bounds__(rows_threads<N>()) void fft_rows_c2r(const float2* 
 // three (and read by the step instead of three).  Colours 1 and 2 pass their
 // values to colour 0's threads through LDS (same positions t + e*TC).
 template <int N>
-constexpr int scv_pairs() { return N / 8 >= 64 ? 1 : 64 / (N / 8); }
+constexpr int scv_pairs() { return N / row_vals<N>() >= 64 ? 1 : 64 / (N / row_vals<N>()); }
 template <int N>
-constexpr int scv_threads() { return 3 * scv_pairs<N>() * (N / 8); }
+constexpr int scv_threads() { return 3 * scv_pairs<N>() * (N / row_vals<N>()); }
 
 template <int N>
 __global__ __launch_bounds__(scv_threads<N>()) void fft_rows_c2r_scv(
         const float2* __restrict__ spec, float* __restrict__ scv,
         const float2* __restrict__ tw, int Hp) {
-    constexpr int TC = N / 8;
+    constexpr int V = row_vals<N>();
+    constexpr int TC = N / V;
     constexpr int PJ = scv_pairs<N>();
     constexpr int PS = rows_pitch<N>();
     constexpr long long NP = (long long)N * N / 2;   // row pairs per colour
@@ -482,9 +488,9 @@ __global__ __launch_bounds__(scv_threads<N>()) void fft_rows_c2r_scv(
         }
     }
     __syncthreads();
-    float2 v[8];
+    float2 v[V];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
+    for (int e = 0; e < V; ++e) {
         const int k = t + e * TC;
         const bool lo = k <= N / 2;
         const int m = lo ? k : N - k;
@@ -493,14 +499,14 @@ __global__ __launch_bounds__(scv_threads<N>()) void fft_rows_c2r_scv(
         if (!lo) { x.y = -x.y; y.y = -y.y; }
         v[e] = make_float2(x.x - y.y, x.y + y.x);
     }
-    fft_line<N, rows_sync<N>()>(v, L, t, 1, twl, [](int pos) { return pos + pos / 8; });
+    fft_line<N, rows_sync<N>(), V>(v, L, t, 1, twl, [](int pos) { return pos + pos / 8; });
     // v[e] = (S_a at row 2 pair, S_a at row 2 pair + 1), position t + e TC.
     // The transform's last LDS reads precede this barrier; colours 1 and 2
     // then park their results in their own slots for colour 0.
     __syncthreads();
     if (a > 0) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) L[t + e * TC] = v[e];
+        for (int e = 0; e < V; ++e) L[t + e * TC] = v[e];
     }
     __syncthreads();
     if (a == 0) {
@@ -509,7 +515,7 @@ __global__ __launch_bounds__(scv_threads<N>()) void fft_rows_c2r_scv(
         float* r0 = scv + pair * 2 * N;
         float* r1 = r0 + N;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
+        for (int e = 0; e < V; ++e) {
             const float2 w1 = L1[t + e * TC], w2 = L2[t + e * TC];
             float clip[3];
             st1nt<N>(r0 + t + e * TC, scv_term(v[e].x, w1.x, w2.x, clip));
