@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define LFM_ABI_VERSION 3
+#define LFM_ABI_VERSION 4
 
 /* Error codes. */
 #define LFM_OK 0
@@ -55,6 +55,14 @@ extern "C" {
                                  three S_a fields (same bits; 4 instead of 12 bytes
                                  per site read); needs kappa_tube > 0 and no
                                  sa_clip_out */
+#define LFM_F_KEEP_PSI 0x40   /* every new Psi value is multiplied by keep[site]
+                                 (lfm_params_t.keep, N^3 floats, one factor per site
+                                 for every component) after the update: the
+                                 in-place `psi *= factor` an experiment driver
+                                 applies to the current buffers after each step
+                                 (lfm/experiment/barrier.py:327-353), same bits.
+                                 Single-step entry points (lfm_step_real_f32,
+                                 _complex_f32, _color_f32) only */
 
 /* Number of tile shapes of the tiled kernel (lfm_params_t.tile = 1..this). */
 #define LFM_TILE_VARIANTS 6
@@ -63,13 +71,16 @@ extern "C" {
  * lfm/core/evolver.py:201-279).  Unused terms are 0.
  * tile:  0 = the default tile shape (1), 1..LFM_TILE_VARIANTS a specific
  *        one ({4,1} {4,2} {4,4} {8,1} {8,2} {16,1} wavefronts x rows/thread).
- * chunk: 0 = automatic, else the number of i planes one workgroup marches. */
+ * chunk: 0 = automatic, else the number of i planes one workgroup marches.
+ * keep:  NULL, or the N^3 per-site factors of LFM_F_KEEP_PSI (device memory,
+ *        read only; it may not overlap an output). */
 typedef struct lfm_params {
     int32_t N;
     int32_t flags;
     float dt2, kappa, lambda_self, chi0, e0_sq, epsilon_w;
     float kappa_c, epsilon_cc, kappa_string, kappa_tube;
     int32_t tile, chunk;
+    const float* keep;
 } lfm_params_t;
 
 int lfm_abi_version(void);

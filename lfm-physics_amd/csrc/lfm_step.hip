@@ -72,6 +72,7 @@ struct Ptrs {
     float* chip_out;     // NULL in rotation mode
     float* sa_clip;      // NULL or max(sa, 0)
     float* psisq;        // NULL or |Psi'_a|^2 of the new fields
+    const float* keep;   // NULL unless LFM_F_KEEP_PSI: per-site factor on Psi'
     size_t n3;           // N^3
 };
 
@@ -260,7 +261,9 @@ __device__ __forceinline__ void site_compute(const KParams& kp, const PointData<
 }
 
 // Per-site update with its stores (site_compute, then the outputs of P).
-template <int L, class O>
+// KEEP: the LFM_F_KEEP_PSI kernels (a separate instantiation, so the others
+// carry none of its code).
+template <int L, class O, bool KEEP = false>
 __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O idx,
                                             const PointData<L>& pd,
                                             const float (&c)[Lv<L>::NPF],
@@ -272,6 +275,12 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
     const size_t n3 = P.n3;
     float nw[NPF], cn, sclip[3];
     site_compute<L>(kp, pd, P.sa != nullptr, c, lap, chi_st, lap_chi, jsum, ccv, nw, cn, sclip);
+    if constexpr (KEEP) {
+        // a driver's `psi *= factor` after the step, folded in
+        const float k = idx.ldp(P.keep);
+#pragma unroll
+        for (int f = 0; f < NPF; ++f) nw[f] = __fmul_rn(nw[f], k);
+    }
     const float a = 1.0f - ((kp.flags & LFM_F_MASK) ? pd.m : 0.0f);
 #pragma unroll
     for (int f = 0; f < NPF; ++f) {
@@ -303,7 +312,7 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
 // Direct kernel: one thread per site, on a (k, j, i) grid of 64 x 4
 // workgroups, so the coordinates come from the launch.
 // ---------------------------------------------------------------------------
-template <int L>
+template <int L, bool KEEP = false>
 __global__ __launch_bounds__(256) void step_direct(Ptrs P, KParams kp) {
     constexpr int NPF = Lv<L>::NPF;
     constexpr int C = Lv<L>::NC;
@@ -364,8 +373,8 @@ __global__ __launch_bounds__(256) void step_direct(Ptrs P, KParams kp) {
             }
         }
     }
-    site_update<L>(kp, P, Off64{idx}, load_point<L>(kp, P, Off64{idx}), c, lap, chi_c, lap_chi,
-                   jsum, ccv);
+    site_update<L, Off64, KEEP>(kp, P, Off64{idx}, load_point<L>(kp, P, Off64{idx}), c, lap,
+                                chi_c, lap_chi, jsum, ccv);
 }
 
 // ---------------------------------------------------------------------------
@@ -381,7 +390,7 @@ constexpr int TK = 64;
 
 // One workgroup's work: the column tile at (k0, j0) over planes [i0, i1) of
 // one lattice (P, kp).  Shared by the single-lattice and batched kernels.
-template <int L, int BJ, int RJ, bool NT>
+template <int L, int BJ, int RJ, bool NT, bool KEEP = false>
 __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int k0, int j0,
                                           int i0, int i1) {
     constexpr int NPF = Lv<L>::NPF;
@@ -550,8 +559,9 @@ __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int 
                         if (kp.ks > 0.0f) ccv = ccvyz[rr] + (qx - (1.0f / 3.0f) * sx * sx);
                     }
                 }
-                site_update<L>(kp, P, Off32T<NT>{pend_pb + ioff0 + rr * rowB}, ptc[rr], cpend, lap,
-                               chim1[rr], lap_chi, jsum, ccv);
+                site_update<L, Off32T<NT>, KEEP>(kp, P, Off32T<NT>{pend_pb + ioff0 + rr * rowB},
+                                                 ptc[rr], cpend, lap, chim1[rr], lap_chi, jsum,
+                                                 ccv);
             }
             // Advance the queue.
 #pragma unroll
@@ -569,7 +579,7 @@ __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int 
     }
 }
 
-template <int L, int BJ, int RJ, bool NT>
+template <int L, int BJ, int RJ, bool NT, bool KEEP = false>
 __global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
     // XCD-aware block remap: blocks are dispatched round-robin over the 8
     // XCDs; this makes consecutive tiles of one i chunk share an XCD (and its
@@ -582,7 +592,7 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nk
     const int jt = rest % njt;
     const int ch = rest / njt;
     const int i0 = ch * CH;
-    tile_body<L, BJ, RJ, NT>(P, kp, kt * TK, jt * BJ * RJ, i0, min(i0 + CH, kp.N));
+    tile_body<L, BJ, RJ, NT, KEEP>(P, kp, kt * TK, jt * BJ * RJ, i0, min(i0 + CH, kp.N));
 }
 
 // Batched sweep: one launch advances many independent lattices.  Each
@@ -644,7 +654,7 @@ int choose_chunk(int N, int tiles, int min_ch = 8) {
     return ch < N ? ch : N;
 }
 
-template <int L, int BJ, int RJ, bool NT>
+template <int L, int BJ, int RJ, bool NT, bool KEEP = false>
 int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
     constexpr int TJ = BJ * RJ;
     const int N = kp.N;
@@ -653,8 +663,8 @@ int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
     const int CH = kp.chunk > 0 ? min(kp.chunk, N) : choose_chunk(N, nkt * njt, NT ? 8 : 2);
     const int nch = (N + CH - 1) / CH;
     const long long nblocks = (long long)nkt * njt * nch;
-    hipLaunchKernelGGL((step_tiled<L, BJ, RJ, NT>), dim3((unsigned)nblocks), dim3(64 * BJ), 0, st, P,
-                       kp, nkt, njt, CH);
+    hipLaunchKernelGGL((step_tiled<L, BJ, RJ, NT, KEEP>), dim3((unsigned)nblocks), dim3(64 * BJ), 0,
+                       st, P, kp, nkt, njt, CH);
     LFM_LAUNCH_CHECK("step_tiled");
     return LFM_OK;
 }
@@ -669,15 +679,22 @@ bool streaming(size_t sites) {
 
 template <int L>
 int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
+    const bool keep = (kp.flags & LFM_F_KEEP_PSI) != 0;
     if ((kp.flags & LFM_F_DIRECT) || kp.N > 1024) {
         const dim3 grid(unsigned((kp.N + 63) / 64), unsigned((kp.N + 3) / 4), unsigned(kp.N));
-        hipLaunchKernelGGL((step_direct<L>), grid, dim3(64, 4), 0, st, P, kp);
+        if (keep) hipLaunchKernelGGL((step_direct<L, true>), grid, dim3(64, 4), 0, st, P, kp);
+        else hipLaunchKernelGGL((step_direct<L>), grid, dim3(64, 4), 0, st, P, kp);
         LFM_LAUNCH_CHECK("step_direct");
         return LFM_OK;
     }
+    const bool nt = streaming<L>(P.n3);
+    if (keep) {
+        // the LFM_F_KEEP_PSI kernels come in one tile shape, {4 x 2}
+        return nt ? launch_tiled<L, 4, 2, true, true>(P, kp, st)
+                  : launch_tiled<L, 4, 2, false, true>(P, kp, st);
+    }
     // Tile shapes (wavefronts BJ x rows per thread RJ); 0 picks the level's default.
     const int tile = kp.tile > 0 ? kp.tile : 1;
-    const bool nt = streaming<L>(P.n3);
 #define LFM_TILES(NT)                                                  \
     switch (tile) {                                                    \
         case 1: return launch_tiled<L, 4, 1, NT>(P, kp, st);           \
@@ -1408,16 +1425,22 @@ extern "C" int lfm_step_real_f32(const float* E, const float* E_prev, const floa
         set_error("lfm_step_real_f32: LFM_F_MASK set but mask is NULL");
         return LFM_E_ARG;
     }
+    if ((p->flags & LFM_F_KEEP_PSI) && !p->keep) {
+        set_error("lfm_step_real_f32: LFM_F_KEEP_PSI set but params.keep is NULL");
+        return LFM_E_ARG;
+    }
     const size_t n3 = size_t(p->N) * p->N * p->N;
-    const float* ins[] = {E, E_prev, chi, chi_prev, mask};
-    const size_t nin[] = {n3, n3, n3, n3, n3};
+    const float* keep = (p->flags & LFM_F_KEEP_PSI) ? p->keep : nullptr;
+    const float* ins[] = {E, E_prev, chi, chi_prev, mask, keep};
+    const size_t nin[] = {n3, n3, n3, n3, n3, n3};
     float* outs[] = {E_out, E_prev_out, chi_out, chi_prev_out};
     const size_t nout[] = {n3, n3, n3, n3};
-    int rc = validate(p, ins, nin, 5, outs, nout, 4, "lfm_step_real_f32");
+    int rc = validate(p, ins, nin, 6, outs, nout, 4, "lfm_step_real_f32");
     if (rc) return rc;
     Ptrs P{};
     P.pr = E; P.prp = E_prev; P.chi = chi; P.chip = chi_prev;
     P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
+    P.keep = keep;
     P.pr_out = E_out; P.prp_out = E_prev_out; P.chi_out = chi_out; P.chip_out = chi_prev_out;
     P.n3 = n3;
     return launch<REAL>(P, to_k(p), (hipStream_t)stream);
@@ -1442,16 +1465,22 @@ extern "C" int lfm_step_complex_f32(const float* pr, const float* pr_prev, const
         set_error("lfm_step_complex_f32: pr_prev_out and pi_prev_out must both be given or both NULL");
         return LFM_E_ARG;
     }
+    if ((p->flags & LFM_F_KEEP_PSI) && !p->keep) {
+        set_error("lfm_step_complex_f32: LFM_F_KEEP_PSI set but params.keep is NULL");
+        return LFM_E_ARG;
+    }
     const size_t n3 = size_t(p->N) * p->N * p->N;
-    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask};
-    const size_t nin[] = {n3, n3, n3, n3, n3, n3, n3};
+    const float* keep = (p->flags & LFM_F_KEEP_PSI) ? p->keep : nullptr;
+    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask, keep};
+    const size_t nin[] = {n3, n3, n3, n3, n3, n3, n3, n3};
     float* outs[] = {pr_out, pr_prev_out, pi_out, pi_prev_out, chi_out, chi_prev_out};
     const size_t nout[] = {n3, n3, n3, n3, n3, n3};
-    int rc = validate(p, ins, nin, 7, outs, nout, 6, "lfm_step_complex_f32");
+    int rc = validate(p, ins, nin, 8, outs, nout, 6, "lfm_step_complex_f32");
     if (rc) return rc;
     Ptrs P{};
     P.pr = pr; P.pi = pi; P.prp = pr_prev; P.pip = pi_prev; P.chi = chi; P.chip = chi_prev;
     P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
+    P.keep = keep;
     P.pr_out = pr_out; P.pi_out = pi_out; P.prp_out = pr_prev_out; P.pip_out = pi_prev_out;
     P.chi_out = chi_out; P.chip_out = chi_prev_out;
     P.n3 = n3;
@@ -1459,8 +1488,8 @@ extern "C" int lfm_step_complex_f32(const float* pr, const float* pr_prev, const
 }
 
 static int step2_check(const lfm_params_t* p, const char* fn) {
-    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI)) {
-        set_error("%s: LFM_F_DIRECT / LFM_F_FREEZE_CHI not supported", fn);
+    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI | LFM_F_KEEP_PSI)) {
+        set_error("%s: LFM_F_DIRECT / LFM_F_FREEZE_CHI / LFM_F_KEEP_PSI not supported", fn);
         return LFM_E_ARG;
     }
     if (p->N > 1024) {
@@ -1566,17 +1595,23 @@ extern "C" int lfm_step_color_f32(const float* pr, const float* pr_prev, const f
                   "(the S_a fields themselves are not passed)");
         return LFM_E_ARG;
     }
+    if ((p->flags & LFM_F_KEEP_PSI) && !p->keep) {
+        set_error("lfm_step_color_f32: LFM_F_KEEP_PSI set but params.keep is NULL");
+        return LFM_E_ARG;
+    }
     const size_t n3 = size_t(p->N) * p->N * p->N;
-    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask, sa};
-    const size_t nin[] = {3 * n3, 3 * n3, 3 * n3, 3 * n3, n3, n3, n3, sa_scv ? n3 : 3 * n3};
+    const float* keep = (p->flags & LFM_F_KEEP_PSI) ? p->keep : nullptr;
+    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask, sa, keep};
+    const size_t nin[] = {3 * n3, 3 * n3, 3 * n3, 3 * n3, n3, n3, n3, sa_scv ? n3 : 3 * n3, n3};
     float* outs[] = {pr_out, pr_prev_out, pi_out, pi_prev_out, chi_out, chi_prev_out,
                      sa_clip_out, psisq_out};
     const size_t nout[] = {3 * n3, 3 * n3, 3 * n3, 3 * n3, n3, n3, 3 * n3, 3 * n3};
-    int rc = validate(p, ins, nin, 8, outs, nout, 8, "lfm_step_color_f32");
+    int rc = validate(p, ins, nin, 9, outs, nout, 8, "lfm_step_color_f32");
     if (rc) return rc;
     Ptrs P{};
     P.pr = pr; P.pi = pi; P.prp = pr_prev; P.pip = pi_prev; P.chi = chi; P.chip = chi_prev;
     P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
+    P.keep = keep;
     P.sa = p->kappa_tube > 0.0f ? sa : nullptr;
     P.pr_out = pr_out; P.pi_out = pi_out; P.prp_out = pr_prev_out; P.pip_out = pi_prev_out;
     P.chi_out = chi_out; P.chip_out = chi_prev_out;
@@ -1604,8 +1639,9 @@ extern "C" int lfm_step_color_fused_f32(const float* pr, const float* pr_prev, c
         set_error("lfm_step_color_fused_f32: needs kappa_tube > 0 (use lfm_step_color_f32)");
         return LFM_E_ARG;
     }
-    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI | LFM_F_SA_SCV)) {
-        set_error("lfm_step_color_fused_f32: LFM_F_DIRECT / LFM_F_FREEZE_CHI / LFM_F_SA_SCV not supported");
+    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI | LFM_F_SA_SCV | LFM_F_KEEP_PSI)) {
+        set_error("lfm_step_color_fused_f32: LFM_F_DIRECT / LFM_F_FREEZE_CHI / LFM_F_SA_SCV / "
+                  "LFM_F_KEEP_PSI not supported");
         return LFM_E_ARG;
     }
     ScvView v{};
@@ -1737,6 +1773,10 @@ extern "C" int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_p
     }
     if ((flags & LFM_F_SA_SCV) && P->level != 2) {
         set_error("lfm_batch_step_f32: LFM_F_SA_SCV is for COLOR batches");
+        return LFM_E_ARG;
+    }
+    if (flags & LFM_F_KEEP_PSI) {
+        set_error("lfm_batch_step_f32: LFM_F_KEEP_PSI is not supported");
         return LFM_E_ARG;
     }
     const size_t ns = size_t(P->total) * (P->level == 2 ? 3 : 1);

@@ -24,7 +24,8 @@ LFM_F_FREEZE_CHI = 0x04
 LFM_F_DIRECT = 0x08
 LFM_F_CHI_DELTA = 0x10
 LFM_F_SA_SCV = 0x20
-LFM_ABI_VERSION = 3
+LFM_F_KEEP_PSI = 0x40
+LFM_ABI_VERSION = 4
 LFM_TILE_VARIANTS = 6
 LFM_RELAX_SLOT_WORDS = 2048
 
@@ -81,7 +82,7 @@ class LfmParams(ctypes.Structure):
                 ("dt2", _f32), ("kappa", _f32), ("lambda_self", _f32), ("chi0", _f32),
                 ("e0_sq", _f32), ("epsilon_w", _f32), ("kappa_c", _f32),
                 ("epsilon_cc", _f32), ("kappa_string", _f32), ("kappa_tube", _f32),
-                ("tile", _i32), ("chunk", _i32)]
+                ("tile", _i32), ("chunk", _i32), ("keep", _vp)]
 
 
 class LfmError(RuntimeError):

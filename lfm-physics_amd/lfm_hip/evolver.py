@@ -156,6 +156,7 @@ class Evolver:
         else:
             self._sa = None
         self._params = {}
+        self._keep = None      # set_psi_keep: per-site factor folded into every step
         self.tile = 0      # tiled-kernel shape (0 = default, see autotune())
         self.chunk = 0     # i planes per workgroup (0 = automatic)
         self._partials = None
@@ -172,15 +173,40 @@ class Evolver:
 
     # ------------------------------------------------------------------ steps
     def _p(self, flags):
-        key = (flags, self.tile, self.chunk)
+        keep = self._keep.data_ptr() if self._keep is not None else 0
+        if keep:
+            flags |= _abi.LFM_F_KEEP_PSI
+        key = (flags, self.tile, self.chunk, keep)
         p = self._params.get(key)
         if p is None:
             c = self.config
             p = params(self.N, self._dt2, c.kappa, c.lambda_self, c.chi0, c.e0_sq, c.epsilon_w,
                        c.kappa_c, c.epsilon_cc, c.kappa_string,
                        c.kappa_tube if self.sa_enabled else 0.0, flags, self.tile, self.chunk)
+            p.keep = keep or None
             self._params[key] = p
         return p
+
+    def set_psi_keep(self, keep) -> None:
+        """Multiply every new Psi value by keep[site] inside each step (all
+        components; N^3 factors, or None to stop).  It is the in-place
+        `psi *= factor` an experiment driver applies to the current buffers
+        after every step (lfm/experiment/barrier.py:327-353: the barrier's
+        absorber and which-path detectors), done by the step kernel as it
+        writes Psi, bit for bit the same.  With set_chi(template) and
+        evolve(..., freeze_chi=True) it reproduces a barrier's per-step
+        enforcement (chi rewritten to the template in all four buffers, Psi
+        multiplied) without its per-step passes.  Steps then run one at a
+        time (no two-step launches) with the separate S_a transform."""
+        if keep is None:
+            self._keep = None
+        else:
+            t = torch.as_tensor(np.asarray(keep, dtype=np.float32) if not torch.is_tensor(keep)
+                                else keep, dtype=torch.float32, device=self.device)
+            if t.numel() != self.total:
+                raise ValueError(f"keep must hold N^3 = {self.total} values, got {t.numel()}")
+            self._keep = t.reshape(-1).contiguous().clone()
+        self._graphs.clear()
 
     def autotune(self, reps: int = 6, chunks=(0, 1, 2, 4, 8, 16, 32, 64, 128)) -> dict:
         """Time every tile shape and the given plane chunks (0: the library's
@@ -193,7 +219,7 @@ class Evolver:
         results = {}
         old = (self.tile, self.chunk, self.kernel_events)
         N = self.N
-        pairs = self.two_step and N <= 1024
+        pairs = self.two_step and N <= 1024 and self._keep is None
         if self._fused:
             # the fused COLOR kernel has one tile shape (a full k-row); tune its chunk
             combos = [(0, ch) for ch in dict.fromkeys((0, N, N // 2, N // 4, N // 8)) if ch == 0 or ch >= 4]
@@ -248,7 +274,8 @@ class Evolver:
             flags |= _abi.LFM_F_FREEZE_CHI
         if direct:
             flags |= _abi.LFM_F_DIRECT
-        separate_scv = self.level == 2 and self._sa_scv and not (self._fused and not (freeze_chi or direct))
+        fused = self._fused and not (freeze_chi or direct) and self._keep is None
+        separate_scv = self.level == 2 and self._sa_scv and not fused
         if separate_scv:
             flags |= _abi.LFM_F_SA_SCV
         p = ctypes.byref(self._p(flags))
@@ -269,7 +296,7 @@ class Evolver:
                                           pi[cur].data_ptr(), pi[prev].data_ptr(), chi_in,
                                           chip_in, mask, pr[nxt].data_ptr(), None,
                                           pi[nxt].data_ptr(), None, chi_out, None, p, st)
-        elif self._fused and not (freeze_chi or direct):
+        elif fused:
             cfg = self.config
             if not self._spec_valid:
                 _abi.check(lib.lfm_scv_prepare_f32(self._scv.handle, pr[cur].data_ptr(),
@@ -421,7 +448,7 @@ class Evolver:
             return None
         if self.level == 2 and self.sa_enabled and self._scv.method != 2:
             return None              # rocFFT launches stay out of graphs
-        pair = self.two_step and self.N <= 1024 and hook is None
+        pair = self.two_step and self.N <= 1024 and hook is None and self._keep is None
         key0 = self._key()
         k, m = key0, 0
         while True:
@@ -478,7 +505,8 @@ class Evolver:
         report = self.config.report_interval
         left = int(steps)
         hook = device_hook
-        pair_ok = self.two_step and not freeze_chi and self.N <= 1024 and hook is None
+        pair_ok = (self.two_step and not freeze_chi and self.N <= 1024 and hook is None
+                   and self._keep is None)
         if hook is not None:
             if freeze_chi:
                 raise ValueError("device_hook with freeze_chi is not supported")

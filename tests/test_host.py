@@ -55,7 +55,7 @@ def test_binding_arity_matches_header():
 def test_abi_version_and_error_string():
     from lfm_hip import _abi
     lib = _abi.load()
-    assert lib.lfm_abi_version() == _abi.LFM_ABI_VERSION == 3
+    assert lib.lfm_abi_version() == _abi.LFM_ABI_VERSION == 4
     # argument validation happens before any device work: N < 8 is refused
     p = _abi.LfmParams(4, 0, 4e-4, 1 / 63, 0, 19, 0, 0.1, 0, 0, 0, 0)
     dummy = [1 << 20] * 4
@@ -89,6 +89,20 @@ def test_abi_rejects_in_place_and_missing_inputs():
     rc = lib.lfm_step_real_f32(E, Ep, chi, chip, None, base + 16 * N ** 3, None,
                                base + 20 * N ** 3, None, ctypes.byref(p), None)
     assert rc == -1 and b"mask" in lib.lfm_last_error()
+    # keep flag without the factors; keep overlapping an output
+    p.flags = _abi.LFM_F_KEEP_PSI
+    rc = lib.lfm_step_real_f32(E, Ep, chi, chip, None, base + 16 * N ** 3, None,
+                               base + 20 * N ** 3, None, ctypes.byref(p), None)
+    assert rc == -1 and b"keep" in lib.lfm_last_error()
+    p.keep = base + 16 * N ** 3
+    rc = lib.lfm_step_real_f32(E, Ep, chi, chip, None, base + 16 * N ** 3, None,
+                               base + 20 * N ** 3, None, ctypes.byref(p), None)
+    assert rc == -1 and b"overlaps" in lib.lfm_last_error()
+    # not supported by the two-step kernel
+    rc = lib.lfm_step2_real_f32(E, Ep, chi, chip, None, base + 24 * N ** 3, base + 28 * N ** 3,
+                                base + 32 * N ** 3, base + 36 * N ** 3, ctypes.byref(p), None)
+    assert rc == -1 and b"KEEP" in lib.lfm_last_error()
+    p.keep = None
     # COLOR with kappa_tube but no S_a
     p.flags, p.kappa_tube = 0, 0.5
     n3 = N ** 3 * 4
@@ -172,13 +186,14 @@ def test_python_constants_match_header_defines():
 
 
 def test_params_struct_layout():
-    """LfmParams mirrors lfm_params_t: two int32, ten float32, two int32,
-    packed in declaration order (no padding)."""
+    """LfmParams mirrors lfm_params_t: two int32, ten float32, two int32 and
+    the keep pointer, in declaration order (no padding before the pointer:
+    14 four-byte fields put it on an 8-byte boundary)."""
     from lfm_hip import _abi
     names = [n for n, _ in _abi.LfmParams._fields_]
-    assert names[:2] == ["N", "flags"] and names[-2:] == ["tile", "chunk"]
-    assert ctypes.sizeof(_abi.LfmParams) == 14 * 4
-    assert _abi.LfmParams.tile.offset == 12 * 4
+    assert names[:2] == ["N", "flags"] and names[-3:] == ["tile", "chunk", "keep"]
+    assert ctypes.sizeof(_abi.LfmParams) == 14 * 4 + 8
+    assert _abi.LfmParams.tile.offset == 12 * 4 and _abi.LfmParams.keep.offset == 14 * 4
     text = open(HEADER).read()
     body = re.search(r"typedef struct lfm_params \{(.*?)\} lfm_params_t;", text, re.S).group(1)
     fields = re.findall(r"[A-Za-z_0-9]+(?=[,;])", body)
