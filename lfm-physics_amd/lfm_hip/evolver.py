@@ -409,12 +409,15 @@ class Evolver:
     def _key(self):
         return (self._cur, self._prev, self._cc, self._cp, self._oc, self._op, self._damped)
 
-    def _rotate(self, key, pair):
-        """The slot bookkeeping of one _step (pair=False, chi evolving) or one
-        _step2 (pair=True), applied to a state key without launching."""
+    def _rotate(self, key, pair, freeze=False):
+        """The slot bookkeeping of one _step (pair=False, chi evolving; with
+        freeze, chi frozen: the two chi reference sets swap) or one _step2
+        (pair=True), applied to a state key without launching."""
         cur, prev, cc, cp, oc, op, _ = key
         n = len(self._pr)
         free = lambda *u: next(q for q in range(n) if q not in u)
+        if freeze:
+            return (free(cur, prev), cur, oc, op, cc, cp, True)
         if pair:
             n1 = free(cur, prev)
             c1 = free(cc, cp)
@@ -433,7 +436,7 @@ class Evolver:
             self._step_dev.add_(1)
             hook(self, self._step_dev)
 
-    def _cycle_graph(self, cycles: int = 1, hook=None):
+    def _cycle_graph(self, cycles: int = 1, hook=None, freeze: bool = False):
         """(graph, launches, steps) replaying `cycles` whole rotation cycles
         from the current state, or None when the state is not steady or
         graphs are off.  A cycle is the shortest launch sequence that returns
@@ -444,22 +447,23 @@ class Evolver:
         parameters of every launch are fixed by it), cycle count and hook."""
         if not self.graphs or self.kernel_events is not None or not self._damped:
             return None
-        if self._cc == self._cp:
+        if self._cc == self._cp and not freeze:
             return None
         if self.level == 2 and self.sa_enabled and self._scv.method != 2:
             return None              # rocFFT launches stay out of graphs
-        pair = self.two_step and self.N <= 1024 and hook is None and self._keep is None
+        pair = (self.two_step and self.N <= 1024 and hook is None and self._keep is None
+                and not freeze)
         key0 = self._key()
         k, m = key0, 0
         while True:
-            k, m = self._rotate(k, pair), m + 1
+            k, m = self._rotate(k, pair, freeze), m + 1
             if k == key0 or m > 6:
                 break
         if k != key0:
             return None
         m *= cycles
         flags0 = self._sa_flags()
-        gkey = key0 + flags0 + (self.tile, self.chunk, pair, cycles, hook)
+        gkey = key0 + flags0 + (self.tile, self.chunk, pair, cycles, hook, freeze)
         if gkey in self._graphs:
             return self._graphs[gkey]
         while len(self._graphs) >= self.GRAPH_CACHE:
@@ -471,7 +475,12 @@ class Evolver:
         with torch.cuda.stream(s):
             with torch.cuda.graph(g, stream=s):
                 for _ in range(m):
-                    self._step2() if pair else self._hooked_step(hook)
+                    if pair:
+                        self._step2()
+                    elif freeze:
+                        self._step(freeze_chi=True)
+                    else:
+                        self._hooked_step(hook)
         torch.cuda.current_stream(self.device).wait_stream(s)
         # Capture recorded the launches without running them.  The bookkeeping
         # it advanced is back at the start (a whole cycle); when the S_a flags
@@ -495,7 +504,9 @@ class Evolver:
         COMPLEX run pairs of steps per launch when no callback falls between
         them; the result is the same as single steps.  Once the slot rotation
         is steady, whole rotation cycles are replayed from HIP graphs (same
-        launches, so the same bits as eager stepping).
+        launches, so the same bits as eager stepping).  With freeze_chi a
+        cycle is six steps: Psi rotates through three buffers while the two
+        chi reference sets swap every step.
 
         device_hook(evolver, step): called after every step with `step` a
         0-d int64 device tensor holding the step number.  It must only queue
@@ -515,12 +526,12 @@ class Evolver:
             self._step_dev.fill_(self.step)
         while left > 0:
             to_report = report - self.step % report if (callback is not None and report > 0) else left
-            cyc = None if freeze_chi else self._cycle_graph(hook=hook)
+            cyc = self._cycle_graph(hook=hook, freeze=freeze_chi)
             if cyc is not None and min(left, to_report) >= cyc[2]:
                 # long runs replay a graph of several cycles (fewer host
                 # launches per step), then single cycles
-                big = self._cycle_graph(self.GRAPH_CYCLES, hook) if min(left, to_report) >= \
-                    self.GRAPH_CYCLES * cyc[2] else None
+                big = self._cycle_graph(self.GRAPH_CYCLES, hook, freeze_chi) \
+                    if min(left, to_report) >= self.GRAPH_CYCLES * cyc[2] else None
                 for g, _, n in ((big or cyc), cyc):
                     while min(left, to_report) >= n:
                         g.replay()

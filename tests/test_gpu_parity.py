@@ -632,6 +632,49 @@ def test_evolver_graph_replay_equals_eager(gpu, level, N, extra):
         assert used >= 1
 
 
+@pytest.mark.parametrize("level,N,keep", [
+    ("REAL", 24, False), ("COMPLEX", 20, True), ("COLOR", 32, False), ("COLOR", 32, True)])
+def test_frozen_chi_graph_replay_equals_eager(gpu, level, N, keep):
+    """Frozen-chi runs replay six-step cycles (Psi rotates with period 3, the
+    two chi reference sets swap with period 2) from HIP graphs: the same bits
+    as eager launches, with and without a per-site Psi factor, after free
+    steps and before free steps again."""
+    import torch
+    import lfm_hip as L
+    log(f"frozen-chi graph replay vs eager: {level} N={N} keep={keep}")
+    lvl = getattr(L.FieldLevel, level)
+    if lvl == L.FieldLevel.COLOR:
+        cfg = L.full_physics(N)
+    else:
+        cfg = L.SimulationConfig(grid_size=N, field_level=lvl, lambda_self=4 / 31)
+    r, i = make_ics(int(lvl), N, seed=13)
+    fac = np.ones((N, N, N), np.float32)
+    fac[N // 2 - 1:N // 2 + 1] = 0.5
+    evs = {}
+    for g in (True, False):
+        ev = L.Evolver(cfg, backend="hip", graphs=g)
+        ev.set_psi_real(r)
+        if i is not None:
+            ev.set_psi_imag(i)
+        evs[g] = ev
+    used = 0
+    for n, frz, k in [(5, False, False), (31, True, keep), (4, True, keep), (9, False, False)]:
+        for ev in evs.values():
+            if k != (ev._keep is not None):
+                ev.set_psi_keep(fac if k else None)
+            ev.evolve(n, freeze_chi=frz)
+            torch.cuda.synchronize()
+        a, b = evs[True], evs[False]
+        used = max(used, sum(1 for key, v in a._graphs.items() if v is not None and key[-1] is True))
+        assert a.step == b.step
+        np.testing.assert_array_equal(a.get_psi_real(), b.get_psi_real())
+        np.testing.assert_array_equal(a.get_chi(), b.get_chi())
+        if i is not None:
+            np.testing.assert_array_equal(a.get_psi_imag(), b.get_psi_imag())
+    log(f"  frozen-chi graphs captured: {used}")
+    assert used >= 1
+
+
 def test_run_with_snapshots_and_native_hooks_match_oracle(gpu):
     """run_with_snapshots with a per-step hook written the way the
     reference's experiment drivers write theirs (experiment/barrier.py:295-330,
