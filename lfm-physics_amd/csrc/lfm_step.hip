@@ -53,6 +53,10 @@ struct KParams {
     int N, flags;
     float dt2, kappa, lam, chi0, e0sq, epsw, kc, ecc, ks, kt;
     int tile, chunk;
+    // kappa / chi0 and kappa_c / chi0, each one correctly rounded float32
+    // division done once on the host (the same value a device division with
+    // round-to-nearest gives), so no kernel divides per site
+    float kc0, kcc0;
 };
 
 struct Ptrs {
@@ -213,7 +217,7 @@ __device__ __forceinline__ void site_compute(const KParams& kp, const PointData<
     cn = chi_st;
     if (kp.flags & LFM_F_FREEZE_CHI) return;
 
-    const float kc0 = __fdiv_rn(kp.kappa, kp.chi0);
+    const float kc0 = kp.kc0;
     float psi_sq;
     float src;
     float cv = 0.0f, scv = 0.0f;
@@ -236,7 +240,7 @@ __device__ __forceinline__ void site_compute(const KParams& kp, const PointData<
                 if (total_sq > 1e-30f) {
                     const float sum_sq = fmaf(e[0], e[0], fmaf(e[1], e[1], __fmul_rn(e[2], e[2])));
                     const float f_c = __fsub_rn(__fdiv_rn(sum_sq, total_sq), 1.0f / 3.0f);
-                    cv = __fmul_rn(__fmul_rn(__fmul_rn(__fdiv_rn(kp.kc, kp.chi0), chi_c), f_c), psi_sq);
+                    cv = __fmul_rn(__fmul_rn(__fmul_rn(kp.kcc0, chi_c), f_c), psi_sq);
                 }
             }
             if (scv_on) {
@@ -642,13 +646,13 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled_batch(BatchArenas A,
     tile_body<L, BJ, RJ, NT>(P, kp, d.y, d.z, d.w, min(d.w + Lb.CH, Lb.N));
 }
 
-int choose_chunk(int N, int tiles, int min_ch = 8) {
-    // Aim for ~8192 workgroups (32 per CU on 256 CUs) so every CU keeps
+int choose_chunk(int N, int tiles, int min_ch = 8, int target = 8192) {
+    // Aim for ~target workgroups (8192: 32 per CU on 256 CUs) so every CU keeps
     // several tiles in flight, with at least min_ch planes per chunk: 8 when
     // the step streams from HBM (the two halo planes a chunk re-reads cost at
     // most a quarter of its traffic), fewer when the lattice's fields stay in
     // the last-level cache and a small lattice needs the extra workgroups.
-    const int nch = (8192 + tiles - 1) / tiles;
+    const int nch = (target + tiles - 1) / tiles;
     int ch = (N + nch - 1) / nch;
     if (ch < min_ch) ch = min_ch;
     return ch < N ? ch : N;
@@ -1039,21 +1043,34 @@ int launch_kfft_any(const Ptrs& P, const KParams& kp, const ScvView& v, hipStrea
 // ---------------------------------------------------------------------------
 // Two leapfrog steps per launch (REAL and COMPLEX).
 //
-// A workgroup owns a 64 (k) x T2J (j) tile and marches along i.  For each
-// input plane it stages Psi_n and chi_n on the tile plus a two-cell ring,
-// computes step n -> n+1 on the tile plus a one-cell ring (the ring values
-// are what step n+1 -> n+2 needs as neighbours), keeps those in a second
-// three-plane window, and computes step n+1 -> n+2 on the tile one plane
-// behind.  Step n+1 reads Psi_n as its (damped) prev and chi_n as its chi
-// prev straight from the first window.  Per site the two steps read
-// Psi_n, Psi_{n-1}, chi_n, chi_{n-1} and the mask once and write Psi and chi
-// of both steps (the rotation keeps step n+1 as the next prev): 52 B for two
-// COMPLEX steps with mask instead of 2 x 40 B.
+// A workgroup owns a 64 (k) x 8 (j) tile and marches along i.  Its
+// wavefronts take one of two roles, which run side by side between one
+// barrier per plane:
+//  * 11 first-step wavefronts compute step n -> n+1 on the tile plus a
+//    one-cell ring (660 positions, one per lane, each with its own
+//    plane-march queue).  Fed plane q of Psi_n / chi_n from LDS window 0,
+//    they finish plane q-1: the new Psi and chi go to LDS window 1 and, on
+//    the tile, to global memory; Psi_n, chi_n and the mask of the tile's
+//    sites go to a three-plane buffer as the second step's pointwise inputs.
+//  * 4 second-step wavefronts compute step n+1 -> n+2 on the tile, two rows
+//    per lane.  Fed plane q-2 of window 1 (written one plane earlier), they
+//    finish plane q-3.
+// Every lane also stages plane q+1 of Psi_n and chi_n on the tile plus a
+// two-cell ring (816 positions) into the other half of window 0.  Both
+// windows are double-buffered, so the one barrier per plane orders every
+// write before its reads.  Each role is its own loop and keeps only its own
+// queues in registers.
 //
-// Both steps use site_compute, and the Laplacian and current are summed in
-// the same order as in tile_body, so the results equal two single steps.
+// Per site the two steps read Psi_n, Psi_{n-1}, chi_n, chi_{n-1} and the
+// mask once and write Psi and chi of both steps (the rotation keeps step
+// n+1 as the next prev): 52 B for two COMPLEX steps with mask instead of
+// 2 x 40 B.  Both steps use site_compute, and the Laplacian and current are
+// summed in the same order as in tile_body, so the results equal two single
+// steps bit for bit.
 // ---------------------------------------------------------------------------
-constexpr int T2K = 64, T2J = 8, T2T = 256;   // tile and workgroup size
+constexpr int T2K = 64, T2J = 8;                 // tile
+constexpr int T2W1 = 11, T2W2 = 4;               // wavefronts of each role
+constexpr int T2T = 64 * (T2W1 + T2W2);          // 960 lanes per workgroup
 
 struct Ptrs2 {
     Ptrs P;            // inputs, and step n+1 outputs in pr_out / pi_out / chi_out
@@ -1068,14 +1085,19 @@ struct Win2 {
     static constexpr int NF = NPF + 1;                 // + chi
     static constexpr int R0 = T2J + 4, C0 = T2K + 4;  // two-cell ring
     static constexpr int R1 = T2J + 2, C1 = T2K + 2;  // one-cell ring
+    static_assert(R0 * C0 <= T2T, "one staging position per lane");
+    static_assert(R1 * C1 <= 64 * T2W1, "one first-step position per lane");
+    static_assert(T2J == 2 * T2W2, "two second-step rows per lane");
     // Psi (float2 for COMPLEX, float for REAL) and chi in separate arrays:
     // lanes read consecutive 8- or 4-byte slots, free of bank conflicts.
     using VP = std::conditional_t<NPF == 2, float2, float>;
-    VP p0[3][R0][C0];       // window 0: Psi_n
-    float c0[3][R0][C0];    //           chi_n
-    VP p1[3][R1][C1];       // window 1: Psi_{n+1}
-    float c1[3][R1][C1];    //           chi_{n+1}
-    float m1[3][R1][C1];    //           mask
+    VP p0[2][R0][C0];         // window 0: Psi_n
+    float c0[2][R0][C0];      //           chi_n
+    VP p1[2][R1][C1];         // window 1: Psi_{n+1}
+    float c1[2][R1][C1];      //           chi_{n+1}
+    VP pp[3][T2J][T2K];       // second step's pointwise inputs on the tile:
+    float cp[3][T2J][T2K];    //   Psi_n, chi_n and the mask
+    float mp[3][T2J][T2K];
 };
 
 __device__ __forceinline__ float fld(const float2& v, int f) { return f == 0 ? v.x : v.y; }
@@ -1174,11 +1196,9 @@ __device__ __forceinline__ void tile2_body(const Ptrs2& P2, const KParams& kp, i
     using VP = typename Wn::VP;
     constexpr int NPF = Lv<L>::NPF;
     constexpr int NF = Wn::NF;
-    constexpr int P0 = Wn::R0 * Wn::C0;           // staged positions of window 0
-    constexpr int P1 = Wn::R1 * Wn::C1;           // step n+1 positions
-    constexpr int U0 = (P0 + T2T - 1) / T2T;
-    constexpr int U1 = (P1 + T2T - 1) / T2T;
-    constexpr int RB = T2J / (T2T / 64);          // step n+2 rows per thread (2)
+    constexpr int C0 = Wn::C0, C1 = Wn::C1;
+    constexpr int P0 = Wn::R0 * C0;           // staged positions of window 0
+    constexpr int P1 = Wn::R1 * C1;           // first-step positions
     __shared__ Wn S;
     const Ptrs& P = P2.P;
     const int N = kp.N;
@@ -1193,120 +1213,134 @@ __device__ __forceinline__ void tile2_body(const Ptrs2& P2, const KParams& kp, i
 #pragma unroll
     for (int f = 0; f < NPF; ++f) src[f] = field<L>(P.pr, P.pi, f, n3);
     src[NPF] = P.chi;
-    // window-0 staging positions (two-cell ring around the tile)
-    uint32_t o0[U0];
-#pragma unroll
-    for (int u = 0; u < U0; ++u) {
-        const int p = min(t + u * T2T, P0 - 1);
-        o0[u] = uint32_t(wrap(j0 - 2 + p / Wn::C0)) * rowB + uint32_t(wrap(k0 - 2 + p % Wn::C0)) * 4u;
-    }
-    // step n+1 positions (one-cell ring), fixed per thread across planes
-    uint32_t o1[U1];
-    int r1s[U1], c1s[U1];
-#pragma unroll
-    for (int u = 0; u < U1; ++u) {
-        const int p = min(t + u * T2T, P1 - 1);
-        r1s[u] = p / Wn::C1;
-        c1s[u] = p % Wn::C1;
-        o1[u] = uint32_t(wrap(j0 - 1 + r1s[u])) * rowB + uint32_t(wrap(k0 - 1 + c1s[u])) * 4u;
-    }
-    float pre[U0][NF];
+    // this lane's staging position (two-cell ring around the tile); lanes
+    // past the last position load a valid address and discard it
+    const bool stg = t < P0;
+    const int ps = min(t, P0 - 1);
+    const int sr = ps / C0, sc = ps % C0;
+    const uint32_t o0 = uint32_t(wrap(j0 - 2 + sr)) * rowB + uint32_t(wrap(k0 - 2 + sc)) * 4u;
+    float pre[NF];
     auto prefetch = [&](int q) {
         const uint32_t pb = uint32_t(wrap(q)) * planeB;
 #pragma unroll
-        for (int u = 0; u < U0; ++u)
-#pragma unroll
-            for (int f = 0; f < NF; ++f) pre[u][f] = Off32{pb + o0[u]}.ld(src[f]);
+        for (int f = 0; f < NF; ++f) pre[f] = Off32{pb + o0}.ld(src[f]);
     };
-    // pointwise inputs of step n -> n+1 at the step positions of plane q
-    PointData<L> pa[U1], pn[U1];
-    auto prefetch_pt = [&](int q) {
-        const uint32_t pb = uint32_t(wrap(q)) * planeB;
+    auto stage = [&](int slot) {
+        if (stg) {
+            float pv[NPF];
 #pragma unroll
-        for (int u = 0; u < U1; ++u) pn[u] = load_point<L>(kp, P, Off32T<NT>{pb + o1[u]});
+            for (int f = 0; f < NPF; ++f) pv[f] = pre[f];
+            put(S.p0[slot][sr][sc], pv);
+            S.c0[slot][sr][sc] = pre[NPF];
+        }
     };
-    const bool has_mask = (kp.flags & LFM_F_MASK) != 0;
-    KParams kpB = kp;                      // step n+1: its prev is Psi_n, read damped
-    if (has_mask) kpB.flags |= LFM_F_DAMP_PREV;
-    const int tx = t & 63, ty = t >> 6;
-    Queue2<L> qa[U1], qb[RB];
-#pragma unroll
-    for (int u = 0; u < U1; ++u) qa[u].clear();
-#pragma unroll
-    for (int rr = 0; rr < RB; ++rr) qb[rr].clear();
-
-    const int cnt = (i1 - i0) + 4;         // input planes i0-2 .. i1+1
+    // iterations: the first step is fed input planes i0-2 .. i1+1 (iterations
+    // 0 .. cnt-2), the second step window-1 planes i0-1 .. i1 (3 .. cnt-1)
+    const int cnt = (i1 - i0) + 5;
+    // at iteration s the lanes stage plane q+1 and load plane q+2 (q = i0-2+s)
+    auto stage_next = [&](int s) {
+        if (s <= cnt - 3) {
+            stage((s + 1) & 1);
+            if (s <= cnt - 4) prefetch(i0 + s);
+        }
+    };
     prefetch(i0 - 2);
-    for (int s = 0; s < cnt; ++s) {
-        const int q = i0 - 2 + s;
-        const int sq = s % 3;              // window-0 slot of plane q (window 1 alike)
-        __syncthreads();
+    stage(0);
+    prefetch(i0 - 1);
+    __syncthreads();
+
+    if ((t >> 6) < T2W1) {
+        // ---- first step: n -> n+1 on the tile plus a one-cell ring ----
+        const bool act = t < P1;
+        const int p1 = min(t, P1 - 1);
+        const int r1 = p1 / C1, c1 = p1 % C1;
+        const uint32_t o1 = uint32_t(wrap(j0 - 1 + r1)) * rowB + uint32_t(wrap(k0 - 1 + c1)) * 4u;
+        const int rt = r1 - 1, ct = c1 - 1;   // tile coordinates
+        const bool on_tile = act && rt >= 0 && rt < T2J && ct >= 0 && ct < T2K;
+        const bool in_lat = on_tile && rt < nj && ct < nk;
+        const bool has_mask = (kp.flags & LFM_F_MASK) != 0;
+        PointData<L> pa{}, pn{};
+        Queue2<L> qa;
+        qa.clear();
+        for (int s = 0; s < cnt; ++s) {
+            const int q = i0 - 2 + s;
+            const int cur = s & 1;
+            stage_next(s);
+            if (s <= cnt - 2) {
+                if (s >= 2) pa = pn;
+                // site q is finished next iteration: its pointwise inputs now
+                if (s >= 1 && q <= i1) pn = load_point<L>(kp, P, Off32T<NT>{uint32_t(wrap(q)) * planeB + o1});
+                if (act) {
+                    float A[NF], B[NF], ctr[NF], dj[NF], dk[NF], nw[NPF], cn = 0.0f;
+                    plane_parts<NF>(S.p0[cur], S.c0[cur], r1 + 1, c1 + 1, A, B, ctr, dj, dk);
+                    // Psi_n and chi_n of the pending site (plane q-1)
+                    float cpend[NPF];
 #pragma unroll
-        for (int u = 0; u < U0; ++u) {
-            const int p = t + u * T2T;
-            if (p < P0) {
-                float pv[NPF];
+                    for (int f = 0; f < NPF; ++f) cpend[f] = qa.cm1[f];
+                    const float chpend = qa.chim1;
+                    qa.feed(kp, A, B, ctr, dj, dk, s >= 2, pa, nw, cn);
+                    if (s >= 2) {
+                        put(S.p1[cur][r1][c1], nw);
+                        S.c1[cur][r1][c1] = cn;
+                        if (on_tile) {
+                            const int sl = s % 3;
+                            put(S.pp[sl][rt][ct], cpend);
+                            S.cp[sl][rt][ct] = chpend;
+                            S.mp[sl][rt][ct] = has_mask ? pa.m : 0.0f;
+                            if (in_lat && q - 1 >= i0 && q - 1 < i1) {
+                                const Off32T<NT> o{uint32_t(q - 1) * planeB + o1};
 #pragma unroll
-                for (int f = 0; f < NPF; ++f) pv[f] = pre[u][f];
-                put(S.p0[sq][p / Wn::C0][p % Wn::C0], pv);
-                S.c0[sq][p / Wn::C0][p % Wn::C0] = pre[u][NPF];
-            }
-        }
-        __syncthreads();
-        if (s + 1 < cnt) prefetch(q + 1);
-        if (s >= 2) {
-            // pointwise inputs of plane q-1, loaded last iteration
-#pragma unroll
-            for (int u = 0; u < U1; ++u) pa[u] = pn[u];
-        }
-        // planes i0-1 .. i1 are centres of step n -> n+1; plane q is one next iteration
-        if (s >= 1) prefetch_pt(q);
-        // step n -> n+1: feed plane q, finish plane q-1 (written to window 1)
-        const int s1 = (s + 2) % 3;        // window-1 slot of plane q-1
-        const bool own = q - 1 >= i0 && q - 1 < i1;
-#pragma unroll
-        for (int u = 0; u < U1; ++u) {
-            const int p = t + u * T2T;
-            float A[NF], B[NF], ctr[NF], dj[NF], dk[NF], nw[NPF], cn = 0.0f;
-            plane_parts<NF>(S.p0[sq], S.c0[sq], r1s[u] + 1, c1s[u] + 1, A, B, ctr, dj, dk);
-            qa[u].feed(kp, A, B, ctr, dj, dk, s >= 2, pa[u], nw, cn);
-            if (s >= 2 && p < P1) {
-                put(S.p1[s1][r1s[u]][c1s[u]], nw);
-                S.c1[s1][r1s[u]][c1s[u]] = cn;
-                S.m1[s1][r1s[u]][c1s[u]] = has_mask ? pa[u].m : 0.0f;
-                if (own && r1s[u] >= 1 && r1s[u] <= nj && c1s[u] >= 1 && c1s[u] <= nk) {
-                    const Off32T<NT> o{uint32_t(q - 1) * planeB + o1[u]};
-#pragma unroll
-                    for (int f = 0; f < NPF; ++f) o.st(field_out<L>(P.pr_out, P.pi_out, f, n3), nw[f]);
-                    o.st(P.chi_out, cn);
+                                for (int f = 0; f < NPF; ++f)
+                                    o.st(field_out<L>(P.pr_out, P.pi_out, f, n3), nw[f]);
+                                o.st(P.chi_out, cn);
+                            }
+                        }
+                    }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
-        if (s >= 2) {
-            // step n+1 -> n+2: feed window-1 plane q-1, finish plane q-2
-            const int sb = (s + 1) % 3;    // window-0 and window-1 slot of plane q-2
-            const bool fin = s >= 4;
+    } else {
+        // ---- second step: n+1 -> n+2 on the tile, rows r0, r0+1 ----
+        const int tx = t & 63, r0 = 2 * ((t >> 6) - T2W1);
+        KParams kpB = kp;                      // its prev is Psi_n, read damped
+        if (kp.flags & LFM_F_MASK) kpB.flags |= LFM_F_DAMP_PREV;
+        Queue2<L> qb[2];
+        qb[0].clear();
+        qb[1].clear();
+        for (int s = 0; s < cnt; ++s) {
+            const int q = i0 - 2 + s;
+            stage_next(s);
+            if (s >= 3) {
+                // window 1 slot written last iteration holds plane q-2; the
+                // pointwise inputs of plane q-3 were written two iterations ago
+                const int w1 = (s - 1) & 1, sl = (s + 1) % 3;
+                const bool fin = s >= 5;
 #pragma unroll
-            for (int rr = 0; rr < RB; ++rr) {
-                const int r = ty * RB + rr;
-                float A[NF], B[NF], ctr[NF], dj[NF], dk[NF], nw[NPF], cn = 0.0f;
-                plane_parts<NF>(S.p1[s1], S.c1[s1], r + 1, tx + 1, A, B, ctr, dj, dk);
-                PointData<L> pd;
-                const VP z = S.p0[sb][r + 2][tx + 2];
+                for (int rr = 0; rr < 2; ++rr) {
+                    const int r = r0 + rr;
+                    float A[NF], B[NF], ctr[NF], dj[NF], dk[NF], nw[NPF], cn = 0.0f;
+                    plane_parts<NF>(S.p1[w1], S.c1[w1], r + 1, tx + 1, A, B, ctr, dj, dk);
+                    PointData<L> pd{};
+                    if (fin) {
+                        const VP z = S.pp[sl][r][tx];
 #pragma unroll
-                for (int f = 0; f < NPF; ++f) pd.prev[f] = fld(z, f);
-                pd.chip = S.c0[sb][r + 2][tx + 2];
-                pd.m = S.m1[sb][r + 1][tx + 1];
-                qb[rr].feed(kpB, A, B, ctr, dj, dk, fin, pd, nw, cn);
-                if (fin && r < nj && tx < nk) {
-                    const Off32T<NT> o{uint32_t(q - 2) * planeB + uint32_t(j0 + r) * rowB +
-                                       uint32_t(k0 + tx) * 4u};
+                        for (int f = 0; f < NPF; ++f) pd.prev[f] = fld(z, f);
+                        pd.chip = S.cp[sl][r][tx];
+                        pd.m = S.mp[sl][r][tx];
+                    }
+                    qb[rr].feed(kpB, A, B, ctr, dj, dk, fin, pd, nw, cn);
+                    if (fin && r < nj && tx < nk) {
+                        const Off32T<NT> o{uint32_t(q - 3) * planeB + uint32_t(j0 + r) * rowB +
+                                           uint32_t(k0 + tx) * 4u};
 #pragma unroll
-                    for (int f = 0; f < NPF; ++f) o.st(field_out<L>(P2.pr_out2, P2.pi_out2, f, n3), nw[f]);
-                    o.st(P2.chi_out2, cn);
+                        for (int f = 0; f < NPF; ++f)
+                            o.st(field_out<L>(P2.pr_out2, P2.pi_out2, f, n3), nw[f]);
+                        o.st(P2.chi_out2, cn);
+                    }
                 }
             }
+            __syncthreads();
         }
     }
 }
@@ -1326,7 +1360,9 @@ template <int L, bool NT>
 int launch_step2_t(const Ptrs2& P2, const KParams& kp, hipStream_t st) {
     const int N = kp.N;
     const int nkt = (N + T2K - 1) / T2K, njt = (N + T2J - 1) / T2J;
-    const int CH = kp.chunk > 0 ? min(kp.chunk, N) : choose_chunk(N, nkt * njt, NT ? 8 : 2);
+    // one 960-lane workgroup per CU: ~2048 workgroups, and chunks of at least
+    // 8 planes (each chunk also feeds 4 planes around it)
+    const int CH = kp.chunk > 0 ? min(kp.chunk, N) : choose_chunk(N, nkt * njt, 8, 2048);
     const int nch = (N + CH - 1) / CH;
     const long long nblocks = (long long)nkt * njt * nch;
     hipLaunchKernelGGL((step2_tiled<L, NT>), dim3((unsigned)nblocks), dim3(T2T), 0, st, P2, kp, nkt,
@@ -1379,6 +1415,9 @@ KParams to_k(const lfm_params_t* p) {
     k.chi0 = p->chi0; k.e0sq = p->e0_sq; k.epsw = p->epsilon_w; k.kc = p->kappa_c;
     k.ecc = p->epsilon_cc; k.ks = p->kappa_string; k.kt = p->kappa_tube;
     k.tile = p->tile; k.chunk = p->chunk;
+    // IEEE single division, round to nearest: bit-equal to __fdiv_rn
+    k.kc0 = p->kappa / p->chi0;
+    k.kcc0 = p->kappa_c / p->chi0;
     return k;
 }
 
