@@ -1262,9 +1262,9 @@ __device__ __forceinline__ void tile2_body(const Ptrs2& P2, const KParams& kp, i
         PointData<L> pa{}, pn{};
         Queue2<L> qa;
         qa.clear();
-        for (int s = 0; s < cnt; ++s) {
+        // one plane; cur = s & 1, passed as a constant by the loop below
+        auto iter = [&](const int s, const int cur) {
             const int q = i0 - 2 + s;
-            const int cur = s & 1;
             stage_next(s);
             if (s <= cnt - 2) {
                 if (s >= 2) pa = pn;
@@ -1299,6 +1299,11 @@ __device__ __forceinline__ void tile2_body(const Ptrs2& P2, const KParams& kp, i
                 }
             }
             __syncthreads();
+        };
+        // two planes per trip, so the window halves are fixed in each copy
+        for (int s = 0; s < cnt; s += 2) {
+            iter(s, 0);
+            if (s + 1 < cnt) iter(s + 1, 1);
         }
     } else {
         // ---- second step: n+1 -> n+2 on the tile, rows r0, r0+1 ----
@@ -1308,13 +1313,13 @@ __device__ __forceinline__ void tile2_body(const Ptrs2& P2, const KParams& kp, i
         Queue2<L> qb[2];
         qb[0].clear();
         qb[1].clear();
-        for (int s = 0; s < cnt; ++s) {
+        auto iter = [&](const int s, const int cur) {
             const int q = i0 - 2 + s;
             stage_next(s);
             if (s >= 3) {
                 // window 1 slot written last iteration holds plane q-2; the
                 // pointwise inputs of plane q-3 were written two iterations ago
-                const int w1 = (s - 1) & 1, sl = (s + 1) % 3;
+                const int w1 = cur ^ 1, sl = (s + 1) % 3;
                 const bool fin = s >= 5;
 #pragma unroll
                 for (int rr = 0; rr < 2; ++rr) {
@@ -1341,6 +1346,10 @@ __device__ __forceinline__ void tile2_body(const Ptrs2& P2, const KParams& kp, i
                 }
             }
             __syncthreads();
+        };
+        for (int s = 0; s < cnt; s += 2) {
+            iter(s, 0);
+            if (s + 1 < cnt) iter(s + 1, 1);
         }
     }
 }
