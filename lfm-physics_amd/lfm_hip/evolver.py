@@ -95,10 +95,11 @@ class Evolver:
         # two_step: REAL / COMPLEX advance two steps per launch (lfm_step2_*,
         # bit-identical to single steps), which needs a fourth rotation slot:
         # both outputs must differ from both inputs.  Off by default: measured
-        # on MI355X it moves fewer HBM bytes (COMPLEX 512^3: 29.3 against 38.2
-        # B per site and step) but is not faster (1.04 vs 0.92 ms per step;
-        # REAL 64^3 slower still, too few workgroups): its ring recomputation,
-        # 254 VGPRs and 61 KB of LDS hold it at 2 waves/SIMD.
+        # on MI355X it moves fewer HBM bytes (COMPLEX 512^3: 26.3 against 39.1
+        # B per site and step) but is not faster (0.86-0.90 against 0.85-0.92
+        # ms per step; slower below 128^3, where a chunk's four extra planes
+        # and the few workgroups dominate): its ring recomputation leaves it
+        # latency-bound at 15 waves per CU.
         self.two_step = bool(two_step) and lvl != FieldLevel.COLOR
         ns = 4 if self.two_step else 3
         self._pr = [torch.zeros(n, dtype=f32, device=dev) for _ in range(ns)]
