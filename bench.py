@@ -268,7 +268,7 @@ def run_batch(args):
     steps = args.steps
     sites = sum(c.grid_size ** 3 for c in cfgs)
     batch = L.LatticeBatch(cfgs, backend="hip", tile=0 if args.tile == "auto" else int(args.tile),
-                           chunk=args.chunk)
+                           chunk=args.chunk, two_step=args.two_step)
 
     def reset_batch():
         for b, (r, i) in enumerate(ics):
@@ -295,11 +295,12 @@ def run_batch(args):
     for _ in range(12):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        batch._step()
+        batch._step2() if args.two_step else batch._step()
         e1.record()
         evs.append((e0, e1))
     torch.cuda.synchronize()
-    kms = sorted(a.elapsed_time(b) for a, b in evs[2:])[len(evs[2:]) // 2]
+    # per step (a two-step launch advances every lattice twice)
+    kms = sorted(a.elapsed_time(b) for a, b in evs[2:])[len(evs[2:]) // 2] / (2 if args.two_step else 1)
     ok = all(bool(torch.isfinite(t).all()) for t in batch._chi)
     del batch
     torch.cuda.empty_cache()
@@ -330,14 +331,17 @@ def run_batch(args):
             same = bool(np.array_equal(evols[0].get_chi(), chi_b0))
         del evols, ev
         torch.cuda.empty_cache()
-    b_alg = 40   # COMPLEX with mask: 7 reads + 3 writes of float32
+    # COMPLEX with mask: 7 reads + 3 writes of float32 per step; two steps per
+    # launch read the 7 once and write both steps' 3 (26 B per step)
+    b_alg = 26 if args.two_step else 40
     achieved = b_alg * sites / (kms * 1e-3) / 1e9
     out = {"metric": METRIC, "value": nlat / t_batch, "unit": "lattices/s", "n_gpus": 1,
            "steps": steps, "warmup": args.warmup, "ms_per_step": t_batch * 1e3 / steps,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic",
            "config": {"workload": "batched sweep: 64 COMPLEX lattices, N in {32,48,64,96,128}, "
-                                  f"{steps} steps, mask on", "lattices": nlat, "sites": sites},
+                                  f"{steps} steps, mask on", "lattices": nlat, "sites": sites,
+                      "steps_per_launch": 2 if args.two_step else 1},
            "batched": {"lattices_per_s": nlat / t_batch, "GLUPS": sites * steps / t_batch / 1e9,
                        "wall_s": t_batch, "tile": batch_tile(tuning)},
            "sequential_graphs": {"lattices_per_s": nlat / t_seq[True],
@@ -350,7 +354,8 @@ def run_batch(args):
            "speedup_vs_sequential": min(t_seq.values()) / t_batch,
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                        "kernel": "step_tiled_batch<COMPLEX>", "kernel_ms": kms,
+                        "kernel": "step2_tiled_batch<COMPLEX>" if args.two_step else "step_tiled_batch<COMPLEX>",
+                        "kernel_ms": kms,
                         "bytes_per_site": b_alg},
            "finite": ok, "batch_equals_sequential_lattice0": same}
     return out
@@ -506,7 +511,8 @@ def main():
     ap.add_argument("--tile", default="auto", help="tile shape 1..6, 0 default, or auto (tune)")
     ap.add_argument("--chunk", type=int, default=0, help="i planes per workgroup (0 auto)")
     ap.add_argument("--two-step", action="store_true",
-                    help="REAL / COMPLEX: two steps per launch (lfm_step2_*)")
+                    help="REAL / COMPLEX and batch64: two steps per launch (lfm_step2_*, "
+                         "lfm_batch_step2_f32)")
     ap.add_argument("--sa-transform", default="auto", choices=["auto", "fused", "separate"],
                     help="COLOR S_a: k-axis passes fused into the step kernel, or five separate passes")
     args = ap.parse_args()
@@ -679,7 +685,7 @@ def main():
         # the metric's other two clauses: batched sweep and adjoint speedups
         # against plain runs, at their own configurations
         out["batched"] = _summary_batch(run_batch(argparse.Namespace(
-            steps=500, warmup=args.warmup, tile="auto", chunk=0)))
+            steps=500, warmup=args.warmup, tile="auto", chunk=0, two_step=False)))
         torch.cuda.empty_cache()
         out["adjoint"] = _summary_adjoint(run_adjoint(argparse.Namespace(steps=400, size=None)))
         torch.cuda.empty_cache()

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define LFM_ABI_VERSION 4
+#define LFM_ABI_VERSION 5
 
 /* Error codes. */
 #define LFM_OK 0
@@ -225,6 +225,18 @@ int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_prev, const 
                        const float* pi_prev, const float* chi, const float* chi_prev,
                        const float* mask, float* pr_out, float* pi_out, float* chi_out,
                        const float* sa, float* psisq_out, int32_t flags, void* stream);
+/* Two steps of every lattice of a REAL or COMPLEX batch in one launch (the
+ * batched form of lfm_step2_real_f32 / lfm_step2_complex_f32, bit-identical
+ * to two lfm_batch_step_f32 calls): step n+1 into *_out1, step n+2 into
+ * *_out2.  The plan's column tiles must be 64 x 8 (tile variants 2 and 4);
+ * flags: LFM_F_DAMP_PREV (for step n+1; step n+2 always reads Psi_n damped
+ * where a lattice has its mask) and LFM_F_CHI_DELTA.  Outputs must not alias
+ * inputs or each other (a four-slot rotation). */
+int lfm_batch_step2_f32(void* plan, const float* pr, const float* pr_prev, const float* pi,
+                        const float* pi_prev, const float* chi, const float* chi_prev,
+                        const float* mask, float* pr_out1, float* pi_out1, float* chi_out1,
+                        float* pr_out2, float* pi_out2, float* chi_out2, int32_t flags,
+                        void* stream);
 
 /* Reverse-mode (adjoint) sweep of the COMPLEX leapfrog, for gradients of a
  * loss of the final state.  It has no counterpart in the reference (which

@@ -1386,6 +1386,39 @@ int launch_step2(const Ptrs2& P2, const KParams& kp, hipStream_t st) {
                                  : launch_step2_t<L, false>(P2, kp, st);
 }
 
+// Batched two steps per launch: the descriptors of a batch plan whose
+// column tiles are 64 x 8 (tile variants 2 and 4), one 960-lane workgroup per
+// (lattice, tile, plane chunk), tile2_body on the lattice's arena spans.  The
+// arenas are the single step's (outputs of step n+1 in pr_out / pi_out /
+// chi_out) plus the outputs of step n+2.
+struct BatchArenas2 {
+    BatchArenas a;
+    float* pr_out2; float* pi_out2; float* chi_out2;
+};
+
+template <int L, bool NT>
+__global__ __launch_bounds__(T2T) void step2_tiled_batch(BatchArenas2 A2,
+                                                         const BatchLattice* __restrict__ lat,
+                                                         const int4* __restrict__ desc,
+                                                         int flags_or) {
+    const int4 d = desc[blockIdx.x];
+    if (d.x < 0) return;   // padding descriptor (whole workgroup exits together)
+    const BatchLattice& Lb = lat[d.x];
+    KParams kp = Lb.kp;
+    kp.flags |= flags_or;
+    const BatchArenas& A = A2.a;
+    const long long o = Lb.off;
+    Ptrs2 P2{};
+    Ptrs& P = P2.P;
+    P.pr = A.pr + o; P.prp = A.prp + o; P.chi = A.chi + o; P.chip = A.chip + o;
+    if (A.pi) { P.pi = A.pi + o; P.pip = A.pip + o; P.pi_out = A.pi_out + o; P2.pi_out2 = A2.pi_out2 + o; }
+    P.mask = (kp.flags & LFM_F_MASK) ? A.mask + o : nullptr;
+    P.pr_out = A.pr_out + o; P.chi_out = A.chi_out + o;
+    P2.pr_out2 = A2.pr_out2 + o; P2.chi_out2 = A2.chi_out2 + o;
+    P.n3 = size_t(Lb.N) * Lb.N * Lb.N;
+    tile2_body<L, NT>(P2, kp, d.y, d.z, d.w, min(d.w + Lb.CH, Lb.N));
+}
+
 constexpr int tile_tj(int tile) {
     return tile == 1 ? 4 : tile == 2 ? 8 : tile == 3 ? 16 : tile == 4 ? 8 : tile == 5 ? 16 : 16;
 }
@@ -1853,4 +1886,71 @@ extern "C" int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_p
         default: return launch_batch<COLOR>(A, P->d_lat, P->d_desc, P->ndesc, P->tile, fo,
                                             streaming<COLOR>(size_t(P->total)), st);
     }
+}
+
+extern "C" int lfm_batch_step2_f32(void* plan, const float* pr, const float* pr_prev,
+                                   const float* pi, const float* pi_prev, const float* chi,
+                                   const float* chi_prev, const float* mask, float* pr_out1,
+                                   float* pi_out1, float* chi_out1, float* pr_out2,
+                                   float* pi_out2, float* chi_out2, int32_t flags,
+                                   void* stream) {
+    auto* P = static_cast<BatchPlan*>(plan);
+    if (!P || !pr || !pr_prev || !chi || !chi_prev || !mask || !pr_out1 || !chi_out1 ||
+        !pr_out2 || !chi_out2) {
+        set_error("lfm_batch_step2_f32: required pointer is NULL");
+        return LFM_E_ARG;
+    }
+    if (P->level == 2) {
+        set_error("lfm_batch_step2_f32: REAL and COMPLEX batches only");
+        return LFM_E_ARG;
+    }
+    if (P->level == 1 && (!pi || !pi_prev || !pi_out1 || !pi_out2)) {
+        set_error("lfm_batch_step2_f32: COMPLEX batches need pi, pi_prev and both pi outputs");
+        return LFM_E_ARG;
+    }
+    if (tile_tj(P->tile) != T2J) {
+        set_error("lfm_batch_step2_f32: the plan's column tiles must be 64 x %d (tile variant 2 "
+                  "or 4), plan has tile variant %d", T2J, P->tile);
+        return LFM_E_ARG;
+    }
+    if (flags & ~(LFM_F_DAMP_PREV | LFM_F_CHI_DELTA)) {
+        set_error("lfm_batch_step2_f32: only LFM_F_DAMP_PREV and LFM_F_CHI_DELTA are supported");
+        return LFM_E_ARG;
+    }
+    const size_t n = size_t(P->total);
+    const bool cx = P->level == 1;
+    const float* ins[] = {pr, pr_prev, cx ? pi : nullptr, cx ? pi_prev : nullptr, chi, chi_prev,
+                          mask};
+    float* outs[] = {pr_out1, cx ? pi_out1 : nullptr, chi_out1, pr_out2, cx ? pi_out2 : nullptr,
+                     chi_out2};
+    for (int a = 0; a < 6; ++a) {
+        if (!outs[a]) continue;
+        for (int b = 0; b < 7; ++b)
+            if (ins[b] && overlaps(outs[a], n * 4, ins[b], n * 4)) {
+                set_error("lfm_batch_step2_f32: output %d overlaps input %d", a, b);
+                return LFM_E_ARG;
+            }
+        for (int b = a + 1; b < 6; ++b)
+            if (outs[b] && overlaps(outs[a], n * 4, outs[b], n * 4)) {
+                set_error("lfm_batch_step2_f32: outputs %d and %d overlap", a, b);
+                return LFM_E_ARG;
+            }
+    }
+    BatchArenas2 A2{};
+    A2.a = BatchArenas{pr, pr_prev, cx ? pi : nullptr, cx ? pi_prev : nullptr, chi, chi_prev, mask,
+                       pr_out1, cx ? pi_out1 : nullptr, chi_out1, nullptr, nullptr};
+    A2.pr_out2 = pr_out2;
+    A2.pi_out2 = cx ? pi_out2 : nullptr;
+    A2.chi_out2 = chi_out2;
+    const dim3 g(P->ndesc), b(T2T);
+    hipStream_t st = (hipStream_t)stream;
+    if (P->level == 0) {
+        if (streaming<REAL>(n)) hipLaunchKernelGGL((step2_tiled_batch<REAL, true>), g, b, 0, st, A2, P->d_lat, P->d_desc, flags);
+        else hipLaunchKernelGGL((step2_tiled_batch<REAL, false>), g, b, 0, st, A2, P->d_lat, P->d_desc, flags);
+    } else {
+        if (streaming<COMPLEX>(n)) hipLaunchKernelGGL((step2_tiled_batch<COMPLEX, true>), g, b, 0, st, A2, P->d_lat, P->d_desc, flags);
+        else hipLaunchKernelGGL((step2_tiled_batch<COMPLEX, false>), g, b, 0, st, A2, P->d_lat, P->d_desc, flags);
+    }
+    LFM_LAUNCH_CHECK("step2_tiled_batch");
+    return LFM_OK;
 }

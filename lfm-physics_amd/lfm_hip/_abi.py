@@ -25,7 +25,7 @@ LFM_F_DIRECT = 0x08
 LFM_F_CHI_DELTA = 0x10
 LFM_F_SA_SCV = 0x20
 LFM_F_KEEP_PSI = 0x40
-LFM_ABI_VERSION = 4
+LFM_ABI_VERSION = 5
 LFM_TILE_VARIANTS = 6
 LFM_RELAX_SLOT_WORDS = 2048
 
@@ -58,6 +58,7 @@ SIGNATURES = [
     ("lfm_batch_total_sites", _i64, [_vp]),
     ("lfm_batch_workgroups", _i32, [_vp]),
     ("lfm_batch_step_f32", ctypes.c_int, [_vp] * 13 + [_i32, _vp]),
+    ("lfm_batch_step2_f32", ctypes.c_int, [_vp] * 14 + [_i32, _vp]),
     ("lfm_adjoint_partials_len", _i64, [_i32]),
     ("lfm_calib_stream_f32", ctypes.c_int, [_vp, _i32, _vp, _i32, ctypes.c_int64, _i32, _i32, _vp]),
     ("lfm_relax_step_f32", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_float, _i32,

@@ -53,7 +53,7 @@ class LatticeBatch:
 
     def __init__(self, configs, backend="auto", tile: int = 0, chunk: int = 0,
                  mask_policy: str = "reference", group_bytes: int | None = None,
-                 cycles: int = 1):
+                 cycles: int = 1, two_step: bool = False):
         if not configs:
             raise ValueError("LatticeBatch needs at least one config")
         levels = {int(c.field_level) for c in configs}
@@ -62,6 +62,14 @@ class LatticeBatch:
         self.level = levels.pop()
         if self.level == 2 and any(c.n_colors != 3 for c in configs):
             raise NotImplementedError("the COLOR step is defined for 3 colours")
+        # two_step: REAL / COMPLEX pairs of steps in one launch
+        # (lfm_batch_step2_f32, bit-identical to single steps) over a fourth
+        # rotation slot, since both outputs must differ from both inputs
+        if two_step and self.level == 2:
+            raise ValueError("two_step: REAL and COMPLEX batches only (a COLOR step needs its "
+                             "S_a transform between steps)")
+        self.two_step = bool(two_step)
+        self._ns = 4 if self.two_step else 3
         self.backend = get_backend(backend)
         self.device = self.backend.device
         self._lib = _abi.load()
@@ -75,10 +83,11 @@ class LatticeBatch:
         f32 = torch.float32
         dev = self.device
         npsi = self.ncomp * self.total
-        self._pr = [torch.zeros(npsi, dtype=f32, device=dev) for _ in range(3)]
-        self._pi = ([torch.zeros(npsi, dtype=f32, device=dev) for _ in range(3)]
+        ns = self._ns
+        self._pr = [torch.zeros(npsi, dtype=f32, device=dev) for _ in range(ns)]
+        self._pi = ([torch.zeros(npsi, dtype=f32, device=dev) for _ in range(ns)]
                     if self.level > 0 else None)
-        self._chi = [torch.zeros(self.total, dtype=f32, device=dev) for _ in range(3)]
+        self._chi = [torch.zeros(self.total, dtype=f32, device=dev) for _ in range(ns)]
         self._chi0 = [float(np.float32(c.chi0)) for c in configs]
         self.mask = torch.zeros(self.total, dtype=f32, device=dev)
         self.has_mask = []
@@ -127,9 +136,10 @@ class LatticeBatch:
     # ----------------------------------------------------------- groups
     @property
     def bytes_per_site(self) -> int:
-        """Three slots of Psi (and Psi_i) and chi plus the mask, float32."""
+        """The rotation slots (three, four with two_step) of Psi (and Psi_i)
+        and chi plus the mask, float32."""
         per_slot = self.ncomp * (2 if self.level > 0 else 1) + 1
-        return 4 * (3 * per_slot + 1)
+        return 4 * (self._ns * per_slot + 1)
 
     def _set_groups(self, group_bytes: int) -> None:
         """Contiguous lattice ranges [b0, b1) of at most `group_bytes` of
@@ -151,7 +161,13 @@ class LatticeBatch:
     # ------------------------------------------------------------- plan
     def _make_plan(self, tile: int, chunk: int) -> None:
         """One batch plan per lattice group; plan g addresses the arenas from
-        the group's first site on (pointer offsets at launch)."""
+        the group's first site on (pointer offsets at launch).  With two_step
+        the column tiles are 64 x 8 (tile variant 2, the default, or 4)."""
+        if self.two_step:
+            tile = 2 if int(tile) <= 0 else int(tile)
+            if tile not in (2, 4):
+                raise ValueError(f"two_step batches take tile variant 2 or 4 (64 x 8 column "
+                                 f"tiles), got {tile}")
         plans = []
         try:
             for b0, b1 in self.groups:
@@ -270,9 +286,28 @@ class LatticeBatch:
         """(cur, prev, next, chi cur, chi prev, chi next, flags) of the next step."""
         cur, prev = self._cur, self._prev
         cc, cp = self._chi_cur, self._chi_prev
-        cn = 3 - cc - cp if cc != cp else (cc + 1) % 3
+        ns = self._ns
+        nxt = next(s for s in range(ns) if s not in (cur, prev))
+        cn = next(s for s in range(ns) if s not in (cc, cp)) if cc != cp else (cc + 1) % ns
         flags = _abi.LFM_F_CHI_DELTA | (_abi.LFM_F_DAMP_PREV if self._damped else 0)
-        return cur, prev, 3 - cur - prev, cc, cp, cn, flags
+        return cur, prev, nxt, cc, cp, cn, flags
+
+    def _slots2(self):
+        """(cur, prev, out1, out2, chi cur, chi prev, chi out1, chi out2, flags)
+        of the next pair of steps (two_step: the two free slots of four)."""
+        cur, prev = self._cur, self._prev
+        cc, cp = self._chi_cur, self._chi_prev
+        assert self.two_step and cc != cp
+        o1, o2 = (s for s in range(4) if s not in (cur, prev))
+        c1, c2 = (s for s in range(4) if s not in (cc, cp))
+        flags = _abi.LFM_F_CHI_DELTA | (_abi.LFM_F_DAMP_PREV if self._damped else 0)
+        return cur, prev, o1, o2, cc, cp, c1, c2, flags
+
+    def _advance2(self, slots2) -> None:
+        _, _, o1, o2, _, _, c1, c2, _ = slots2
+        self._prev, self._cur = o1, o2
+        self._chi_prev, self._chi_cur = c1, c2
+        self._damped = True
 
     def _advance(self, slots) -> None:
         cur, _, nxt, cc, _, cn, _ = slots
@@ -323,6 +358,25 @@ class LatticeBatch:
         b0, b1 = self.groups[g]
         self._psisq_ok[b0:b1] = [True] * (b1 - b0)
 
+    def _launch2(self, g: int, slots2) -> None:
+        cur, prev, o1, o2, cc, cp, c1, c2, flags = slots2
+        o = int(self.offsets[self.groups[g][0]]) * 4
+        pr, pi, chi = self._pr, self._pi, self._chi
+        ptr = lambda arena, slot: arena[slot].data_ptr() + o if arena else None
+        rc = self._lib.lfm_batch_step2_f32(
+            self._plans[g], ptr(pr, cur), ptr(pr, prev), ptr(pi, cur), ptr(pi, prev),
+            ptr(chi, cc), ptr(chi, cp), self.mask.data_ptr() + o,
+            ptr(pr, o1), ptr(pi, o1), ptr(chi, c1), ptr(pr, o2), ptr(pi, o2), ptr(chi, c2),
+            flags, stream_handle())
+        _abi.check(rc, "lfm_batch_step2_f32")
+
+    def _step2(self) -> None:
+        """Two steps of every lattice (two_step: one launch per group)."""
+        slots2 = self._slots2()
+        for g in range(len(self.groups)):
+            self._launch2(g, slots2)
+        self._advance2(slots2)
+
     def _step(self) -> None:
         """One step of every lattice (one launch per group)."""
         slots = self._slots()
@@ -348,6 +402,9 @@ class LatticeBatch:
         (steps // (3 cycles) replays, the remainder launched directly)."""
         steps = int(steps)
         if steps <= 0:
+            return
+        if self.two_step:
+            self._evolve2(steps, graph)
             return
         if not graph or any(p.method != 2 for p in self._scv_plans.values()):
             # rocFFT plans (non-power-of-two S_a sizes) stay out of graphs
@@ -376,6 +433,57 @@ class LatticeBatch:
             done += 1
         self.step += steps
 
+    def _evolve2(self, steps: int, graph: bool) -> None:
+        """evolve() with two_step: pairs of steps per launch, replayed from HIP
+        graphs of `cycles` x 2 pairs (a pair's outputs take the two free slots,
+        so two pairs return every slot to where it was); an odd last step is a
+        single launch."""
+        done = 0
+        if graph:
+            # steady state: prev damped and the current slot above the prev
+            # slot, from which every two pairs come back to the same slots
+            while steps - done >= 2 and (not self._damped or self._cur < self._prev):
+                self._step2()
+                done += 2
+            per = 4 * self.cycles
+            reps = (steps - done) // per
+            if reps:
+                g = self._graphs.get(self._state_key())
+                if g is None:
+                    g = self._capture2()
+                for _ in range(reps):
+                    g.replay()
+                done += per * reps
+        while steps - done >= 2:
+            self._step2()
+            done += 2
+        if done < steps:
+            self._step()
+        self.step += steps
+
+    def _capture2(self):
+        """Capture 2 x `cycles` pair launches of every group from the current
+        (steady) state; see _capture."""
+        key = self._state_key()
+        saved = (self._cur, self._prev, self._chi_cur, self._chi_prev)
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        g = torch.cuda.CUDAGraph()
+        seq = []
+        for _ in range(2 * self.cycles):
+            seq.append(self._slots2())
+            self._advance2(seq[-1])
+        if (self._cur, self._prev, self._chi_cur, self._chi_prev) != saved:
+            raise RuntimeError("two-step slot rotation did not return to its start")
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                for grp in range(len(self.groups)):
+                    for slots2 in seq:
+                        self._launch2(grp, slots2)
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        self._graphs[key] = g
+        return g
+
     def _state_key(self):
         return (self._cur, self._prev, self._chi_cur, self._chi_prev, self._damped)
 
@@ -402,24 +510,35 @@ class LatticeBatch:
         self._graphs[key] = g
         return g
 
-    def autotune(self, steps: int = 6, tiles=None, chunks=(8, 16, 32)) -> dict:
+    def autotune(self, steps: int = 6, tiles=None, chunks=None) -> dict:
         """Time tile shapes x plane chunks of the batched kernel on the
-        current state (restored afterwards) and keep the fastest."""
-        tiles = tiles or range(1, _abi.LFM_TILE_VARIANTS + 1)
+        current state (restored afterwards) and keep the fastest (two_step:
+        tile variants 2 and 4, pairs of steps)."""
+        if self.two_step:
+            tiles = tiles or (2, 4)
+            chunks = chunks or (16, 32, 64)
+        else:
+            tiles = tiles or range(1, _abi.LFM_TILE_VARIANTS + 1)
+            chunks = chunks or (8, 16, 32)
         snap = ([t.clone() for t in self._pr], [t.clone() for t in self._pi] if self._pi else None,
                 [t.clone() for t in self._chi], self._state_key())
         res = {}
         for tile, ch in itertools.product(tiles, chunks):
             self._make_plan(tile, ch)
-            self._step()
-            reps = max(1, -(-steps // 3))
+            self._step2() if self.two_step else self._step()
+            per = 4 if self.two_step else 3
+            reps = max(1, -(-steps // per))
             ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             ev0.record()
             for _ in range(reps):
-                self._cycle()
+                if self.two_step:
+                    self._step2()
+                    self._step2()
+                else:
+                    self._cycle()
             ev1.record()
             torch.cuda.synchronize(self.device)
-            res[(tile, ch)] = ev0.elapsed_time(ev1) / (3 * reps)
+            res[(tile, ch)] = ev0.elapsed_time(ev1) / (per * reps)
             for dst, src in zip(self._pr, snap[0]):
                 dst.copy_(src)
             if snap[1]:

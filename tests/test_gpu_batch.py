@@ -211,3 +211,56 @@ def test_batched_sweep_2d_rows(gpu):
         one = sweep(c, "kappa", [r["kappa"]], 20, soliton={"amplitude": 3.0}, backend="hip",
                     metric_names=["chi_min", "energy_total"])[0]
         assert one["chi_min"] == r["chi_min"] and one["energy_total"] == r["energy_total"]
+
+
+@pytest.mark.parametrize("level,group_bytes,cycles,tile,periodic",
+                         [(0, None, 1, 0, False), (1, None, 1, 0, False), (1, None, 2, 4, True),
+                          (1, 1 << 21, 1, 0, False), (0, 1 << 21, 2, 4, True)])
+def test_two_step_batch_equals_single_step_batch(gpu, level, group_bytes, cycles, tile, periodic):
+    """Pairs of steps per launch (lfm_batch_step2_f32, four-slot rotation)
+    against the one-step batch: ragged sizes, a mid-run setter, odd step
+    counts, graph replay and eager launches, lattice groups; every lattice bit
+    for bit (Psi, chi, and the leapfrog prev through metrics' kinetic term)."""
+    import torch
+    import lfm_hip as L
+    sizes = [24, 40, 16, 72, 33, 20]
+    log(f"two-step batch vs one-step batch, level {level}, groups {group_bytes}, cycles {cycles}, "
+        f"tile {tile}, periodic {periodic}")
+    rng = np.random.default_rng(31 + level)
+    cfgs = _configs(level, sizes, rng)
+    if periodic:
+        for c in cfgs[::2]:
+            c.boundary_type = L.BoundaryType.PERIODIC
+    ics = [_ics(level, N, rng) for N in sizes]
+    batches = {}
+    for two in (True, False):
+        bt = L.LatticeBatch(cfgs, backend="hip", group_bytes=group_bytes, cycles=cycles,
+                            two_step=two, tile=tile if two else 0,
+                            mask_policy="boundary_type" if periodic else "reference")
+        assert bt.two_step == two
+        for b, (r, i) in enumerate(ics):
+            bt.set_psi_real(b, r)
+            if i is not None:
+                bt.set_psi_imag(b, i)
+        batches[two] = bt
+    for n, graph in ((7, True), (10, False), (33, True), (1, True), (24, True)):
+        for bt in batches.values():
+            bt.evolve(n, graph=graph)
+        if n == 10:
+            for bt in batches.values():
+                bt.set_psi_real(2, bt.get_psi_real(2) * 0.99)
+        torch.cuda.synchronize()
+        a, b2 = batches[True], batches[False]
+        for b in range(len(sizes)):
+            np.testing.assert_array_equal(a.get_psi_real(b), b2.get_psi_real(b), err_msg=f"psi_r {b}")
+            np.testing.assert_array_equal(a.get_chi(b), b2.get_chi(b), err_msg=f"chi {b}")
+            if level:
+                np.testing.assert_array_equal(a.get_psi_imag(b), b2.get_psi_imag(b))
+        ka = [a.metrics(b, clusters=False)["energy_kinetic"] for b in range(len(sizes))]
+        kb = [b2.metrics(b, clusters=False)["energy_kinetic"] for b in range(len(sizes))]
+        assert ka == kb
+    if group_bytes is not None:
+        assert len(batches[True].groups) >= 3
+    used = [k for k, v in batches[True]._graphs.items() if v is not None]
+    log(f"  two-step graphs captured: {len(used)}")
+    assert used

@@ -55,7 +55,7 @@ def test_binding_arity_matches_header():
 def test_abi_version_and_error_string():
     from lfm_hip import _abi
     lib = _abi.load()
-    assert lib.lfm_abi_version() == _abi.LFM_ABI_VERSION == 4
+    assert lib.lfm_abi_version() == _abi.LFM_ABI_VERSION == 5
     # argument validation happens before any device work: N < 8 is refused
     p = _abi.LfmParams(4, 0, 4e-4, 1 / 63, 0, 19, 0, 0.1, 0, 0, 0, 0)
     dummy = [1 << 20] * 4
