@@ -432,6 +432,84 @@ def run_adjoint(args):
     return out
 
 
+def run_slab(args):
+    """One GPU's share of an 8-way i-slab decomposition of the COLOR
+    full_physics 512^3 workload: 64 interior planes plus one halo plane on
+    each side (lfm_step_slab_f32), against the whole-lattice step on the same
+    fields (lfm_step_color_f32 with the SCV term), both timed with HIP events
+    over `steps` launches on the launch stream.  The share's interior equals
+    the whole step's planes bit for bit (checked here on the last launch).
+    The halo exchange between shares is not part of one GPU's work here."""
+    import ctypes
+    import torch
+    import lfm_hip as L
+    from lfm_hip import _abi
+    lib = _abi.load()
+    w = WORKLOADS["color512"]
+    N, S, p0 = 512, 64, 192
+    cfg = make_config(w, N)
+    pr, pi = gpu_ics(w, N)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    prp = pr + 1e-3 * torch.randn(pr.shape, generator=g, device="cuda")
+    pip = pi + 1e-3 * torch.randn(pi.shape, generator=g, device="cuda")
+    chi = 1e-2 * torch.randn((1, N, N, N), generator=g, device="cuda")     # chi - chi0
+    chip = chi + 1e-4 * torch.randn((1, N, N, N), generator=g, device="cuda")
+    scv = torch.rand((1, N, N, N), generator=g, device="cuda")
+    mask = torch.from_numpy(L.fields.boundary_mask_host(N, cfg.boundary_fraction)).cuda().view(1, N, N, N)
+    pr, pi = pr.view(3, N, N, N), pi.view(3, N, N, N)
+    p = _abi.LfmParams(N, _abi.LFM_F_MASK | _abi.LFM_F_DAMP_PREV | _abi.LFM_F_CHI_DELTA | _abi.LFM_F_SA_SCV,
+                       cfg.dt ** 2, cfg.kappa, cfg.lambda_self, cfg.chi0, cfg.e0_sq, cfg.epsilon_w,
+                       cfg.kappa_c, cfg.epsilon_cc, cfg.kappa_string, cfg.kappa_tube,
+                       2 if args.tile == "auto" else int(args.tile), args.chunk)
+    st = torch.cuda.current_stream().cuda_stream
+    ptr = lambda t: t.data_ptr() if t is not None else None
+    idx = torch.arange(p0 - 1, p0 + S + 1, device="cuda")
+    cut = lambda t: t.index_select(1, idx).contiguous()
+    sl = [cut(t) for t in (pr, prp, pi, pip, chi, chip, mask, scv)]
+    o_w = [torch.empty_like(pr), torch.empty_like(pi), torch.empty_like(chi), torch.empty_like(pr)]
+    o_s = [torch.empty_like(sl[0]), torch.empty_like(sl[0]), torch.empty_like(sl[4]),
+           torch.empty_like(sl[0])]
+
+    def whole():
+        return lib.lfm_step_color_f32(ptr(pr), ptr(prp), ptr(pi), ptr(pip), ptr(chi), ptr(chip),
+                                      ptr(mask), ptr(o_w[0]), None, ptr(o_w[1]), None, ptr(o_w[2]),
+                                      None, ptr(scv), None, ptr(o_w[3]), ctypes.byref(p), st)
+
+    def share():
+        return lib.lfm_step_slab_f32(2, S + 2, *[ptr(t) for t in sl], *[ptr(t) for t in o_s],
+                                     ctypes.byref(p), st)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            _abi.check(fn(), "step")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    t_w, t_s = timed(whole), timed(share)
+    same = all(bool(torch.equal(a[:, 1:-1], b.index_select(1, idx[1:-1])))
+               for a, b in zip(o_s, o_w))
+    sites = S * N * N
+    achieved = w["b_alg"] * sites / (t_s * 1e-3) / 1e9
+    return {"metric": METRIC, "value": sites / (t_s * 1e-3) / 1e9, "unit": "GLUPS", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_s,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": f"one 8-way i-slab share of COLOR full_physics {N}^3: {S} interior "
+                                   f"+ 2 halo planes of {N}^2 (stencil kernel, SCV term given)",
+                       "grid": N, "interior_planes": S, "halo_planes": 2},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "step_tiled<COLOR> (slab)", "kernel_ms": t_s,
+                         "bytes_per_site": w["b_alg"]},
+            "whole_lattice_kernel_ms": t_w, "ideal_share_ms": t_w * S / N,
+            "share_efficiency": t_w * S / N / t_s, "interior_equals_whole_step": same}
+
+
 def run_relax(args):
     """Eigenmode relaxation at the reference's default N = 64: the Gaussian
     seed (amplitude 14, sigma 4), one Poisson chi, then `steps` imaginary-time
@@ -502,7 +580,7 @@ def main():
                     help="timed steps (default: 100; batch64 500, adjoint128 400, relax64 1000)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="color512",
-                    choices=sorted(WORKLOADS) + ["copy", "batch64", "adjoint128", "relax64"])
+                    choices=sorted(WORKLOADS) + ["copy", "batch64", "adjoint128", "relax64", "slab512"])
     ap.add_argument("--size", type=int, default=None, help="override the lattice size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true",
@@ -535,6 +613,9 @@ def main():
         return
     if args.workload == "relax64":
         run_relax(args)
+        return
+    if args.workload == "slab512":
+        print(json.dumps(run_slab(args)), flush=True)
         return
     if args.workload == "copy":
         # Device-to-device copy of a known size: the HBM calibration for the

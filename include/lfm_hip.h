@@ -109,6 +109,24 @@ int lfm_step_complex_f32(const float* pr, const float* pr_prev, const float* pi,
  * psisq_out:   NULL, or receives |Psi_a'|^2 = fmaf(pr', pr', pi'*pi') of the
  *              NEW fields (3*N^3), the input of the next step's S_a transform
  *              (the rounding the transforms use when they form it from Psi). */
+/* One GPU's share of a lattice sharded along i (no counterpart in the
+ * reference, which runs one lattice on one device; it is the step of
+ * lfm_step_real_f32 / _complex_f32 / _color_f32 on a slab).  The slab holds
+ * `planes` planes of N x N sites per field (COLOR: colour-major, 3 x planes x
+ * N^2): planes 1 .. planes-2 are this share's interior, planes 0 and
+ * planes-1 the neighbouring shares' boundary planes (one halo plane per side
+ * covers the 19-point stencil and the currents).  The step reads all
+ * planes, periodic in j and k, and writes the interior planes of pr_out /
+ * pi_out / chi_out (and, COLOR with kappa_tube > 0, psisq_out); their halo
+ * planes are left as they are.  Rotation-mode outputs only; `sa` as for
+ * lfm_step_color_f32 (the SCV term under LFM_F_SA_SCV), slab-shaped.  The
+ * interior equals the same planes of a whole-lattice step bit for bit.
+ * level: 0 REAL, 1 COMPLEX, 2 COLOR; 3 <= planes <= N <= 1024. */
+int lfm_step_slab_f32(int32_t level, int32_t planes, const float* pr, const float* pr_prev,
+                      const float* pi, const float* pi_prev, const float* chi,
+                      const float* chi_prev, const float* mask, const float* sa, float* pr_out,
+                      float* pi_out, float* chi_out, float* psisq_out, const lfm_params_t* p,
+                      void* stream);
 /* Two leapfrog steps per call (REAL / COMPLEX), rotation mode: from
  * (Psi_n = pr/pi, Psi_{n-1} = pr_prev/pi_prev, chi_n, chi_{n-1}) it writes
  * step n+1 into *_out1 and step n+2 into *_out2, exactly as two calls of

@@ -57,6 +57,9 @@ struct KParams {
     // division done once on the host (the same value a device division with
     // round-to-nearest gives), so no kernel divides per site
     float kc0, kcc0;
+    // planes [ib, ie) along i are computed (0, N for a whole lattice; a slab
+    // share computes its interior planes and reads the two halo planes)
+    int ib, ie;
 };
 
 struct Ptrs {
@@ -595,8 +598,8 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nk
     const int rest = lb / nkt;
     const int jt = rest % njt;
     const int ch = rest / njt;
-    const int i0 = ch * CH;
-    tile_body<L, BJ, RJ, NT, KEEP>(P, kp, kt * TK, jt * BJ * RJ, i0, min(i0 + CH, kp.N));
+    const int i0 = kp.ib + ch * CH;
+    tile_body<L, BJ, RJ, NT, KEEP>(P, kp, kt * TK, jt * BJ * RJ, i0, min(i0 + CH, kp.ie));
 }
 
 // Batched sweep: one launch advances many independent lattices.  Each
@@ -664,8 +667,9 @@ int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
     const int N = kp.N;
     const int nkt = (N + TK - 1) / TK;
     const int njt = (N + TJ - 1) / TJ;
-    const int CH = kp.chunk > 0 ? min(kp.chunk, N) : choose_chunk(N, nkt * njt, NT ? 8 : 2);
-    const int nch = (N + CH - 1) / CH;
+    const int span = kp.ie - kp.ib;       // planes computed
+    const int CH = kp.chunk > 0 ? min(kp.chunk, span) : choose_chunk(span, nkt * njt, NT ? 8 : 2);
+    const int nch = (span + CH - 1) / CH;
     const long long nblocks = (long long)nkt * njt * nch;
     hipLaunchKernelGGL((step_tiled<L, BJ, RJ, NT, KEEP>), dim3((unsigned)nblocks), dim3(64 * BJ), 0,
                        st, P, kp, nkt, njt, CH);
@@ -1460,6 +1464,8 @@ KParams to_k(const lfm_params_t* p) {
     // IEEE single division, round to nearest: bit-equal to __fdiv_rn
     k.kc0 = p->kappa / p->chi0;
     k.kcc0 = p->kappa_c / p->chi0;
+    k.ib = 0;
+    k.ie = p->N;
     return k;
 }
 
@@ -1700,6 +1706,94 @@ extern "C" int lfm_step_color_f32(const float* pr, const float* pr_prev, const f
     P.psisq = psisq_out;
     P.n3 = n3;
     return launch<COLOR>(P, to_k(p), (hipStream_t)stream);
+}
+
+extern "C" int lfm_step_slab_f32(int32_t level, int32_t planes, const float* pr,
+                                 const float* pr_prev, const float* pi, const float* pi_prev,
+                                 const float* chi, const float* chi_prev, const float* mask,
+                                 const float* sa, float* pr_out, float* pi_out, float* chi_out,
+                                 float* psisq_out, const lfm_params_t* p, void* stream) {
+    if (!p || !pr || !pr_prev || !chi || !chi_prev || !pr_out || !chi_out) {
+        set_error("lfm_step_slab_f32: required pointer is NULL");
+        return LFM_E_ARG;
+    }
+    if (level < 0 || level > 2) {
+        set_error("lfm_step_slab_f32: level %d is not 0 (REAL), 1 (COMPLEX) or 2 (COLOR)", level);
+        return LFM_E_ARG;
+    }
+    if (p->N < 8 || p->N > 1024) {
+        set_error("lfm_step_slab_f32: N=%d out of range [8, 1024]", p->N);
+        return LFM_E_ARG;
+    }
+    if (planes < 3 || planes > p->N) {
+        set_error("lfm_step_slab_f32: planes=%d out of range [3, N=%d] (a slab stores its "
+                  "interior planes and one halo plane on each side)", planes, p->N);
+        return LFM_E_ARG;
+    }
+    if (level > 0 && (!pi || !pi_prev || !pi_out)) {
+        set_error("lfm_step_slab_f32: complex levels need pi, pi_prev and pi_out");
+        return LFM_E_ARG;
+    }
+    if ((p->flags & LFM_F_MASK) && !mask) {
+        set_error("lfm_step_slab_f32: LFM_F_MASK set but mask is NULL");
+        return LFM_E_ARG;
+    }
+    if (p->flags & (LFM_F_DIRECT | LFM_F_KEEP_PSI)) {
+        set_error("lfm_step_slab_f32: LFM_F_DIRECT / LFM_F_KEEP_PSI not supported");
+        return LFM_E_ARG;
+    }
+    const bool scv = level == 2 && p->kappa_tube > 0.0f;
+    if (scv && !sa) {
+        set_error("lfm_step_slab_f32: kappa_tube > 0 needs the S_a input (sa is NULL)");
+        return LFM_E_ARG;
+    }
+    if ((p->flags & LFM_F_SA_SCV) && !scv) {
+        set_error("lfm_step_slab_f32: LFM_F_SA_SCV is for COLOR with kappa_tube > 0");
+        return LFM_E_ARG;
+    }
+    const size_t ns = size_t(planes) * p->N * p->N;     // one field of the slab
+    const size_t np = level == 2 ? 3 * ns : ns;
+    const size_t nsa = (p->flags & LFM_F_SA_SCV) ? ns : np;
+    const float* ins[] = {pr, pr_prev, level ? pi : nullptr, level ? pi_prev : nullptr, chi,
+                          chi_prev, mask, scv ? sa : nullptr};
+    const size_t nin[] = {np, np, np, np, ns, ns, ns, nsa};
+    float* outs[] = {pr_out, level ? pi_out : nullptr, chi_out, scv ? psisq_out : nullptr};
+    const size_t nout[] = {np, np, ns, np};
+    for (int a = 0; a < 4; ++a) {
+        if (!outs[a]) continue;
+        for (int b = 0; b < 8; ++b)
+            if (ins[b] && overlaps(outs[a], nout[a] * 4, ins[b], nin[b] * 4)) {
+                set_error("lfm_step_slab_f32: output %d overlaps input %d", a, b);
+                return LFM_E_ARG;
+            }
+        for (int b = a + 1; b < 4; ++b)
+            if (outs[b] && overlaps(outs[a], nout[a] * 4, outs[b], nout[b] * 4)) {
+                set_error("lfm_step_slab_f32: outputs %d and %d overlap", a, b);
+                return LFM_E_ARG;
+            }
+    }
+    Ptrs P{};
+    P.pr = pr; P.prp = pr_prev; P.chi = chi; P.chip = chi_prev;
+    if (level) { P.pi = pi; P.pip = pi_prev; P.pi_out = pi_out; }
+    P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
+    P.sa = scv ? sa : nullptr;
+    P.pr_out = pr_out; P.chi_out = chi_out;
+    P.psisq = scv ? psisq_out : nullptr;
+    P.n3 = ns;
+    KParams k = to_k(p);
+    k.ib = 1;
+    k.ie = planes - 1;
+    const hipStream_t st = (hipStream_t)stream;
+    const int tile = k.tile > 0 ? k.tile : 1;
+    if (tile > LFM_TILE_VARIANTS) {
+        set_error("tile variant %d unknown (1..%d)", tile, LFM_TILE_VARIANTS);
+        return LFM_E_ARG;
+    }
+    switch (level) {
+        case 0: return launch<REAL>(P, k, st);
+        case 1: return launch<COMPLEX>(P, k, st);
+        default: return launch<COLOR>(P, k, st);
+    }
 }
 
 extern "C" int lfm_step_color_fused_f32(const float* pr, const float* pr_prev, const float* pi,

@@ -38,6 +38,7 @@ SIGNATURES = [
     ("lfm_last_error", ctypes.c_char_p, []),
     ("lfm_step_real_f32", ctypes.c_int, _STEP_REAL),
     ("lfm_step_complex_f32", ctypes.c_int, _STEP_COMPLEX),
+    ("lfm_step_slab_f32", ctypes.c_int, [_i32, _i32] + [_vp] * 12 + [_vp, _vp]),
     ("lfm_step2_real_f32", ctypes.c_int, [_vp] * 9 + [_vp, _vp]),
     ("lfm_step2_complex_f32", ctypes.c_int, [_vp] * 13 + [_vp, _vp]),
     ("lfm_step_color_f32", ctypes.c_int, _STEP_COLOR),

@@ -112,6 +112,33 @@ def test_abi_rejects_in_place_and_missing_inputs():
     assert rc == -1 and b"S_a" in lib.lfm_last_error()
 
 
+def test_slab_step_validation():
+    """lfm_step_slab_f32 refuses bad shapes and flags before any device work."""
+    from lfm_hip import _abi
+    lib = _abi.load()
+    N = 16
+    p = _abi.LfmParams(N, 0, 4e-4, 1 / 63, 0, 19, 0, 0.1, 0, 0, 0, 0)
+    fld = 4 * 3 * N * N * N
+    ptr = [(1 << 24) + q * fld for q in range(16)]
+    ins = [ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5], None, None]
+    outs = [ptr[6], ptr[7], ptr[8], None]
+    for level, planes, flags, what in ((1, 2, 0, b"planes"), (1, N + 1, 0, b"planes"),
+                                       (3, 8, 0, b"level"), (1, 8, _abi.LFM_F_DIRECT, b"DIRECT"),
+                                       (1, 8, _abi.LFM_F_MASK, b"mask"),
+                                       (0, 8, _abi.LFM_F_SA_SCV, b"SA_SCV")):
+        p.flags = flags
+        rc = lib.lfm_step_slab_f32(level, planes, *ins, *outs, ctypes.byref(p), None)
+        assert rc == -1 and what in lib.lfm_last_error(), (level, planes, flags)
+    p.flags = 0
+    # an output aliasing an input
+    rc = lib.lfm_step_slab_f32(1, 8, *ins, ptr[0], ptr[7], ptr[8], None, ctypes.byref(p), None)
+    assert rc == -1 and b"overlaps" in lib.lfm_last_error()
+    # COLOR with kappa_tube and no S_a input
+    p.kappa_tube = 0.5
+    rc = lib.lfm_step_slab_f32(2, 8, *ins, *outs, ctypes.byref(p), None)
+    assert rc == -1 and b"S_a" in lib.lfm_last_error()
+
+
 def test_host_mask_bit_identical_to_oracle():
     from lfm_hip.fields import boundary_mask_host
     for N, bf in ((8, 0.3), (16, 0.15), (48, 0.3), (64, 0.3)):
