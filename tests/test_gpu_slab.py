@@ -5,7 +5,8 @@ step bit for bit (the same per-site arithmetic), at every level, with the
 mask, damped prev, chi deviation storage and, at COLOR, the SCV term and the
 |Psi'|^2 output; at N = 64 for several slab positions and thicknesses, and at
 the real shape of an 8-way share of the 512^3 benchmark lattice (64 interior
-planes + 2 halo planes of 512^2).
+planes + 2 halo planes of 512^2).  A 2-way decomposition stepped with halo
+copies between the shares reproduces the whole-lattice run over 12 steps.
 """
 import ctypes
 import sys
@@ -108,3 +109,94 @@ def test_slab_equals_whole_lattice_512_share(gpu, level):
     log(f"slab share vs whole lattice, level {level}, an 8-way share of 512^3")
     _case(level, 512, 192, 64, seed=5)
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_two_shares_with_halo_exchange_equal_whole_lattice(gpu, level):
+    """A 2-way i-slab decomposition run on one GPU: each share keeps its own
+    three-slot rotation of interior + halo planes, and before every step the
+    current Psi and chi halo planes are copied from the neighbouring share
+    (the exchange RCCL would do between ranks; prev fields are read only at
+    interior sites and need none).  After 12 steps both shares' interiors
+    equal the whole-lattice run bit for bit (COLOR without the S_a term,
+    which needs the global transform)."""
+    import torch
+    from lfm_hip import _abi
+    lib = _abi.load()
+    N, T = 48, 12
+    C = 3 if level == 2 else 1
+    log(f"two shares with halo exchange vs whole lattice, level {level}, N {N}, {T} steps")
+    p = _params(level, N)
+    if level == 2:
+        p.kappa_tube = 0.0
+        p.flags &= ~_abi.LFM_F_SA_SCV
+    g = torch.Generator(device="cuda").manual_seed(23 + level)
+    from lfm_hip.fields import boundary_mask_host
+    mask = torch.from_numpy(boundary_mask_host(N, 0.3)).cuda().view(1, N, N, N)
+    psi0 = [0.3 * torch.randn((C, N, N, N), generator=g, device="cuda") for _ in range(2)]
+    chi0 = 0.01 * torch.randn((1, N, N, N), generator=g, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    ptr = lambda t: t.data_ptr() if t is not None else None
+    fields = lambda: [torch.empty((C, N, N, N), device="cuda") for _ in range(2 if level else 1)] + \
+        [torch.empty((1, N, N, N), device="cuda")]
+
+    def step_whole(cur, prev, out, flags):
+        p.flags = flags
+        if level == 0:
+            rc = lib.lfm_step_real_f32(ptr(cur[0]), ptr(prev[0]), ptr(cur[1]), ptr(prev[1]), ptr(mask),
+                                       ptr(out[0]), None, ptr(out[1]), None, ctypes.byref(p), st)
+        elif level == 1:
+            rc = lib.lfm_step_complex_f32(ptr(cur[0]), ptr(prev[0]), ptr(cur[1]), ptr(prev[1]),
+                                          ptr(cur[2]), ptr(prev[2]), ptr(mask), ptr(out[0]), None,
+                                          ptr(out[1]), None, ptr(out[2]), None, ctypes.byref(p), st)
+        else:
+            rc = lib.lfm_step_color_f32(ptr(cur[0]), ptr(prev[0]), ptr(cur[1]), ptr(prev[1]),
+                                        ptr(cur[2]), ptr(prev[2]), ptr(mask), ptr(out[0]), None,
+                                        ptr(out[1]), None, ptr(out[2]), None, None, None, None,
+                                        ctypes.byref(p), st)
+        _abi.check(rc, "whole step")
+
+    base = p.flags & ~_abi.LFM_F_DAMP_PREV
+    damp = _abi.LFM_F_DAMP_PREV
+    # whole lattice: three slots, the first step reads prev undamped
+    init = [psi0[0], psi0[1]][:(2 if level else 1)] + [chi0]
+    slots = [[t.clone() for t in init], [t.clone() for t in init], fields()]
+    cur, prev, nxt = 0, 1, 2
+    for s in range(T):
+        step_whole(slots[cur], slots[prev], slots[nxt], base | (damp if s else 0))
+        cur, prev, nxt = nxt, cur, prev
+    whole = slots[cur]
+    # two shares: interior planes [0, 20) and [20, 48), each stored with one
+    # halo plane per side
+    bounds = [(0, 20), (20, N)]
+    shares = []
+    for a, b in bounds:
+        idx = torch.arange(a - 1, b + 1, device="cuda") % N
+        cut = lambda t: t.index_select(1, idx).contiguous()
+        sl = [[cut(t) for t in init], [cut(t) for t in init],
+              [torch.empty_like(cut(t)) for t in init]]
+        shares.append({"idx": idx, "slots": sl, "mask": cut(mask), "planes": b - a + 2})
+    cur, prev, nxt = 0, 1, 2
+    for s in range(T):
+        # halo exchange of the current fields: plane 0 of a share is the last
+        # interior plane of the share before it, its last plane the first
+        # interior plane of the share after it
+        for k, sh in enumerate(shares):
+            lo, hi = shares[k - 1], shares[(k + 1) % 2]
+            for f, t in enumerate(sh["slots"][cur]):
+                t[:, 0].copy_(lo["slots"][cur][f][:, -2])
+                t[:, -1].copy_(hi["slots"][cur][f][:, 1])
+        for sh in shares:
+            c, pv, o = sh["slots"][cur], sh["slots"][prev], sh["slots"][nxt]
+            pi_c, pi_p, pi_o = (c[1], pv[1], o[1]) if level else (None, None, None)
+            p.flags = base | (damp if s else 0)
+            rc = lib.lfm_step_slab_f32(level, sh["planes"], ptr(c[0]), ptr(pv[0]), ptr(pi_c),
+                                       ptr(pi_p), ptr(c[-1]), ptr(pv[-1]), ptr(sh["mask"]), None,
+                                       ptr(o[0]), ptr(pi_o), ptr(o[-1]), None, ctypes.byref(p), st)
+            _abi.check(rc, "slab step")
+        cur, prev, nxt = nxt, cur, prev
+    torch.cuda.synchronize()
+    for (a, b), sh in zip(bounds, shares):
+        for f, (t_sh, t_wh) in enumerate(zip(sh["slots"][cur], whole)):
+            assert torch.equal(t_sh[:, 1:-1], t_wh[:, a:b]), (a, b, f)
+    log(f"  both shares equal the whole lattice after {T} steps")
