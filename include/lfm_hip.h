@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define LFM_ABI_VERSION 5
+#define LFM_ABI_VERSION 6
 
 /* Error codes. */
 #define LFM_OK 0
@@ -63,6 +63,21 @@ extern "C" {
                                  (lfm/experiment/barrier.py:327-353), same bits.
                                  Single-step entry points (lfm_step_real_f32,
                                  _complex_f32, _color_f32) only */
+#define LFM_F_SOURCE 0x80     /* a plane source is added to the new Psi as it is
+                                 written (before the LFM_F_KEEP_PSI factor): at
+                                 every site whose coordinate along src_axis is
+                                 src_plane, Pr' += src_amp[2s] * src_env[t0*N + t1]
+                                 and Pi' += src_amp[2s+1] * src_env[t0*N + t1]
+                                 (each product and sum rounded once, a zero
+                                 amount adds nothing), t0 < t1 the two other
+                                 coordinates, s = *src_step + src_off the step
+                                 index (no addition when s is outside
+                                 [0, src_len)): the in-place
+                                 `psi[plane] += amount * envelope` of a continuous
+                                 source's per-step callback
+                                 (lfm/experiment/source.py:196-228), same bits.
+                                 lfm_step_real_f32 and _complex_f32 only (the
+                                 reference's source writes one N^3 component) */
 
 /* Number of tile shapes of the tiled kernel (lfm_params_t.tile = 1..this). */
 #define LFM_TILE_VARIANTS 6
@@ -73,7 +88,13 @@ extern "C" {
  *        one ({4,1} {4,2} {4,4} {8,1} {8,2} {16,1} wavefronts x rows/thread).
  * chunk: 0 = automatic, else the number of i planes one workgroup marches.
  * keep:  NULL, or the N^3 per-site factors of LFM_F_KEEP_PSI (device memory,
- *        read only; it may not overlap an output). */
+ *        read only; it may not overlap an output).
+ * src_*: the plane source of LFM_F_SOURCE (device memory, read only):
+ *        src_env N*N envelope values, src_amp 2*src_len amounts (real part,
+ *        imaginary part per step index), src_step one step counter the
+ *        caller advances on the device (so launches replayed from a graph
+ *        read their own step's amounts: each launch of a replayed sequence
+ *        gets its own src_off); src_axis 0..2, src_plane 0..N-1. */
 typedef struct lfm_params {
     int32_t N;
     int32_t flags;
@@ -81,6 +102,11 @@ typedef struct lfm_params {
     float kappa_c, epsilon_cc, kappa_string, kappa_tube;
     int32_t tile, chunk;
     const float* keep;
+    const float* src_env;
+    const float* src_amp;
+    const int64_t* src_step;
+    int64_t src_len;
+    int32_t src_axis, src_plane, src_off, src_reserved;
 } lfm_params_t;
 
 int lfm_abi_version(void);

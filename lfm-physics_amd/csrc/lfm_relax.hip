@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kThreads) void relax_step_kernel(const float* __res
         bool on;
         __device__ __forceinline__ float operator[](size_t t) const {
             const float v = p[t];
-            return on ? __fmul_rn(v, s) : v;
+            return on ? mul_rn(v, s) : v;
         }
     } const E{E_raw, s, scale};
     float m = 0.0f;
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kThreads) void relax_step_kernel(const float* __res
         // E + dt * (lap - chi2 * E), one rounding per operation as the
         // reference's elementwise array arithmetic
         const size_t t = pl + rc + k;
-        const float v = __fadd_rn(c, __fmul_rn(dt, __fsub_rn(lap, __fmul_rn(chi2[t], c))));
+        const float v = add_rn(c, mul_rn(dt, sub_rn(lap, mul_rn(chi2[t], c))));
         out[t] = v;
         m = fabsf(v);
     }
@@ -129,10 +129,10 @@ __global__ __launch_bounds__(kThreads) void relax_scale_kernel(float* __restrict
          t += size_t(gridDim.x) * kThreads) {
         float v = E[t];
         if (scale) {
-            v = __fmul_rn(v, s);
+            v = mul_rn(v, s);
             E[t] = v;
         }
-        if (E_prev) d = fmaxf(d, fabsf(__fsub_rn(v, E_prev[t])));
+        if (E_prev) d = fmaxf(d, fabsf(sub_rn(v, E_prev[t])));
     }
     if (clear_slot && blockIdx.x == 0 && threadIdx.x < kLines) clear_slot[threadIdx.x * kStride] = 0u;
     if (diff_slot) block_max_publish(d, diff_slot);

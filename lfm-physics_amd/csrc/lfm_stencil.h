@@ -1,7 +1,7 @@
 // The 19-point Laplacian (lfm/core/stencils.py:26-71, weights
 // lfm/constants.py:135-142) in the plane-partial form every kernel of the
-// library uses, with each rounding spelled out (fmaf and non-contractible
-// products) so kernels evaluating the same site in different code round
+// library uses, with each rounding spelled out (fmaf and the mul_rn / add_rn /
+// sub_rn below) so kernels evaluating the same site in different code round
 // identically:
 //   A(q) = 1/3 (4 in-plane faces) + 1/6 (4 in-plane edges) - 4 f
 //   B(q) = 1/3 f + 1/6 (4 in-plane faces)
@@ -12,21 +12,40 @@
 
 namespace lfm {
 
+// One float32 operation, rounded to nearest, never fused with a neighbouring
+// product or sum into a multiply-add: contraction is off inside these
+// functions, and the compiler fuses a product and a sum only when both allow
+// it.  (HIP's __fmul_rn / __fadd_rn / __fsub_rn are plain operators in
+// contractible code under the default -ffp-contract=fast-honor-pragmas, so
+// they do not give this guarantee.)
+__device__ __forceinline__ float mul_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ float add_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ float sub_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+
 constexpr float WF = 1.0f / 3.0f;  // face weight   (lfm/constants.py:135)
 constexpr float WE = 1.0f / 6.0f;  // edge weight   (lfm/constants.py:138)
 
 __device__ __forceinline__ float lap_part_a(float faces, float diag, float ctr) {
-    return fmaf(WF, faces, fmaf(WE, diag, __fmul_rn(-4.0f, ctr)));
+    return fmaf(WF, faces, fmaf(WE, diag, mul_rn(-4.0f, ctr)));
 }
 __device__ __forceinline__ float lap_part_b(float ctr, float faces) {
-    return fmaf(WF, ctr, __fmul_rn(WE, faces));
+    return fmaf(WF, ctr, mul_rn(WE, faces));
 }
 
 // |Psi_a|^2 = Pr^2 + Pi^2 (numpy_backend.py:340-341, the S_a transform's
 // input) with its roundings fixed, so the transform pass that forms it from
 // Psi and any kernel writing it for later use give the same bits.
 __device__ __forceinline__ float psi_sq(float r, float m) {
-    return fmaf(r, r, __fmul_rn(m, m));
+    return fmaf(r, r, mul_rn(m, m));
 }
 
 // The S_a colour-variance term of GOV-02 (kernel_source.py:187-197):
@@ -41,10 +60,10 @@ __device__ __forceinline__ float scv_term(float s0, float s1, float s2, float (&
     for (int q = 0; q < 3; ++q) {
         const float sv = s[q] > 0.0f ? s[q] : 0.0f;
         clip[q] = sv;
-        s_sum = __fadd_rn(s_sum, sv);
+        s_sum = add_rn(s_sum, sv);
         s_sq = fmaf(sv, sv, s_sq);
     }
-    return fmaf(-1.0f / 3.0f, __fmul_rn(s_sum, s_sum), s_sq);
+    return fmaf(-1.0f / 3.0f, mul_rn(s_sum, s_sum), s_sq);
 }
 
 }  // namespace lfm

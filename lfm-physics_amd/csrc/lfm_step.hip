@@ -37,7 +37,7 @@ namespace {
 // Current arithmetic with every rounding spelled out, as the Laplacian
 // partials in lfm_stencil.h: a b - c d (one current component)
 __device__ __forceinline__ float cross2(float a, float b, float c, float d) {
-    return fmaf(a, b, -__fmul_rn(c, d));
+    return fmaf(a, b, -mul_rn(c, d));
 }
 
 enum Level { REAL = 0, COMPLEX = 1, COLOR = 2 };
@@ -60,6 +60,10 @@ struct KParams {
     // planes [ib, ie) along i are computed (0, N for a whole lattice; a slab
     // share computes its interior planes and reads the two halo planes)
     int ib, ie;
+    // plane source (LFM_F_SOURCE): axis, plane, this launch's step offset and
+    // the number of step indexes in the amount table
+    int src_axis, src_plane, src_off;
+    long long src_len;
 };
 
 struct Ptrs {
@@ -80,6 +84,10 @@ struct Ptrs {
     float* sa_clip;      // NULL or max(sa, 0)
     float* psisq;        // NULL or |Psi'_a|^2 of the new fields
     const float* keep;   // NULL unless LFM_F_KEEP_PSI: per-site factor on Psi'
+    // NULL unless LFM_F_SOURCE: N*N envelope, 2 amounts per step index, step counter
+    const float* src_env;
+    const float* src_amp;
+    const long long* src_step;
     size_t n3;           // N^3
 };
 
@@ -183,67 +191,73 @@ __device__ __forceinline__ void site_compute(const KParams& kp, const PointData<
                                              float lap_chi, float jsum, float ccv,
                                              float (&nw)[Lv<L>::NPF], float& cn,
                                              float (&sclip)[3]) {
-    // Every rounding is explicit (fmaf, __fmul_rn, __fadd_rn, __fsub_rn), so
-    // the result does not depend on how the compiler contracts the code in a
-    // given kernel: all kernels produce bit-identical updates for identical
-    // inputs.
+    // Every rounding is explicit (fmaf, mul_rn, add_rn, sub_rn: none is
+    // fused into a multiply-add), so the result does not depend on the code
+    // around the call in a given kernel: all kernels produce bit-identical
+    // updates for identical inputs.
     constexpr int NPF = Lv<L>::NPF;
     constexpr int C = Lv<L>::NC;
     // chi_st is chi as stored: chi itself, or chi - chi0 with LFM_F_CHI_DELTA.
     const bool delta = (kp.flags & LFM_F_CHI_DELTA) != 0;
-    const float chi_c = delta ? __fadd_rn(kp.chi0, chi_st) : chi_st;
+    const float chi_c = delta ? add_rn(kp.chi0, chi_st) : chi_st;
     const bool has_mask = (kp.flags & LFM_F_MASK) != 0;
     const float m = has_mask ? pd.m : 0.0f;
-    const float a = __fsub_rn(1.0f, m);
-    const float chi_sq = __fmul_rn(chi_c, chi_c);
+    const float a = sub_rn(1.0f, m);
+    const float chi_sq = mul_rn(chi_c, chi_c);
 
     float avg[2] = {0.0f, 0.0f};
     if constexpr (L == COLOR) {
         if (kp.ecc > 0.0f) {
-            avg[0] = __fmul_rn(__fadd_rn(__fadd_rn(c[0], c[1]), c[2]), 1.0f / 3.0f);
-            avg[1] = __fmul_rn(__fadd_rn(__fadd_rn(c[3], c[4]), c[5]), 1.0f / 3.0f);
+            avg[0] = mul_rn(add_rn(add_rn(c[0], c[1]), c[2]), 1.0f / 3.0f);
+            avg[1] = mul_rn(add_rn(add_rn(c[3], c[4]), c[5]), 1.0f / 3.0f);
         }
     }
     const bool damp = has_mask && (kp.flags & LFM_F_DAMP_PREV);
 #pragma unroll
     for (int f = 0; f < NPF; ++f) {
         float pv = pd.prev[f];
-        if (damp) pv = __fmul_rn(pv, a);
-        // 2 c - pv + dt2 (lap - chi^2 c)
-        float v = fmaf(kp.dt2, fmaf(-chi_sq, c[f], lap[f]), __fsub_rn(__fmul_rn(2.0f, c[f]), pv));
+        if (damp) pv = mul_rn(pv, a);
+        // 2 c - pv + dt2 (lap - chi^2 c); 2 c is exact, so fmaf(2, c, -pv)
+        // rounds 2 c - pv once, as the subtraction does
+        float v = fmaf(kp.dt2, fmaf(-chi_sq, c[f], lap[f]), fmaf(2.0f, c[f], -pv));
         if constexpr (L == COLOR) {
             if (kp.ecc > 0.0f)
-                v = fmaf(-__fmul_rn(__fmul_rn(kp.dt2, kp.ecc), chi_sq), __fsub_rn(c[f], avg[f / C]), v);
+                v = fmaf(-mul_rn(mul_rn(kp.dt2, kp.ecc), chi_sq), sub_rn(c[f], avg[f / C]), v);
         }
-        nw[f] = __fmul_rn(v, a);
+        nw[f] = mul_rn(v, a);
     }
     cn = chi_st;
     if (kp.flags & LFM_F_FREEZE_CHI) return;
 
     const float kc0 = kp.kc0;
     float psi_sq;
-    float src;
-    float cv = 0.0f, scv = 0.0f;
+    // the coupling source (kappa/chi0) chi (|Psi|^2 + eps_w j - E0^2) as
+    // factor src_f = (kappa/chi0) chi times src_e, and the colour-variance
+    // term CV as cv_f psi_sq
+    float src_f, src_e;
+    float cv_f = 0.0f, scv = 0.0f;
     if constexpr (L == REAL) {
-        psi_sq = __fmul_rn(c[0], c[0]);
-        src = __fmul_rn(__fmul_rn(kc0, chi_c), __fsub_rn(psi_sq, kp.e0sq));
+        psi_sq = mul_rn(c[0], c[0]);
+        src_f = mul_rn(kc0, chi_c);
+        src_e = sub_rn(psi_sq, kp.e0sq);
     } else {
         float e[C > 0 ? C : 1];
         psi_sq = 0.0f;
 #pragma unroll
         for (int q = 0; q < C; ++q) {
-            e[q] = fmaf(c[q], c[q], __fmul_rn(c[q + C], c[q + C]));
-            psi_sq = __fadd_rn(psi_sq, e[q]);
+            e[q] = fmaf(c[q], c[q], mul_rn(c[q + C], c[q + C]));
+            psi_sq = add_rn(psi_sq, e[q]);
         }
-        const float j_total = __fmul_rn(0.5f, jsum);
-        src = __fmul_rn(__fmul_rn(kc0, chi_c), __fsub_rn(fmaf(kp.epsw, j_total, psi_sq), kp.e0sq));
+        const float j_total = mul_rn(0.5f, jsum);
+        src_f = mul_rn(kc0, chi_c);
+        src_e = sub_rn(fmaf(kp.epsw, j_total, psi_sq), kp.e0sq);
         if constexpr (L == COLOR) {
             if (kp.kc > 0.0f) {
-                const float total_sq = __fmul_rn(psi_sq, psi_sq);
+                const float total_sq = mul_rn(psi_sq, psi_sq);
                 if (total_sq > 1e-30f) {
-                    const float sum_sq = fmaf(e[0], e[0], fmaf(e[1], e[1], __fmul_rn(e[2], e[2])));
-                    const float f_c = __fsub_rn(__fdiv_rn(sum_sq, total_sq), 1.0f / 3.0f);
-                    cv = __fmul_rn(__fmul_rn(__fmul_rn(kp.kcc0, chi_c), f_c), psi_sq);
+                    const float sum_sq = fmaf(e[0], e[0], fmaf(e[1], e[1], mul_rn(e[2], e[2])));
+                    const float f_c = sub_rn(__fdiv_rn(sum_sq, total_sq), 1.0f / 3.0f);
+                    cv_f = mul_rn(mul_rn(kp.kcc0, chi_c), f_c);
                 }
             }
             if (scv_on) {
@@ -255,38 +269,69 @@ __device__ __forceinline__ void site_compute(const KParams& kp, const PointData<
     }
     // chi_self = -4 lam chi (chi^2 - chi0^2); with deviation storage the
     // factor chi^2 - chi0^2 is formed as d (2 chi0 + d), free of cancellation.
-    const float hat = delta ? __fmul_rn(chi_st, fmaf(2.0f, kp.chi0, chi_st))
+    const float hat = delta ? mul_rn(chi_st, fmaf(2.0f, kp.chi0, chi_st))
                             : fmaf(-kp.chi0, kp.chi0, chi_sq);
-    const float chi_self = __fmul_rn(__fmul_rn(-4.0f * kp.lam, chi_c), hat);
-    float accel = __fadd_rn(__fsub_rn(__fsub_rn(lap_chi, src), cv), chi_self);
+    // lap chi - src - CV + chi_self, each product folded into its sum (fmaf)
+    float accel = fmaf(-src_f, src_e, lap_chi);
+    if constexpr (L == COLOR) accel = fmaf(-cv_f, psi_sq, accel);
+    accel = fmaf(mul_rn(-4.0f * kp.lam, chi_c), hat, accel);
     if constexpr (L == COLOR) accel = fmaf(-kp.kt, scv, fmaf(-kp.ks, ccv, accel));
-    // leapfrog in storage units (a constant offset drops out of 2c - c_prev and of lap)
-    float cnv = fmaf(kp.dt2, accel, __fsub_rn(__fmul_rn(2.0f, chi_st), pd.chip));
+    // leapfrog in storage units (a constant offset drops out of 2c - c_prev and
+    // of lap); 2 chi_st is exact, so the fmaf rounds 2 chi_st - chi_prev once
+    float cnv = fmaf(kp.dt2, accel, fmaf(2.0f, chi_st, -pd.chip));
     const float floor_v = delta ? -2.0f * kp.chi0 : -kp.chi0;
     if (cnv < floor_v) cnv = floor_v;   // BH excision clamp chi >= -chi0 (NaN passes through)
-    cn = delta ? __fmul_rn(a, cnv) : fmaf(m, kp.chi0, __fmul_rn(a, cnv));   // frozen boundary: chi -> chi0
+    cn = delta ? mul_rn(a, cnv) : fmaf(m, kp.chi0, mul_rn(a, cnv));   // frozen boundary: chi -> chi0
+}
+
+// Plane source (LFM_F_SOURCE) at site (i, j, k): a continuous source's
+// `psi[plane] += amount * envelope` after the step, folded in.  The amounts of
+// step index *src_step + src_off are read only on the source plane.
+template <int L>
+__device__ __forceinline__ void plane_source(const KParams& kp, const Ptrs& P, int i, int j,
+                                             int k, float (&nw)[Lv<L>::NPF]) {
+    const int ax = kp.src_axis;
+    if ((ax == 0 ? i : ax == 1 ? j : k) != kp.src_plane) return;
+    const long long s = *P.src_step + kp.src_off;
+    if (s < 0 || s >= kp.src_len) return;
+    // transverse coordinates in ascending axis order
+    const int t0 = ax == 0 ? j : i, t1 = ax == 2 ? j : k;
+    const float e = P.src_env[t0 * kp.N + t1];
+    // product and sum rounded separately, as the callback's two array operations
+    const float ar = P.src_amp[2 * s];
+    if (ar != 0.0f) nw[0] = add_rn(nw[0], mul_rn(ar, e));
+    if constexpr (L == COMPLEX) {
+        const float ai = P.src_amp[2 * s + 1];
+        if (ai != 0.0f) nw[1] = add_rn(nw[1], mul_rn(ai, e));
+    }
 }
 
 // Per-site update with its stores (site_compute, then the outputs of P).
-// KEEP: the LFM_F_KEEP_PSI kernels (a separate instantiation, so the others
-// carry none of its code).
-template <int L, class O, bool KEEP = false>
+// EXTRA: the kernels for LFM_F_SOURCE and LFM_F_KEEP_PSI (a separate
+// instantiation, so the others carry none of their code); (i, j, k) is the
+// site, used by the plane source only.
+template <int L, class O, bool EXTRA = false>
 __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O idx,
                                             const PointData<L>& pd,
                                             const float (&c)[Lv<L>::NPF],
                                             const float (&lap)[Lv<L>::NPF], float chi_st,
-                                            float lap_chi, float jsum, float ccv,
-                                            float* ret3 = nullptr) {
+                                            float lap_chi, float jsum, float ccv, int si, int sj,
+                                            int sk, float* ret3 = nullptr) {
     constexpr int NPF = Lv<L>::NPF;
     constexpr int C = Lv<L>::NC;
     const size_t n3 = P.n3;
     float nw[NPF], cn, sclip[3];
     site_compute<L>(kp, pd, P.sa != nullptr, c, lap, chi_st, lap_chi, jsum, ccv, nw, cn, sclip);
-    if constexpr (KEEP) {
-        // a driver's `psi *= factor` after the step, folded in
-        const float k = idx.ldp(P.keep);
+    if constexpr (EXTRA && L != COLOR) {
+        if (P.src_env) plane_source<L>(kp, P, si, sj, sk, nw);
+    }
+    if constexpr (EXTRA) {
+        if (P.keep) {
+            // a driver's `psi *= factor` after the step, folded in
+            const float k = idx.ldp(P.keep);
 #pragma unroll
-        for (int f = 0; f < NPF; ++f) nw[f] = __fmul_rn(nw[f], k);
+            for (int f = 0; f < NPF; ++f) nw[f] = mul_rn(nw[f], k);
+        }
     }
     const float a = 1.0f - ((kp.flags & LFM_F_MASK) ? pd.m : 0.0f);
 #pragma unroll
@@ -319,7 +364,7 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
 // Direct kernel: one thread per site, on a (k, j, i) grid of 64 x 4
 // workgroups, so the coordinates come from the launch.
 // ---------------------------------------------------------------------------
-template <int L, bool KEEP = false>
+template <int L, bool EXTRA = false>
 __global__ __launch_bounds__(256) void step_direct(Ptrs P, KParams kp) {
     constexpr int NPF = Lv<L>::NPF;
     constexpr int C = Lv<L>::NC;
@@ -380,8 +425,8 @@ __global__ __launch_bounds__(256) void step_direct(Ptrs P, KParams kp) {
             }
         }
     }
-    site_update<L, Off64, KEEP>(kp, P, Off64{idx}, load_point<L>(kp, P, Off64{idx}), c, lap,
-                                chi_c, lap_chi, jsum, ccv);
+    site_update<L, Off64, EXTRA>(kp, P, Off64{idx}, load_point<L>(kp, P, Off64{idx}), c, lap,
+                                chi_c, lap_chi, jsum, ccv, i, j, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -397,7 +442,7 @@ constexpr int TK = 64;
 
 // One workgroup's work: the column tile at (k0, j0) over planes [i0, i1) of
 // one lattice (P, kp).  Shared by the single-lattice and batched kernels.
-template <int L, int BJ, int RJ, bool NT, bool KEEP = false>
+template <int L, int BJ, int RJ, bool NT, bool EXTRA = false>
 __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int k0, int j0,
                                           int i0, int i1) {
     constexpr int NPF = Lv<L>::NPF;
@@ -566,9 +611,9 @@ __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int 
                         if (kp.ks > 0.0f) ccv = ccvyz[rr] + (qx - (1.0f / 3.0f) * sx * sx);
                     }
                 }
-                site_update<L, Off32T<NT>, KEEP>(kp, P, Off32T<NT>{pend_pb + ioff0 + rr * rowB},
+                site_update<L, Off32T<NT>, EXTRA>(kp, P, Off32T<NT>{pend_pb + ioff0 + rr * rowB},
                                                  ptc[rr], cpend, lap, chim1[rr], lap_chi, jsum,
-                                                 ccv);
+                                                 ccv, q - 1, j0 + ty * RJ + rr, k0 + tx);
             }
             // Advance the queue.
 #pragma unroll
@@ -586,7 +631,7 @@ __device__ __forceinline__ void tile_body(const Ptrs& P, const KParams& kp, int 
     }
 }
 
-template <int L, int BJ, int RJ, bool NT, bool KEEP = false>
+template <int L, int BJ, int RJ, bool NT, bool EXTRA = false>
 __global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nkt, int njt, int CH) {
     // XCD-aware block remap: blocks are dispatched round-robin over the 8
     // XCDs; this makes consecutive tiles of one i chunk share an XCD (and its
@@ -599,7 +644,7 @@ __global__ __launch_bounds__(64 * BJ) void step_tiled(Ptrs P, KParams kp, int nk
     const int jt = rest % njt;
     const int ch = rest / njt;
     const int i0 = kp.ib + ch * CH;
-    tile_body<L, BJ, RJ, NT, KEEP>(P, kp, kt * TK, jt * BJ * RJ, i0, min(i0 + CH, kp.ie));
+    tile_body<L, BJ, RJ, NT, EXTRA>(P, kp, kt * TK, jt * BJ * RJ, i0, min(i0 + CH, kp.ie));
 }
 
 // Batched sweep: one launch advances many independent lattices.  Each
@@ -661,7 +706,7 @@ int choose_chunk(int N, int tiles, int min_ch = 8, int target = 8192) {
     return ch < N ? ch : N;
 }
 
-template <int L, int BJ, int RJ, bool NT, bool KEEP = false>
+template <int L, int BJ, int RJ, bool NT, bool EXTRA = false>
 int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
     constexpr int TJ = BJ * RJ;
     const int N = kp.N;
@@ -671,7 +716,7 @@ int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
     const int CH = kp.chunk > 0 ? min(kp.chunk, span) : choose_chunk(span, nkt * njt, NT ? 8 : 2);
     const int nch = (span + CH - 1) / CH;
     const long long nblocks = (long long)nkt * njt * nch;
-    hipLaunchKernelGGL((step_tiled<L, BJ, RJ, NT, KEEP>), dim3((unsigned)nblocks), dim3(64 * BJ), 0,
+    hipLaunchKernelGGL((step_tiled<L, BJ, RJ, NT, EXTRA>), dim3((unsigned)nblocks), dim3(64 * BJ), 0,
                        st, P, kp, nkt, njt, CH);
     LFM_LAUNCH_CHECK("step_tiled");
     return LFM_OK;
@@ -687,17 +732,17 @@ bool streaming(size_t sites) {
 
 template <int L>
 int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
-    const bool keep = (kp.flags & LFM_F_KEEP_PSI) != 0;
+    const bool extra = (kp.flags & (LFM_F_KEEP_PSI | LFM_F_SOURCE)) != 0;
     if ((kp.flags & LFM_F_DIRECT) || kp.N > 1024) {
         const dim3 grid(unsigned((kp.N + 63) / 64), unsigned((kp.N + 3) / 4), unsigned(kp.N));
-        if (keep) hipLaunchKernelGGL((step_direct<L, true>), grid, dim3(64, 4), 0, st, P, kp);
+        if (extra) hipLaunchKernelGGL((step_direct<L, true>), grid, dim3(64, 4), 0, st, P, kp);
         else hipLaunchKernelGGL((step_direct<L>), grid, dim3(64, 4), 0, st, P, kp);
         LFM_LAUNCH_CHECK("step_direct");
         return LFM_OK;
     }
     const bool nt = streaming<L>(P.n3);
-    if (keep) {
-        // the LFM_F_KEEP_PSI kernels come in one tile shape, {4 x 2}
+    if (extra) {
+        // the LFM_F_KEEP_PSI / LFM_F_SOURCE kernels come in one tile shape, {4 x 2}
         return nt ? launch_tiled<L, 4, 2, true, true>(P, kp, st)
                   : launch_tiled<L, 4, 2, false, true>(P, kp, st);
     }
@@ -977,7 +1022,7 @@ __global__ __launch_bounds__(N) void step_color_kfft(Ptrs P, KParams kp, float2*
             for (int c = 0; c < C; ++c) pd.sa[c] = S.sab[(q - 1) & 1][c][k];
             float e2[C];
             site_update<L>(kp, P, Off32T<NT>{uint32_t(q - 1) * planeB + ioff}, pd, cpend, lap,
-                           chim1, lap_chi, jsum, ccv, e2);
+                           chim1, lap_chi, jsum, ccv, 0, 0, 0, e2);
 #pragma unroll
             for (int c = 0; c < C; ++c) S.fz[c][k] = e2[c];
         }
@@ -1461,12 +1506,43 @@ KParams to_k(const lfm_params_t* p) {
     k.chi0 = p->chi0; k.e0sq = p->e0_sq; k.epsw = p->epsilon_w; k.kc = p->kappa_c;
     k.ecc = p->epsilon_cc; k.ks = p->kappa_string; k.kt = p->kappa_tube;
     k.tile = p->tile; k.chunk = p->chunk;
-    // IEEE single division, round to nearest: bit-equal to __fdiv_rn
+    // IEEE single division, round to nearest: bit-equal to a device division
     k.kc0 = p->kappa / p->chi0;
     k.kcc0 = p->kappa_c / p->chi0;
     k.ib = 0;
     k.ie = p->N;
+    const bool src = (p->flags & LFM_F_SOURCE) != 0;
+    k.src_axis = src ? p->src_axis : 0;
+    k.src_plane = src ? p->src_plane : -1;
+    k.src_off = src ? p->src_off : 0;
+    k.src_len = src ? p->src_len : 0;
     return k;
+}
+
+// LFM_F_SOURCE arguments: pointers given, axis and plane in range.  Fills
+// the three source pointers (NULL without the flag) and their float counts
+// for the overlap checks.
+int source_check(const lfm_params_t* p, const float* (&sp)[3], size_t (&sn)[3], const char* who) {
+    sp[0] = sp[1] = sp[2] = nullptr;
+    sn[0] = sn[1] = sn[2] = 0;
+    if (!(p->flags & LFM_F_SOURCE)) return LFM_OK;
+    if (!p->src_env || !p->src_amp || !p->src_step) {
+        set_error("%s: LFM_F_SOURCE set but src_env, src_amp or src_step is NULL", who);
+        return LFM_E_ARG;
+    }
+    if (p->src_axis < 0 || p->src_axis > 2 || p->src_plane < 0 || p->src_plane >= p->N) {
+        set_error("%s: source axis %d / plane %d out of range (axis 0..2, plane 0..N-1)", who,
+                  p->src_axis, p->src_plane);
+        return LFM_E_ARG;
+    }
+    if (p->src_len < 0 || p->src_len > (int64_t(1) << 40)) {
+        set_error("%s: src_len %lld out of range", who, (long long)p->src_len);
+        return LFM_E_ARG;
+    }
+    sp[0] = p->src_env; sn[0] = size_t(p->N) * p->N;
+    sp[1] = p->src_amp; sn[1] = size_t(2 * p->src_len);
+    sp[2] = reinterpret_cast<const float*>(p->src_step); sn[2] = 2;
+    return LFM_OK;
 }
 
 // Argument validation common to all levels.  `ins`/`outs` list (pointer,
@@ -1518,16 +1594,21 @@ extern "C" int lfm_step_real_f32(const float* E, const float* E_prev, const floa
     }
     const size_t n3 = size_t(p->N) * p->N * p->N;
     const float* keep = (p->flags & LFM_F_KEEP_PSI) ? p->keep : nullptr;
-    const float* ins[] = {E, E_prev, chi, chi_prev, mask, keep};
-    const size_t nin[] = {n3, n3, n3, n3, n3, n3};
+    const float* sp[3];
+    size_t sn[3];
+    if (int rc = source_check(p, sp, sn, "lfm_step_real_f32")) return rc;
+    const float* ins[] = {E, E_prev, chi, chi_prev, mask, keep, sp[0], sp[1], sp[2]};
+    const size_t nin[] = {n3, n3, n3, n3, n3, n3, sn[0], sn[1], sn[2]};
     float* outs[] = {E_out, E_prev_out, chi_out, chi_prev_out};
     const size_t nout[] = {n3, n3, n3, n3};
-    int rc = validate(p, ins, nin, 6, outs, nout, 4, "lfm_step_real_f32");
+    int rc = validate(p, ins, nin, 9, outs, nout, 4, "lfm_step_real_f32");
     if (rc) return rc;
     Ptrs P{};
     P.pr = E; P.prp = E_prev; P.chi = chi; P.chip = chi_prev;
     P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
     P.keep = keep;
+    P.src_env = sp[0]; P.src_amp = sp[1];
+    P.src_step = reinterpret_cast<const long long*>(p->src_step);
     P.pr_out = E_out; P.prp_out = E_prev_out; P.chi_out = chi_out; P.chip_out = chi_prev_out;
     P.n3 = n3;
     return launch<REAL>(P, to_k(p), (hipStream_t)stream);
@@ -1558,16 +1639,21 @@ extern "C" int lfm_step_complex_f32(const float* pr, const float* pr_prev, const
     }
     const size_t n3 = size_t(p->N) * p->N * p->N;
     const float* keep = (p->flags & LFM_F_KEEP_PSI) ? p->keep : nullptr;
-    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask, keep};
-    const size_t nin[] = {n3, n3, n3, n3, n3, n3, n3, n3};
+    const float* sp[3];
+    size_t sn[3];
+    if (int rc = source_check(p, sp, sn, "lfm_step_complex_f32")) return rc;
+    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask, keep, sp[0], sp[1], sp[2]};
+    const size_t nin[] = {n3, n3, n3, n3, n3, n3, n3, n3, sn[0], sn[1], sn[2]};
     float* outs[] = {pr_out, pr_prev_out, pi_out, pi_prev_out, chi_out, chi_prev_out};
     const size_t nout[] = {n3, n3, n3, n3, n3, n3};
-    int rc = validate(p, ins, nin, 8, outs, nout, 6, "lfm_step_complex_f32");
+    int rc = validate(p, ins, nin, 11, outs, nout, 6, "lfm_step_complex_f32");
     if (rc) return rc;
     Ptrs P{};
     P.pr = pr; P.pi = pi; P.prp = pr_prev; P.pip = pi_prev; P.chi = chi; P.chip = chi_prev;
     P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
     P.keep = keep;
+    P.src_env = sp[0]; P.src_amp = sp[1];
+    P.src_step = reinterpret_cast<const long long*>(p->src_step);
     P.pr_out = pr_out; P.pi_out = pi_out; P.prp_out = pr_prev_out; P.pip_out = pi_prev_out;
     P.chi_out = chi_out; P.chip_out = chi_prev_out;
     P.n3 = n3;
@@ -1575,8 +1661,9 @@ extern "C" int lfm_step_complex_f32(const float* pr, const float* pr_prev, const
 }
 
 static int step2_check(const lfm_params_t* p, const char* fn) {
-    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI | LFM_F_KEEP_PSI)) {
-        set_error("%s: LFM_F_DIRECT / LFM_F_FREEZE_CHI / LFM_F_KEEP_PSI not supported", fn);
+    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI | LFM_F_KEEP_PSI | LFM_F_SOURCE)) {
+        set_error("%s: LFM_F_DIRECT / LFM_F_FREEZE_CHI / LFM_F_KEEP_PSI / LFM_F_SOURCE not "
+                  "supported", fn);
         return LFM_E_ARG;
     }
     if (p->N > 1024) {
@@ -1682,6 +1769,10 @@ extern "C" int lfm_step_color_f32(const float* pr, const float* pr_prev, const f
                   "(the S_a fields themselves are not passed)");
         return LFM_E_ARG;
     }
+    if (p->flags & LFM_F_SOURCE) {
+        set_error("lfm_step_color_f32: LFM_F_SOURCE is for the REAL and COMPLEX levels");
+        return LFM_E_ARG;
+    }
     if ((p->flags & LFM_F_KEEP_PSI) && !p->keep) {
         set_error("lfm_step_color_f32: LFM_F_KEEP_PSI set but params.keep is NULL");
         return LFM_E_ARG;
@@ -1738,8 +1829,8 @@ extern "C" int lfm_step_slab_f32(int32_t level, int32_t planes, const float* pr,
         set_error("lfm_step_slab_f32: LFM_F_MASK set but mask is NULL");
         return LFM_E_ARG;
     }
-    if (p->flags & (LFM_F_DIRECT | LFM_F_KEEP_PSI)) {
-        set_error("lfm_step_slab_f32: LFM_F_DIRECT / LFM_F_KEEP_PSI not supported");
+    if (p->flags & (LFM_F_DIRECT | LFM_F_KEEP_PSI | LFM_F_SOURCE)) {
+        set_error("lfm_step_slab_f32: LFM_F_DIRECT / LFM_F_KEEP_PSI / LFM_F_SOURCE not supported");
         return LFM_E_ARG;
     }
     const bool scv = level == 2 && p->kappa_tube > 0.0f;
@@ -1814,9 +1905,10 @@ extern "C" int lfm_step_color_fused_f32(const float* pr, const float* pr_prev, c
         set_error("lfm_step_color_fused_f32: needs kappa_tube > 0 (use lfm_step_color_f32)");
         return LFM_E_ARG;
     }
-    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI | LFM_F_SA_SCV | LFM_F_KEEP_PSI)) {
+    if (p->flags &
+        (LFM_F_DIRECT | LFM_F_FREEZE_CHI | LFM_F_SA_SCV | LFM_F_KEEP_PSI | LFM_F_SOURCE)) {
         set_error("lfm_step_color_fused_f32: LFM_F_DIRECT / LFM_F_FREEZE_CHI / LFM_F_SA_SCV / "
-                  "LFM_F_KEEP_PSI not supported");
+                  "LFM_F_KEEP_PSI / LFM_F_SOURCE not supported");
         return LFM_E_ARG;
     }
     ScvView v{};
@@ -1950,8 +2042,8 @@ extern "C" int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_p
         set_error("lfm_batch_step_f32: LFM_F_SA_SCV is for COLOR batches");
         return LFM_E_ARG;
     }
-    if (flags & LFM_F_KEEP_PSI) {
-        set_error("lfm_batch_step_f32: LFM_F_KEEP_PSI is not supported");
+    if (flags & (LFM_F_KEEP_PSI | LFM_F_SOURCE)) {
+        set_error("lfm_batch_step_f32: LFM_F_KEEP_PSI / LFM_F_SOURCE are not supported");
         return LFM_E_ARG;
     }
     const size_t ns = size_t(P->total) * (P->level == 2 ? 3 : 1);

@@ -25,7 +25,8 @@ LFM_F_DIRECT = 0x08
 LFM_F_CHI_DELTA = 0x10
 LFM_F_SA_SCV = 0x20
 LFM_F_KEEP_PSI = 0x40
-LFM_ABI_VERSION = 5
+LFM_F_SOURCE = 0x80
+LFM_ABI_VERSION = 6
 LFM_TILE_VARIANTS = 6
 LFM_RELAX_SLOT_WORDS = 2048
 
@@ -84,7 +85,10 @@ class LfmParams(ctypes.Structure):
                 ("dt2", _f32), ("kappa", _f32), ("lambda_self", _f32), ("chi0", _f32),
                 ("e0_sq", _f32), ("epsilon_w", _f32), ("kappa_c", _f32),
                 ("epsilon_cc", _f32), ("kappa_string", _f32), ("kappa_tube", _f32),
-                ("tile", _i32), ("chunk", _i32), ("keep", _vp)]
+                ("tile", _i32), ("chunk", _i32), ("keep", _vp),
+                ("src_env", _vp), ("src_amp", _vp), ("src_step", _vp), ("src_len", _i64),
+                ("src_axis", _i32), ("src_plane", _i32), ("src_off", _i32),
+                ("src_reserved", _i32)]
 
 
 class LfmError(RuntimeError):

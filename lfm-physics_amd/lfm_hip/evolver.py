@@ -158,6 +158,8 @@ class Evolver:
             self._sa = None
         self._params = {}
         self._keep = None      # set_psi_keep: per-site factor folded into every step
+        self._src = None       # set_plane_source: plane source folded into every step
+        self._pending = 0      # steps launched since the device step counter was advanced
         self.tile = 0      # tiled-kernel shape (0 = default, see autotune())
         self.chunk = 0     # i planes per workgroup (0 = automatic)
         self._partials = None
@@ -177,7 +179,10 @@ class Evolver:
         keep = self._keep.data_ptr() if self._keep is not None else 0
         if keep:
             flags |= _abi.LFM_F_KEEP_PSI
-        key = (flags, self.tile, self.chunk, keep)
+        src = self._src
+        if src is not None:
+            flags |= _abi.LFM_F_SOURCE
+        key = (flags, self.tile, self.chunk, keep, src["table"].data_ptr() if src else 0)
         p = self._params.get(key)
         if p is None:
             c = self.config
@@ -185,8 +190,78 @@ class Evolver:
                        c.kappa_c, c.epsilon_cc, c.kappa_string,
                        c.kappa_tube if self.sa_enabled else 0.0, flags, self.tile, self.chunk)
             p.keep = keep or None
+            if src is not None:
+                p.src_env = src["env"].data_ptr()
+                p.src_amp = src["table"].data_ptr()
+                p.src_step = self._step_dev.data_ptr()
+                p.src_len = src["table"].numel() // 2
+                p.src_axis, p.src_plane = src["axis"], src["plane"]
             self._params[key] = p
+        if src is not None:
+            # this launch's step index is the device counter + src_off
+            p.src_off = self._pending + 1
         return p
+
+    def set_plane_source(self, axis: int, plane: int, envelope, amounts) -> None:
+        """Add a plane source to Psi inside each step (REAL and COMPLEX), or
+        stop it with envelope=None.  After the step that brings the step count
+        to s, every site whose coordinate along `axis` is `plane` gets
+        Pr += a_r(s) * envelope[t0, t1] and Pi += a_i(s) * envelope[t0, t1]
+        (t0 < t1 the other two coordinates; products and sums in float32, a
+        zero amount adds nothing), where amounts(s) -> (a_r, a_i), rounded to
+        float32 on the host.  It is the `psi[plane] += amount * envelope` a
+        continuous source's step callback applies to the current buffers
+        (lfm/experiment/source.py:196-228), done by the step kernel as it
+        writes Psi, bit for bit the same (before a set_psi_keep factor, as a
+        source callback runs before a barrier's).  The amounts are tabulated
+        on the device ahead of each evolve call, so steps replayed from graphs
+        read their own step's amounts.  Steps then run one at a time."""
+        if envelope is None:
+            self._src = None
+            self._graphs.clear()
+            return
+        if self.level == 2:
+            raise ValueError("a plane source is for the REAL and COMPLEX levels (the reference's "
+                             "source writes one N^3 component)")
+        N = self.N
+        axis, plane = int(axis), int(plane)
+        if axis not in (0, 1, 2) or not 0 <= plane < N:
+            raise ValueError(f"source axis {axis} / plane {plane} out of range for N={N}")
+        env = torch.as_tensor(np.asarray(envelope, dtype=np.float32) if not torch.is_tensor(envelope)
+                              else envelope, dtype=torch.float32, device=self.device)
+        if env.numel() != N * N:
+            raise ValueError(f"envelope must hold N^2 = {N * N} values, got {env.numel()}")
+        if not callable(amounts):
+            raise ValueError("amounts must be a callable step -> (real amount, imaginary amount)")
+        if not hasattr(self, "_step_dev"):
+            self._step_dev = torch.zeros((), dtype=torch.int64, device=self.device)
+        self._src = {"axis": axis, "plane": plane, "env": env.reshape(-1).contiguous().clone(),
+                     "fn": amounts, "table": torch.zeros(2, device=self.device)}
+        self._graphs.clear()
+
+    def _source_table(self, last: int) -> None:
+        """Make the source's amount table cover step indexes 0..last."""
+        src = self._src
+        have = src["table"].numel() // 2
+        if last < have:
+            return
+        cap = max(2 * have, last + 1, 1024)
+        amt = np.empty((cap, 2), np.float32)
+        old = src["table"].cpu().numpy().reshape(-1, 2)
+        amt[:have] = old
+        fn = src["fn"]
+        for s in range(have, cap):
+            a_r, a_i = fn(s)
+            amt[s] = (a_r, a_i)
+        src["table"] = torch.from_numpy(amt.reshape(-1)).to(self.device)
+        self._graphs.clear()     # the captured launches point at the old table
+
+    def _advance_counter(self) -> None:
+        """Add the steps launched since the last call to the device step
+        counter (read by a device hook and by the plane source)."""
+        if self._pending and hasattr(self, "_step_dev"):
+            self._step_dev.add_(self._pending)
+        self._pending = 0
 
     def set_psi_keep(self, keep) -> None:
         """Multiply every new Psi value by keep[site] inside each step (all
@@ -220,7 +295,7 @@ class Evolver:
         results = {}
         old = (self.tile, self.chunk, self.kernel_events)
         N = self.N
-        pairs = self.two_step and N <= 1024 and self._keep is None
+        pairs = self.two_step and N <= 1024 and self._keep is None and self._src is None
         if self._fused:
             # the fused COLOR kernel has one tile shape (a full k-row); tune its chunk
             combos = [(0, ch) for ch in dict.fromkeys((0, N, N // 2, N // 4, N // 8)) if ch == 0 or ch >= 4]
@@ -348,6 +423,7 @@ class Evolver:
         if ev0 is not None:
             self._mark(ev0)
         _abi.check(rc, "step")
+        self._pending += 1
         self._prev, self._cur = cur, nxt
         self._damped = True
         if freeze_chi:
@@ -429,12 +505,13 @@ class Evolver:
     def _sa_flags(self):
         return (self._psisq_valid, self._spec_valid, self._sa_stale) if self.sa_enabled else ()
 
-    def _hooked_step(self, hook) -> None:
-        """One step, then (with a device hook) the device step counter and
-        the hook, as evolve runs them."""
-        self._step()
+    def _hooked_step(self, hook, freeze_chi: bool = False) -> None:
+        """One step, then (with a device hook or a plane source) the device
+        step counter and the hook, as evolve runs them."""
+        self._step(freeze_chi=freeze_chi)
+        if hook is not None or self._src is not None:
+            self._advance_counter()
         if hook is not None:
-            self._step_dev.add_(1)
             hook(self, self._step_dev)
 
     def _cycle_graph(self, cycles: int = 1, hook=None, freeze: bool = False):
@@ -453,7 +530,7 @@ class Evolver:
         if self.level == 2 and self.sa_enabled and self._scv.method != 2:
             return None              # rocFFT launches stay out of graphs
         pair = (self.two_step and self.N <= 1024 and hook is None and self._keep is None
-                and not freeze)
+                and self._src is None and not freeze)
         key0 = self._key()
         k, m = key0, 0
         while True:
@@ -473,15 +550,21 @@ class Evolver:
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))
         g = torch.cuda.CUDAGraph()
+        self._pending = 0
         with torch.cuda.stream(s):
             with torch.cuda.graph(g, stream=s):
                 for _ in range(m):
                     if pair:
                         self._step2()
-                    elif freeze:
-                        self._step(freeze_chi=True)
-                    else:
+                    elif hook is not None:
                         self._hooked_step(hook)
+                    else:
+                        # a plane source's launches read counter + 1 .. m;
+                        # the counter advances once per replay
+                        self._step(freeze_chi=freeze)
+                if self._src is not None:
+                    self._advance_counter()
+        self._pending = 0
         torch.cuda.current_stream(self.device).wait_stream(s)
         # Capture recorded the launches without running them.  The bookkeeping
         # it advanced is back at the start (a whole cycle); when the S_a flags
@@ -518,13 +601,16 @@ class Evolver:
         left = int(steps)
         hook = device_hook
         pair_ok = (self.two_step and not freeze_chi and self.N <= 1024 and hook is None
-                   and self._keep is None)
-        if hook is not None:
-            if freeze_chi:
-                raise ValueError("device_hook with freeze_chi is not supported")
+                   and self._keep is None and self._src is None)
+        if hook is not None and freeze_chi:
+            raise ValueError("device_hook with freeze_chi is not supported")
+        if self._src is not None:
+            self._source_table(self.step + left)
+        if hook is not None or self._src is not None:
             if not hasattr(self, "_step_dev"):
                 self._step_dev = torch.zeros((), dtype=torch.int64, device=self.device)
             self._step_dev.fill_(self.step)
+        self._pending = 0
         while left > 0:
             to_report = report - self.step % report if (callback is not None and report > 0) else left
             cyc = self._cycle_graph(hook=hook, freeze=freeze_chi)
@@ -547,10 +633,7 @@ class Evolver:
                 self.step += 2
                 left -= 2
             else:
-                if hook is None:
-                    self._step(freeze_chi=freeze_chi)
-                else:
-                    self._hooked_step(hook)
+                self._hooked_step(hook, freeze_chi)
                 self.step += 1
                 left -= 1
             if callback is not None and report > 0 and self.step % report == 0:

@@ -55,7 +55,7 @@ def test_binding_arity_matches_header():
 def test_abi_version_and_error_string():
     from lfm_hip import _abi
     lib = _abi.load()
-    assert lib.lfm_abi_version() == _abi.LFM_ABI_VERSION == 5
+    assert lib.lfm_abi_version() == _abi.LFM_ABI_VERSION == 6
     # argument validation happens before any device work: N < 8 is refused
     p = _abi.LfmParams(4, 0, 4e-4, 1 / 63, 0, 19, 0, 0.1, 0, 0, 0, 0)
     dummy = [1 << 20] * 4
@@ -139,6 +139,94 @@ def test_slab_step_validation():
     assert rc == -1 and b"S_a" in lib.lfm_last_error()
 
 
+def test_plane_source_validation():
+    """LFM_F_SOURCE is checked before any device work: its pointers, axis,
+    plane and table length, an output overlapping the envelope, and the entry
+    points that do not take it (COLOR, two-step, slab share)."""
+    from lfm_hip import _abi
+    lib = _abi.load()
+    N = 16
+    n3 = N ** 3
+    ptr = [(1 << 24) + q * 4 * 3 * n3 for q in range(16)]
+    env, amp, cnt = ptr[10], ptr[11], ptr[12]
+
+    def params(**kw):
+        p = _abi.LfmParams(N, _abi.LFM_F_SOURCE, 4e-4, 1 / 63, 0, 19, 0, 0.1, 0, 0, 0, 0)
+        p.src_env, p.src_amp, p.src_step, p.src_len = env, amp, cnt, 100
+        p.src_axis, p.src_plane, p.src_off = 2, 4, 1
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+    def real(p, out=ptr[6]):
+        return lib.lfm_step_real_f32(ptr[0], ptr[1], ptr[2], ptr[3], None, out, None, ptr[7], None,
+                                     ctypes.byref(p), None)
+
+    def cplx(p):
+        return lib.lfm_step_complex_f32(ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5], None,
+                                        ptr[6], None, ptr[7], None, ptr[8], None, ctypes.byref(p),
+                                        None)
+    for kw, what in (({"src_env": None}, b"src_env"), ({"src_step": None}, b"src_env"),
+                     ({"src_axis": 3}, b"axis"), ({"src_plane": N}, b"plane"),
+                     ({"src_plane": -1}, b"plane"), ({"src_len": -1}, b"src_len")):
+        for fn in (real, cplx):
+            rc = fn(params(**kw))
+            assert rc == -1 and what in lib.lfm_last_error(), (kw, fn)
+    # an output over the envelope, the amounts or the counter
+    for out in (env, amp + 64, cnt):
+        rc = real(params(), out=out)
+        assert rc == -1 and b"overlaps" in lib.lfm_last_error(), out
+    p = params()
+    rc = lib.lfm_step_color_f32(ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5], None, ptr[6], None,
+                                ptr[7], None, ptr[8], None, None, None, None, ctypes.byref(p), None)
+    assert rc == -1 and b"LFM_F_SOURCE" in lib.lfm_last_error()
+    rc = lib.lfm_step2_real_f32(ptr[0], ptr[1], ptr[2], ptr[3], None, ptr[6], ptr[7], ptr[8],
+                                ptr[9], ctypes.byref(p), None)
+    assert rc == -1 and b"LFM_F_SOURCE" in lib.lfm_last_error()
+    rc = lib.lfm_step_slab_f32(1, 8, ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5], None, None,
+                               ptr[6], ptr[7], ptr[8], None, ctypes.byref(p), None)
+    assert rc == -1 and b"LFM_F_SOURCE" in lib.lfm_last_error()
+
+
+def test_continuous_source_amounts():
+    """ContinuousSource's amounts and envelope follow the reference's formulas
+    (lfm/experiment/source.py:131-149,196-228): injection
+    dt^2 A sin(omega t) boost rounded to float32, skipped below 1e-30, the
+    imaginary amount scaled by sin(phase) only at COMPLEX with a phase; the
+    envelope a float32 Gaussian about N/2 (peak 1 at the centre cell)."""
+    import math
+    import types
+    import lfm_hip as L
+
+    class FakeSim:
+        def __init__(self, level, N):
+            self.config = types.SimpleNamespace(grid_size=N, dt=0.02)
+            self.evolver = types.SimpleNamespace(level=level)
+
+        def _to_device(self, arr):
+            return np.asarray(arr, np.float32).ravel()
+    N = 32
+    s = L.ContinuousSource(FakeSim(1, N), axis=0, position=0.25, omega=21.0, amplitude=2.0,
+                           envelope_sigma=0.25, phase=1.0, boost=10.0)
+    assert s.position == 8
+    env = s.envelope
+    assert env.dtype == np.float32 and env.shape == (N, N) and env[N // 2, N // 2] == 1.0
+    # symmetric about N/2 and Gaussian of width N/4
+    np.testing.assert_array_equal(env[N // 2 + 3], env[N // 2 - 3])
+    assert abs(float(env[N // 2, N // 2 + 8]) - math.exp(-0.5)) < 1e-6
+    for step in (1, 7, 250):
+        inj = 0.02 ** 2 * 2.0 * math.sin(21.0 * step * 0.02) * 10.0
+        a_r, a_i = s.amounts(step)
+        assert a_r == float(np.float32(inj)) and a_i == float(np.float32(inj * math.sin(1.0)))
+    assert s.amounts(0) == (0.0, 0.0)          # sin(0): the reference skips
+    r = L.ContinuousSource(FakeSim(0, N), position=5, phase=1.0)
+    assert r.position == 5 and r.amounts(3)[1] == 0.0     # REAL: no imaginary part
+    c = L.ContinuousSource(FakeSim(1, N), phase=1e-13)
+    assert c.amounts(3)[1] == 0.0                          # phase below 1e-12
+    with pytest.raises(ValueError):
+        L.ContinuousSource(FakeSim(1, N), axis=3)
+
+
 def test_host_mask_bit_identical_to_oracle():
     from lfm_hip.fields import boundary_mask_host
     for N, bf in ((8, 0.3), (16, 0.15), (48, 0.3), (64, 0.3)):
@@ -212,19 +300,33 @@ def test_python_constants_match_header_defines():
         assert getattr(_abi, k) == defs[k], k
 
 
-def test_params_struct_layout():
-    """LfmParams mirrors lfm_params_t: two int32, ten float32, two int32 and
-    the keep pointer, in declaration order (no padding before the pointer:
-    14 four-byte fields put it on an 8-byte boundary)."""
+def test_params_struct_layout(tmp_path):
+    """LfmParams mirrors lfm_params_t: the same fields in declaration order at
+    the offsets a C compiler gives them (gcc compiles the header and prints
+    offsetof of every field and the struct size)."""
+    import shutil
+    import subprocess
     from lfm_hip import _abi
     names = [n for n, _ in _abi.LfmParams._fields_]
-    assert names[:2] == ["N", "flags"] and names[-3:] == ["tile", "chunk", "keep"]
-    assert ctypes.sizeof(_abi.LfmParams) == 14 * 4 + 8
-    assert _abi.LfmParams.tile.offset == 12 * 4 and _abi.LfmParams.keep.offset == 14 * 4
+    assert names[:2] == ["N", "flags"] and names[12:15] == ["tile", "chunk", "keep"]
     text = open(HEADER).read()
     body = re.search(r"typedef struct lfm_params \{(.*?)\} lfm_params_t;", text, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     fields = re.findall(r"[A-Za-z_0-9]+(?=[,;])", body)
     assert fields == names, (fields, names)
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"lfm_hip.h\"\nint main(void) {\n"
+                   + "".join(f'    printf("%zu\\n", offsetof(lfm_params_t, {n}));\n' for n in names)
+                   + '    printf("%zu\\n", sizeof(lfm_params_t));\n    return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                           text=True).stdout.split()]
+    want = [getattr(_abi.LfmParams, n).offset for n in names] + [ctypes.sizeof(_abi.LfmParams)]
+    assert got == want, (got, want)
 
 
 def test_relax_fused_step_bookkeeping():
