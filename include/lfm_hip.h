@@ -70,7 +70,7 @@ extern "C" {
                                  and Pi' += src_amp[2s+1] * src_env[t0*N + t1]
                                  (each product and sum rounded once, a zero
                                  amount adds nothing), t0 < t1 the two other
-                                 coordinates, s = *src_step + src_off the step
+                                 coordinates, s = *step_counter + step_off the step
                                  index (no addition when s is outside
                                  [0, src_len)): the in-place
                                  `psi[plane] += amount * envelope` of a continuous
@@ -78,6 +78,20 @@ extern "C" {
                                  (lfm/experiment/source.py:196-228), same bits.
                                  lfm_step_real_f32 and _complex_f32 only (the
                                  reference's source writes one N^3 component) */
+#define LFM_F_DETECTOR 0x100  /* a detector plane records the new Psi as it is
+                                 written (after the LFM_F_SOURCE addition and the
+                                 LFM_F_KEEP_PSI factor): for the step index
+                                 s = *step_counter + step_off with
+                                 r = s - det_first >= 0, r % det_every == 0 and
+                                 f = r / det_every < det_len, the site of the
+                                 plane det_plane across det_axis at transverse
+                                 coordinates (t0, t1) writes det_frames[f*N*N +
+                                 t0*N + t1] = Pr'^2 + Pi'^2 (det_field 0, each
+                                 square and the sum rounded once; Pr'^2 at REAL)
+                                 or Pr' (det_field 1): a detector screen's
+                                 per-step record of the plane
+                                 (lfm/experiment/detector.py:97-115,268-281).
+                                 lfm_step_real_f32 and _complex_f32 only */
 
 /* Number of tile shapes of the tiled kernel (lfm_params_t.tile = 1..this). */
 #define LFM_TILE_VARIANTS 6
@@ -89,12 +103,16 @@ extern "C" {
  * chunk: 0 = automatic, else the number of i planes one workgroup marches.
  * keep:  NULL, or the N^3 per-site factors of LFM_F_KEEP_PSI (device memory,
  *        read only; it may not overlap an output).
+ * step_counter, step_off: the step index of LFM_F_SOURCE and LFM_F_DETECTOR
+ *        is *step_counter + step_off, one int64 the caller advances on the
+ *        device, so launches replayed from a graph see their own step (each
+ *        launch of a replayed sequence gets its own step_off).
  * src_*: the plane source of LFM_F_SOURCE (device memory, read only):
  *        src_env N*N envelope values, src_amp 2*src_len amounts (real part,
- *        imaginary part per step index), src_step one step counter the
- *        caller advances on the device (so launches replayed from a graph
- *        read their own step's amounts: each launch of a replayed sequence
- *        gets its own src_off); src_axis 0..2, src_plane 0..N-1. */
+ *        imaginary part per step index); src_axis 0..2, src_plane 0..N-1.
+ * det_*: the detector plane of LFM_F_DETECTOR: det_frames det_len frames of
+ *        N*N floats (device memory, written only on the recorded steps);
+ *        det_axis 0..2, det_plane 0..N-1, det_every >= 1, det_field 0 or 1. */
 typedef struct lfm_params {
     int32_t N;
     int32_t flags;
@@ -102,11 +120,16 @@ typedef struct lfm_params {
     float kappa_c, epsilon_cc, kappa_string, kappa_tube;
     int32_t tile, chunk;
     const float* keep;
+    const int64_t* step_counter;
+    int32_t step_off;
+    int32_t src_axis, src_plane;
+    int32_t det_axis, det_plane, det_every, det_field;
+    int32_t reserved;
     const float* src_env;
     const float* src_amp;
-    const int64_t* src_step;
     int64_t src_len;
-    int32_t src_axis, src_plane, src_off, src_reserved;
+    float* det_frames;
+    int64_t det_first, det_len;
 } lfm_params_t;
 
 int lfm_abi_version(void);

@@ -374,8 +374,8 @@ static int adj_check_common(const char* fn, const lfm_params_t* p, const float* 
         set_error("%s: LFM_F_MASK set but mask is NULL", fn);
         return LFM_E_ARG;
     }
-    if (p->flags & (LFM_F_KEEP_PSI | LFM_F_SOURCE)) {
-        set_error("%s: LFM_F_KEEP_PSI / LFM_F_SOURCE are not supported", fn);
+    if (p->flags & (LFM_F_KEEP_PSI | LFM_F_SOURCE | LFM_F_DETECTOR)) {
+        set_error("%s: LFM_F_KEEP_PSI / LFM_F_SOURCE / LFM_F_DETECTOR are not supported", fn);
         return LFM_E_ARG;
     }
     return LFM_OK;

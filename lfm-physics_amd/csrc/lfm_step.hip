@@ -60,10 +60,15 @@ struct KParams {
     // planes [ib, ie) along i are computed (0, N for a whole lattice; a slab
     // share computes its interior planes and reads the two halo planes)
     int ib, ie;
-    // plane source (LFM_F_SOURCE): axis, plane, this launch's step offset and
-    // the number of step indexes in the amount table
-    int src_axis, src_plane, src_off;
+    // step index of this launch = *counter + step_off (LFM_F_SOURCE, LFM_F_DETECTOR)
+    int step_off;
+    // plane source (LFM_F_SOURCE): axis, plane, step indexes in the amount table
+    int src_axis, src_plane;
     long long src_len;
+    // detector plane (LFM_F_DETECTOR): axis, plane, recording interval, field
+    // (0 |Psi|^2, 1 Pr), first recorded step index, frames in the buffer
+    int det_axis, det_plane, det_every, det_field;
+    long long det_first, det_len;
 };
 
 struct Ptrs {
@@ -84,10 +89,13 @@ struct Ptrs {
     float* sa_clip;      // NULL or max(sa, 0)
     float* psisq;        // NULL or |Psi'_a|^2 of the new fields
     const float* keep;   // NULL unless LFM_F_KEEP_PSI: per-site factor on Psi'
-    // NULL unless LFM_F_SOURCE: N*N envelope, 2 amounts per step index, step counter
+    // NULL unless LFM_F_SOURCE: N*N envelope, 2 amounts per step index
     const float* src_env;
     const float* src_amp;
-    const long long* src_step;
+    // NULL unless LFM_F_DETECTOR: det_len frames of N*N values
+    float* det_frames;
+    // NULL unless LFM_F_SOURCE or LFM_F_DETECTOR: the device step counter
+    const long long* counter;
     size_t n3;           // N^3
 };
 
@@ -286,13 +294,13 @@ __device__ __forceinline__ void site_compute(const KParams& kp, const PointData<
 
 // Plane source (LFM_F_SOURCE) at site (i, j, k): a continuous source's
 // `psi[plane] += amount * envelope` after the step, folded in.  The amounts of
-// step index *src_step + src_off are read only on the source plane.
+// step index *counter + step_off are read only on the source plane.
 template <int L>
 __device__ __forceinline__ void plane_source(const KParams& kp, const Ptrs& P, int i, int j,
                                              int k, float (&nw)[Lv<L>::NPF]) {
     const int ax = kp.src_axis;
     if ((ax == 0 ? i : ax == 1 ? j : k) != kp.src_plane) return;
-    const long long s = *P.src_step + kp.src_off;
+    const long long s = *P.counter + kp.step_off;
     if (s < 0 || s >= kp.src_len) return;
     // transverse coordinates in ascending axis order
     const int t0 = ax == 0 ? j : i, t1 = ax == 2 ? j : k;
@@ -306,10 +314,32 @@ __device__ __forceinline__ void plane_source(const KParams& kp, const Ptrs& P, i
     }
 }
 
+// Detector plane (LFM_F_DETECTOR) at site (i, j, k): a detector screen's
+// record of the plane after the step, |Psi'|^2 (Pr'^2 + Pi'^2, each square
+// and the sum rounded once) or Pr', into the frame of step index
+// *counter + step_off when that step is recorded.
+template <int L>
+__device__ __forceinline__ void plane_detector(const KParams& kp, const Ptrs& P, int i, int j,
+                                               int k, const float (&nw)[Lv<L>::NPF]) {
+    const int ax = kp.det_axis;
+    if ((ax == 0 ? i : ax == 1 ? j : k) != kp.det_plane) return;
+    const long long r = *P.counter + kp.step_off - kp.det_first;
+    if (r < 0 || r % kp.det_every != 0) return;
+    const long long f = r / kp.det_every;
+    if (f >= kp.det_len) return;
+    const int t0 = ax == 0 ? j : i, t1 = ax == 2 ? j : k;
+    float v = nw[0];
+    if (kp.det_field == 0) {
+        v = mul_rn(nw[0], nw[0]);
+        if constexpr (L == COMPLEX) v = add_rn(v, mul_rn(nw[1], nw[1]));
+    }
+    P.det_frames[(size_t)f * kp.N * kp.N + size_t(t0) * kp.N + t1] = v;
+}
+
 // Per-site update with its stores (site_compute, then the outputs of P).
-// EXTRA: the kernels for LFM_F_SOURCE and LFM_F_KEEP_PSI (a separate
-// instantiation, so the others carry none of their code); (i, j, k) is the
-// site, used by the plane source only.
+// EXTRA: the kernels for LFM_F_SOURCE, LFM_F_KEEP_PSI and LFM_F_DETECTOR (a
+// separate instantiation, so the others carry none of their code); (i, j, k)
+// is the site, used by the source and detector planes only.
 template <int L, class O, bool EXTRA = false>
 __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O idx,
                                             const PointData<L>& pd,
@@ -332,6 +362,9 @@ __device__ __forceinline__ void site_update(const KParams& kp, const Ptrs& P, O 
 #pragma unroll
             for (int f = 0; f < NPF; ++f) nw[f] = mul_rn(nw[f], k);
         }
+    }
+    if constexpr (EXTRA && L != COLOR) {
+        if (P.det_frames) plane_detector<L>(kp, P, si, sj, sk, nw);
     }
     const float a = 1.0f - ((kp.flags & LFM_F_MASK) ? pd.m : 0.0f);
 #pragma unroll
@@ -732,7 +765,7 @@ bool streaming(size_t sites) {
 
 template <int L>
 int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
-    const bool extra = (kp.flags & (LFM_F_KEEP_PSI | LFM_F_SOURCE)) != 0;
+    const bool extra = (kp.flags & (LFM_F_KEEP_PSI | LFM_F_SOURCE | LFM_F_DETECTOR)) != 0;
     if ((kp.flags & LFM_F_DIRECT) || kp.N > 1024) {
         const dim3 grid(unsigned((kp.N + 63) / 64), unsigned((kp.N + 3) / 4), unsigned(kp.N));
         if (extra) hipLaunchKernelGGL((step_direct<L, true>), grid, dim3(64, 4), 0, st, P, kp);
@@ -742,7 +775,7 @@ int launch(const Ptrs& P, const KParams& kp, hipStream_t st) {
     }
     const bool nt = streaming<L>(P.n3);
     if (extra) {
-        // the LFM_F_KEEP_PSI / LFM_F_SOURCE kernels come in one tile shape, {4 x 2}
+        // the LFM_F_KEEP_PSI / _SOURCE / _DETECTOR kernels come in one tile shape, {4 x 2}
         return nt ? launch_tiled<L, 4, 2, true, true>(P, kp, st)
                   : launch_tiled<L, 4, 2, false, true>(P, kp, st);
     }
@@ -1512,36 +1545,76 @@ KParams to_k(const lfm_params_t* p) {
     k.ib = 0;
     k.ie = p->N;
     const bool src = (p->flags & LFM_F_SOURCE) != 0;
+    const bool det = (p->flags & LFM_F_DETECTOR) != 0;
+    k.step_off = (src || det) ? p->step_off : 0;
     k.src_axis = src ? p->src_axis : 0;
     k.src_plane = src ? p->src_plane : -1;
-    k.src_off = src ? p->src_off : 0;
     k.src_len = src ? p->src_len : 0;
+    k.det_axis = det ? p->det_axis : 0;
+    k.det_plane = det ? p->det_plane : -1;
+    k.det_every = det ? p->det_every : 1;
+    k.det_field = det ? p->det_field : 0;
+    k.det_first = det ? p->det_first : 0;
+    k.det_len = det ? p->det_len : 0;
     return k;
 }
 
-// LFM_F_SOURCE arguments: pointers given, axis and plane in range.  Fills
-// the three source pointers (NULL without the flag) and their float counts
-// for the overlap checks.
-int source_check(const lfm_params_t* p, const float* (&sp)[3], size_t (&sn)[3], const char* who) {
-    sp[0] = sp[1] = sp[2] = nullptr;
-    sn[0] = sn[1] = sn[2] = 0;
-    if (!(p->flags & LFM_F_SOURCE)) return LFM_OK;
-    if (!p->src_env || !p->src_amp || !p->src_step) {
-        set_error("%s: LFM_F_SOURCE set but src_env, src_amp or src_step is NULL", who);
+// LFM_F_SOURCE / LFM_F_DETECTOR arguments: pointers given, axes, planes,
+// lengths and interval in range.  Fills the source inputs (envelope, amounts,
+// counter: NULL without the flags) and the detector frames, with their float
+// counts for the overlap checks.
+struct Extras {
+    const float* in[3] = {nullptr, nullptr, nullptr};
+    size_t nin[3] = {0, 0, 0};
+    float* frames = nullptr;
+    size_t nframes = 0;
+};
+int extras_check(const lfm_params_t* p, Extras& x, const char* who) {
+    const bool src = (p->flags & LFM_F_SOURCE) != 0;
+    const bool det = (p->flags & LFM_F_DETECTOR) != 0;
+    if (!src && !det) return LFM_OK;
+    if (!p->step_counter) {
+        set_error("%s: LFM_F_SOURCE / LFM_F_DETECTOR set but step_counter is NULL", who);
         return LFM_E_ARG;
     }
-    if (p->src_axis < 0 || p->src_axis > 2 || p->src_plane < 0 || p->src_plane >= p->N) {
-        set_error("%s: source axis %d / plane %d out of range (axis 0..2, plane 0..N-1)", who,
-                  p->src_axis, p->src_plane);
-        return LFM_E_ARG;
+    x.in[2] = reinterpret_cast<const float*>(p->step_counter);
+    x.nin[2] = 2;
+    if (src) {
+        if (!p->src_env || !p->src_amp) {
+            set_error("%s: LFM_F_SOURCE set but src_env or src_amp is NULL", who);
+            return LFM_E_ARG;
+        }
+        if (p->src_axis < 0 || p->src_axis > 2 || p->src_plane < 0 || p->src_plane >= p->N) {
+            set_error("%s: source axis %d / plane %d out of range (axis 0..2, plane 0..N-1)",
+                      who, p->src_axis, p->src_plane);
+            return LFM_E_ARG;
+        }
+        if (p->src_len < 0 || p->src_len > (int64_t(1) << 40)) {
+            set_error("%s: src_len %lld out of range", who, (long long)p->src_len);
+            return LFM_E_ARG;
+        }
+        x.in[0] = p->src_env; x.nin[0] = size_t(p->N) * p->N;
+        x.in[1] = p->src_amp; x.nin[1] = size_t(2 * p->src_len);
     }
-    if (p->src_len < 0 || p->src_len > (int64_t(1) << 40)) {
-        set_error("%s: src_len %lld out of range", who, (long long)p->src_len);
-        return LFM_E_ARG;
+    if (det) {
+        if (!p->det_frames) {
+            set_error("%s: LFM_F_DETECTOR set but det_frames is NULL", who);
+            return LFM_E_ARG;
+        }
+        if (p->det_axis < 0 || p->det_axis > 2 || p->det_plane < 0 || p->det_plane >= p->N) {
+            set_error("%s: detector axis %d / plane %d out of range (axis 0..2, plane 0..N-1)",
+                      who, p->det_axis, p->det_plane);
+            return LFM_E_ARG;
+        }
+        if (p->det_every < 1 || p->det_field < 0 || p->det_field > 1 || p->det_len < 0 ||
+            p->det_len > (int64_t(1) << 40) / (int64_t(p->N) * p->N)) {
+            set_error("%s: detector interval %d / field %d / frames %lld out of range", who,
+                      p->det_every, p->det_field, (long long)p->det_len);
+            return LFM_E_ARG;
+        }
+        x.frames = p->det_frames;
+        x.nframes = size_t(p->det_len) * p->N * p->N;
     }
-    sp[0] = p->src_env; sn[0] = size_t(p->N) * p->N;
-    sp[1] = p->src_amp; sn[1] = size_t(2 * p->src_len);
-    sp[2] = reinterpret_cast<const float*>(p->src_step); sn[2] = 2;
     return LFM_OK;
 }
 
@@ -1594,21 +1667,20 @@ extern "C" int lfm_step_real_f32(const float* E, const float* E_prev, const floa
     }
     const size_t n3 = size_t(p->N) * p->N * p->N;
     const float* keep = (p->flags & LFM_F_KEEP_PSI) ? p->keep : nullptr;
-    const float* sp[3];
-    size_t sn[3];
-    if (int rc = source_check(p, sp, sn, "lfm_step_real_f32")) return rc;
-    const float* ins[] = {E, E_prev, chi, chi_prev, mask, keep, sp[0], sp[1], sp[2]};
-    const size_t nin[] = {n3, n3, n3, n3, n3, n3, sn[0], sn[1], sn[2]};
-    float* outs[] = {E_out, E_prev_out, chi_out, chi_prev_out};
-    const size_t nout[] = {n3, n3, n3, n3};
-    int rc = validate(p, ins, nin, 9, outs, nout, 4, "lfm_step_real_f32");
+    Extras x;
+    if (int rc = extras_check(p, x, "lfm_step_real_f32")) return rc;
+    const float* ins[] = {E, E_prev, chi, chi_prev, mask, keep, x.in[0], x.in[1], x.in[2]};
+    const size_t nin[] = {n3, n3, n3, n3, n3, n3, x.nin[0], x.nin[1], x.nin[2]};
+    float* outs[] = {E_out, E_prev_out, chi_out, chi_prev_out, x.frames};
+    const size_t nout[] = {n3, n3, n3, n3, x.nframes};
+    int rc = validate(p, ins, nin, 9, outs, nout, 5, "lfm_step_real_f32");
     if (rc) return rc;
     Ptrs P{};
     P.pr = E; P.prp = E_prev; P.chi = chi; P.chip = chi_prev;
     P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
     P.keep = keep;
-    P.src_env = sp[0]; P.src_amp = sp[1];
-    P.src_step = reinterpret_cast<const long long*>(p->src_step);
+    P.src_env = x.in[0]; P.src_amp = x.in[1]; P.det_frames = x.frames;
+    P.counter = reinterpret_cast<const long long*>(x.in[2]);
     P.pr_out = E_out; P.prp_out = E_prev_out; P.chi_out = chi_out; P.chip_out = chi_prev_out;
     P.n3 = n3;
     return launch<REAL>(P, to_k(p), (hipStream_t)stream);
@@ -1639,21 +1711,21 @@ extern "C" int lfm_step_complex_f32(const float* pr, const float* pr_prev, const
     }
     const size_t n3 = size_t(p->N) * p->N * p->N;
     const float* keep = (p->flags & LFM_F_KEEP_PSI) ? p->keep : nullptr;
-    const float* sp[3];
-    size_t sn[3];
-    if (int rc = source_check(p, sp, sn, "lfm_step_complex_f32")) return rc;
-    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask, keep, sp[0], sp[1], sp[2]};
-    const size_t nin[] = {n3, n3, n3, n3, n3, n3, n3, n3, sn[0], sn[1], sn[2]};
-    float* outs[] = {pr_out, pr_prev_out, pi_out, pi_prev_out, chi_out, chi_prev_out};
-    const size_t nout[] = {n3, n3, n3, n3, n3, n3};
-    int rc = validate(p, ins, nin, 11, outs, nout, 6, "lfm_step_complex_f32");
+    Extras x;
+    if (int rc = extras_check(p, x, "lfm_step_complex_f32")) return rc;
+    const float* ins[] = {pr, pr_prev, pi, pi_prev, chi, chi_prev, mask, keep,
+                          x.in[0], x.in[1], x.in[2]};
+    const size_t nin[] = {n3, n3, n3, n3, n3, n3, n3, n3, x.nin[0], x.nin[1], x.nin[2]};
+    float* outs[] = {pr_out, pr_prev_out, pi_out, pi_prev_out, chi_out, chi_prev_out, x.frames};
+    const size_t nout[] = {n3, n3, n3, n3, n3, n3, x.nframes};
+    int rc = validate(p, ins, nin, 11, outs, nout, 7, "lfm_step_complex_f32");
     if (rc) return rc;
     Ptrs P{};
     P.pr = pr; P.pi = pi; P.prp = pr_prev; P.pip = pi_prev; P.chi = chi; P.chip = chi_prev;
     P.mask = (p->flags & LFM_F_MASK) ? mask : nullptr;
     P.keep = keep;
-    P.src_env = sp[0]; P.src_amp = sp[1];
-    P.src_step = reinterpret_cast<const long long*>(p->src_step);
+    P.src_env = x.in[0]; P.src_amp = x.in[1]; P.det_frames = x.frames;
+    P.counter = reinterpret_cast<const long long*>(x.in[2]);
     P.pr_out = pr_out; P.pi_out = pi_out; P.prp_out = pr_prev_out; P.pip_out = pi_prev_out;
     P.chi_out = chi_out; P.chip_out = chi_prev_out;
     P.n3 = n3;
@@ -1661,9 +1733,10 @@ extern "C" int lfm_step_complex_f32(const float* pr, const float* pr_prev, const
 }
 
 static int step2_check(const lfm_params_t* p, const char* fn) {
-    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI | LFM_F_KEEP_PSI | LFM_F_SOURCE)) {
-        set_error("%s: LFM_F_DIRECT / LFM_F_FREEZE_CHI / LFM_F_KEEP_PSI / LFM_F_SOURCE not "
-                  "supported", fn);
+    if (p->flags &
+        (LFM_F_DIRECT | LFM_F_FREEZE_CHI | LFM_F_KEEP_PSI | LFM_F_SOURCE | LFM_F_DETECTOR)) {
+        set_error("%s: LFM_F_DIRECT / LFM_F_FREEZE_CHI / LFM_F_KEEP_PSI / LFM_F_SOURCE / "
+                  "LFM_F_DETECTOR not supported", fn);
         return LFM_E_ARG;
     }
     if (p->N > 1024) {
@@ -1769,8 +1842,9 @@ extern "C" int lfm_step_color_f32(const float* pr, const float* pr_prev, const f
                   "(the S_a fields themselves are not passed)");
         return LFM_E_ARG;
     }
-    if (p->flags & LFM_F_SOURCE) {
-        set_error("lfm_step_color_f32: LFM_F_SOURCE is for the REAL and COMPLEX levels");
+    if (p->flags & (LFM_F_SOURCE | LFM_F_DETECTOR)) {
+        set_error("lfm_step_color_f32: LFM_F_SOURCE / LFM_F_DETECTOR are for the REAL and "
+                  "COMPLEX levels");
         return LFM_E_ARG;
     }
     if ((p->flags & LFM_F_KEEP_PSI) && !p->keep) {
@@ -1829,8 +1903,9 @@ extern "C" int lfm_step_slab_f32(int32_t level, int32_t planes, const float* pr,
         set_error("lfm_step_slab_f32: LFM_F_MASK set but mask is NULL");
         return LFM_E_ARG;
     }
-    if (p->flags & (LFM_F_DIRECT | LFM_F_KEEP_PSI | LFM_F_SOURCE)) {
-        set_error("lfm_step_slab_f32: LFM_F_DIRECT / LFM_F_KEEP_PSI / LFM_F_SOURCE not supported");
+    if (p->flags & (LFM_F_DIRECT | LFM_F_KEEP_PSI | LFM_F_SOURCE | LFM_F_DETECTOR)) {
+        set_error("lfm_step_slab_f32: LFM_F_DIRECT / LFM_F_KEEP_PSI / LFM_F_SOURCE / "
+                  "LFM_F_DETECTOR not supported");
         return LFM_E_ARG;
     }
     const bool scv = level == 2 && p->kappa_tube > 0.0f;
@@ -1905,10 +1980,10 @@ extern "C" int lfm_step_color_fused_f32(const float* pr, const float* pr_prev, c
         set_error("lfm_step_color_fused_f32: needs kappa_tube > 0 (use lfm_step_color_f32)");
         return LFM_E_ARG;
     }
-    if (p->flags &
-        (LFM_F_DIRECT | LFM_F_FREEZE_CHI | LFM_F_SA_SCV | LFM_F_KEEP_PSI | LFM_F_SOURCE)) {
+    if (p->flags & (LFM_F_DIRECT | LFM_F_FREEZE_CHI | LFM_F_SA_SCV | LFM_F_KEEP_PSI |
+                    LFM_F_SOURCE | LFM_F_DETECTOR)) {
         set_error("lfm_step_color_fused_f32: LFM_F_DIRECT / LFM_F_FREEZE_CHI / LFM_F_SA_SCV / "
-                  "LFM_F_KEEP_PSI / LFM_F_SOURCE not supported");
+                  "LFM_F_KEEP_PSI / LFM_F_SOURCE / LFM_F_DETECTOR not supported");
         return LFM_E_ARG;
     }
     ScvView v{};
@@ -2042,8 +2117,9 @@ extern "C" int lfm_batch_step_f32(void* plan, const float* pr, const float* pr_p
         set_error("lfm_batch_step_f32: LFM_F_SA_SCV is for COLOR batches");
         return LFM_E_ARG;
     }
-    if (flags & (LFM_F_KEEP_PSI | LFM_F_SOURCE)) {
-        set_error("lfm_batch_step_f32: LFM_F_KEEP_PSI / LFM_F_SOURCE are not supported");
+    if (flags & (LFM_F_KEEP_PSI | LFM_F_SOURCE | LFM_F_DETECTOR)) {
+        set_error("lfm_batch_step_f32: LFM_F_KEEP_PSI / LFM_F_SOURCE / LFM_F_DETECTOR are not "
+                  "supported");
         return LFM_E_ARG;
     }
     const size_t ns = size_t(P->total) * (P->level == 2 ? 3 : 1);

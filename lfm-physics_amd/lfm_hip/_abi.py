@@ -26,6 +26,7 @@ LFM_F_CHI_DELTA = 0x10
 LFM_F_SA_SCV = 0x20
 LFM_F_KEEP_PSI = 0x40
 LFM_F_SOURCE = 0x80
+LFM_F_DETECTOR = 0x100
 LFM_ABI_VERSION = 6
 LFM_TILE_VARIANTS = 6
 LFM_RELAX_SLOT_WORDS = 2048
@@ -86,9 +87,12 @@ class LfmParams(ctypes.Structure):
                 ("e0_sq", _f32), ("epsilon_w", _f32), ("kappa_c", _f32),
                 ("epsilon_cc", _f32), ("kappa_string", _f32), ("kappa_tube", _f32),
                 ("tile", _i32), ("chunk", _i32), ("keep", _vp),
-                ("src_env", _vp), ("src_amp", _vp), ("src_step", _vp), ("src_len", _i64),
-                ("src_axis", _i32), ("src_plane", _i32), ("src_off", _i32),
-                ("src_reserved", _i32)]
+                ("step_counter", _vp), ("step_off", _i32),
+                ("src_axis", _i32), ("src_plane", _i32),
+                ("det_axis", _i32), ("det_plane", _i32), ("det_every", _i32), ("det_field", _i32),
+                ("reserved", _i32),
+                ("src_env", _vp), ("src_amp", _vp), ("src_len", _i64),
+                ("det_frames", _vp), ("det_first", _i64), ("det_len", _i64)]
 
 
 class LfmError(RuntimeError):

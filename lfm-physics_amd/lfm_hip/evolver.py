@@ -159,6 +159,7 @@ class Evolver:
         self._params = {}
         self._keep = None      # set_psi_keep: per-site factor folded into every step
         self._src = None       # set_plane_source: plane source folded into every step
+        self._det = None       # set_plane_detector: detector plane recorded by the steps
         self._pending = 0      # steps launched since the device step counter was advanced
         self.tile = 0      # tiled-kernel shape (0 = default, see autotune())
         self.chunk = 0     # i planes per workgroup (0 = automatic)
@@ -179,10 +180,13 @@ class Evolver:
         keep = self._keep.data_ptr() if self._keep is not None else 0
         if keep:
             flags |= _abi.LFM_F_KEEP_PSI
-        src = self._src
+        src, det = self._src, self._det
         if src is not None:
             flags |= _abi.LFM_F_SOURCE
-        key = (flags, self.tile, self.chunk, keep, src["table"].data_ptr() if src else 0)
+        if det is not None:
+            flags |= _abi.LFM_F_DETECTOR
+        key = (flags, self.tile, self.chunk, keep, src["table"].data_ptr() if src else 0,
+               (det["frames"].data_ptr(), det["first"]) if det else 0)
         p = self._params.get(key)
         if p is None:
             c = self.config
@@ -190,16 +194,22 @@ class Evolver:
                        c.kappa_c, c.epsilon_cc, c.kappa_string,
                        c.kappa_tube if self.sa_enabled else 0.0, flags, self.tile, self.chunk)
             p.keep = keep or None
+            if src is not None or det is not None:
+                p.step_counter = self._step_dev.data_ptr()
             if src is not None:
                 p.src_env = src["env"].data_ptr()
                 p.src_amp = src["table"].data_ptr()
-                p.src_step = self._step_dev.data_ptr()
                 p.src_len = src["table"].numel() // 2
                 p.src_axis, p.src_plane = src["axis"], src["plane"]
+            if det is not None:
+                p.det_frames = det["frames"].data_ptr()
+                p.det_len = det["frames"].shape[0]
+                p.det_axis, p.det_plane = det["axis"], det["plane"]
+                p.det_every, p.det_field, p.det_first = det["every"], det["field"], det["first"]
             self._params[key] = p
-        if src is not None:
-            # this launch's step index is the device counter + src_off
-            p.src_off = self._pending + 1
+        if src is not None or det is not None:
+            # this launch's step index is the device counter + step_off
+            p.step_off = self._pending + 1
         return p
 
     def set_plane_source(self, axis: int, plane: int, envelope, amounts) -> None:
@@ -256,6 +266,68 @@ class Evolver:
         src["table"] = torch.from_numpy(amt.reshape(-1)).to(self.device)
         self._graphs.clear()     # the captured launches point at the old table
 
+    def set_plane_detector(self, axis: int, plane: int, every: int = 1,
+                           field: str = "energy_density") -> None:
+        """Record the plane `plane` across `axis` inside the steps (REAL and
+        COMPLEX), or stop with axis=None.  From now on, after every step that
+        brings the step count to start + every, start + 2 every, ..., the
+        step kernel writes the plane of the new Psi into the next N x N frame
+        of a device buffer: |Psi|^2 = Pr^2 + Pi^2 ("energy_density", as
+        lfm/core/evolver.py:346-353 forms it) or Pr ("psi_real"), after a
+        plane source and a set_psi_keep factor.  It is a detector screen's
+        record() called as a per-step callback (lfm/experiment/detector.py:
+        97-115), without the callback's per-step launches and copies;
+        detector_frames() returns the frames recorded so far.  Steps then run
+        one at a time."""
+        if axis is None:
+            self._det = None
+            self._graphs.clear()
+            return
+        if self.level == 2:
+            raise ValueError("a detector plane is for the REAL and COMPLEX levels")
+        kinds = {"energy_density": 0, "psi_real": 1}
+        if field not in kinds:
+            raise ValueError(f"field must be one of {sorted(kinds)}, got {field!r}")
+        axis, plane, every = int(axis), int(plane), int(every)
+        if axis not in (0, 1, 2) or not 0 <= plane < self.N or every < 1:
+            raise ValueError(f"detector axis {axis} / plane {plane} / every {every} out of range "
+                             f"for N={self.N}")
+        if not hasattr(self, "_step_dev"):
+            self._step_dev = torch.zeros((), dtype=torch.int64, device=self.device)
+        N = self.N
+        self._det = {"axis": axis, "plane": plane, "every": every, "field": kinds[field],
+                     "first": self.step + every,
+                     "frames": torch.zeros((0, N, N), device=self.device)}
+        self._graphs.clear()
+
+    def _detector_frames(self, last: int) -> None:
+        """Make the detector's frame buffer hold every frame up to step index
+        `last` (frames already recorded are kept)."""
+        det = self._det
+        need = (last - det["first"]) // det["every"] + 1
+        have = det["frames"].shape[0]
+        if need <= have:
+            return
+        cap = max(2 * have, need, 64)
+        frames = torch.zeros((cap,) + tuple(det["frames"].shape[1:]), device=self.device)
+        frames[:have].copy_(det["frames"])
+        det["frames"] = frames
+        self._graphs.clear()     # the captured launches point at the old buffer
+
+    def detector_frames(self) -> torch.Tensor:
+        """The frames the detector plane has recorded, (n, N, N) on the
+        device, oldest first."""
+        det = self._det
+        if det is None:
+            raise ValueError("no detector plane is set (set_plane_detector)")
+        n = max(0, (self.step - det["first"]) // det["every"] + 1)
+        return det["frames"][:min(n, det["frames"].shape[0])]
+
+    def _counted(self) -> bool:
+        """Whether the steps read the device step counter (a plane source or
+        a detector plane)."""
+        return self._src is not None or self._det is not None
+
     def _advance_counter(self) -> None:
         """Add the steps launched since the last call to the device step
         counter (read by a device hook and by the plane source)."""
@@ -295,7 +367,8 @@ class Evolver:
         results = {}
         old = (self.tile, self.chunk, self.kernel_events)
         N = self.N
-        pairs = self.two_step and N <= 1024 and self._keep is None and self._src is None
+        pairs = (self.two_step and N <= 1024 and self._keep is None and self._src is None
+                 and self._det is None)
         if self._fused:
             # the fused COLOR kernel has one tile shape (a full k-row); tune its chunk
             combos = [(0, ch) for ch in dict.fromkeys((0, N, N // 2, N // 4, N // 8)) if ch == 0 or ch >= 4]
@@ -509,7 +582,7 @@ class Evolver:
         """One step, then (with a device hook or a plane source) the device
         step counter and the hook, as evolve runs them."""
         self._step(freeze_chi=freeze_chi)
-        if hook is not None or self._src is not None:
+        if hook is not None or self._counted():
             self._advance_counter()
         if hook is not None:
             hook(self, self._step_dev)
@@ -530,7 +603,7 @@ class Evolver:
         if self.level == 2 and self.sa_enabled and self._scv.method != 2:
             return None              # rocFFT launches stay out of graphs
         pair = (self.two_step and self.N <= 1024 and hook is None and self._keep is None
-                and self._src is None and not freeze)
+                and not self._counted() and not freeze)
         key0 = self._key()
         k, m = key0, 0
         while True:
@@ -559,10 +632,10 @@ class Evolver:
                     elif hook is not None:
                         self._hooked_step(hook)
                     else:
-                        # a plane source's launches read counter + 1 .. m;
+                        # source / detector launches read counter + 1 .. m;
                         # the counter advances once per replay
                         self._step(freeze_chi=freeze)
-                if self._src is not None:
+                if self._counted():
                     self._advance_counter()
         self._pending = 0
         torch.cuda.current_stream(self.device).wait_stream(s)
@@ -601,12 +674,14 @@ class Evolver:
         left = int(steps)
         hook = device_hook
         pair_ok = (self.two_step and not freeze_chi and self.N <= 1024 and hook is None
-                   and self._keep is None and self._src is None)
+                   and self._keep is None and not self._counted())
         if hook is not None and freeze_chi:
             raise ValueError("device_hook with freeze_chi is not supported")
         if self._src is not None:
             self._source_table(self.step + left)
-        if hook is not None or self._src is not None:
+        if self._det is not None:
+            self._detector_frames(self.step + left)
+        if hook is not None or self._counted():
             if not hasattr(self, "_step_dev"):
                 self._step_dev = torch.zeros((), dtype=torch.int64, device=self.device)
             self._step_dev.fill_(self.step)

@@ -242,3 +242,175 @@ class ContinuousSource:
     def unfold(self) -> None:
         """Stop the in-kernel injection."""
         self._sim.evolver.set_plane_source(0, 0, None, None)
+
+
+class DetectorScreen:
+    """A detector screen recording one lattice plane
+    (lfm/experiment/detector.py:53-287): each ``record()`` stores the plane
+    `position` across `axis` of `field` ("energy_density" = |Psi|^2,
+    "psi_real", "psi_imag" or "chi"), and ``pattern`` is the sum of the
+    recorded frames (the interference pattern), with the reference's
+    ``snapshots``, ``mean_pattern``, ``click_pattern``, ``line_profile`` and
+    ``reset``.
+
+    Two ways to record, with the same frames bit for bit:
+
+    * ``record()`` / ``step_callback(sim, step)``: the reference's form,
+      called by the caller (per step, or per snapshot batch); the plane is
+      cut on the device and only its N x N values are copied;
+    * ``fold(every=1)``: the step kernel writes the plane into a device frame
+      buffer after every `every`-th step from now on
+      (``Evolver.set_plane_detector``; "energy_density" and "psi_real", REAL
+      and COMPLEX), so the steps keep running as replayed graphs; frames are
+      copied to the host only when read.  ``unfold()`` stops it, keeping the
+      frames recorded so far.
+
+    Like the reference's callback form, a folded detector records after a
+    plane source and a barrier's keep factors of the same step."""
+
+    def __init__(self, sim, axis: int = 2, position: int | None = None,
+                 field: str = "energy_density") -> None:
+        N = sim.config.grid_size
+        self._sim = sim
+        self._axis = int(axis)
+        self._pos = position if position is not None else int(N * 0.80)
+        self._field_name = field
+        self._N = N
+        self._frames: list[np.ndarray] = []
+        self._folded = False
+
+    # -- recording -----------------------------------------------------------
+    def _plane(self, t):
+        idx = [slice(None)] * 3
+        idx[self._axis] = self._pos
+        return t[tuple(idx)]
+
+    def record(self) -> None:
+        """Store the detector plane of the current field."""
+        if self._folded:
+            raise RuntimeError("this detector is folded into the steps (unfold() first)")
+        sim, N = self._sim, self._N
+        name = self._field_name
+        if name == "energy_density" and sim.evolver.level < 2:
+            # |Psi|^2 on the plane, squares and sum rounded as the host array
+            # arithmetic rounds them
+            pr = self._plane(sim._native_psi_real())
+            frame = pr * pr
+            pi = sim._native_psi_imag()
+            if pi is not None:
+                pi = self._plane(pi)
+                frame = frame + pi * pi
+            self._frames.append(frame.cpu().numpy().astype(np.float32))
+            return
+        if name in ("psi_real", "psi_imag") and sim.evolver.level < 2:
+            t = sim._native_psi_real() if name == "psi_real" else sim._native_psi_imag()
+            self._frames.append(np.zeros((N, N), np.float32) if t is None
+                                else self._plane(t).cpu().numpy().astype(np.float32))
+            return
+        field = self._get_field()
+        self._frames.append(np.asarray(self._plane(field), dtype=np.float32).copy())
+
+    def step_callback(self, sim, step: int) -> None:
+        """Simulation step callback; the same as record()."""
+        self.record()
+
+    def reset(self) -> None:
+        """Clear all recorded frames (a folded detector records on from the
+        next step)."""
+        self._frames.clear()
+        if self._folded:
+            ev = self._sim.evolver
+            det = ev._det
+            ev.set_plane_detector(det["axis"], det["plane"], det["every"], self._field_name)
+
+    def fold(self, every: int = 1) -> None:
+        """Record inside the steps from now on, after every `every`-th step."""
+        if self._field_name not in ("energy_density", "psi_real"):
+            raise ValueError("only energy_density and psi_real detectors fold into the steps")
+        self._sim.evolver.set_plane_detector(self._axis, self._pos, every, self._field_name)
+        self._folded = True
+
+    def unfold(self) -> None:
+        """Stop recording inside the steps, keeping the frames recorded."""
+        if self._folded:
+            self._frames = self._all_frames()
+            self._sim.evolver.set_plane_detector(None, 0)
+            self._folded = False
+
+    def _all_frames(self) -> list[np.ndarray]:
+        if not self._folded:
+            return self._frames
+        dev = self._sim.evolver.detector_frames().cpu().numpy()
+        return self._frames + [f for f in dev]
+
+    # -- derived quantities ---------------------------------------------------
+    @property
+    def pattern(self) -> np.ndarray:
+        """Sum of the recorded frames, (N, N) float32 (zeros when none)."""
+        frames = self._all_frames()
+        if not frames:
+            return np.zeros((self._N, self._N), dtype=np.float32)
+        return np.sum(frames, axis=0).astype(np.float32)
+
+    @property
+    def snapshots(self) -> np.ndarray:
+        """All recorded frames, (K, N, N)."""
+        frames = self._all_frames()
+        if not frames:
+            return np.zeros((0, self._N, self._N), dtype=np.float32)
+        return np.stack(frames, axis=0)
+
+    def click_pattern(self, n_particles: int = 1000, seed: int | None = None) -> np.ndarray:
+        """n_particles detections drawn from the pattern as a probability
+        distribution, as an (N, N) histogram."""
+        pat = self.pattern
+        total = pat.sum()
+        if total == 0:
+            return np.zeros(pat.shape, dtype=np.int32)
+        prob = pat.ravel() / total
+        rng = np.random.default_rng(seed)
+        indices = rng.choice(len(prob), size=n_particles, p=prob)
+        hist, _ = np.histogram(indices, bins=np.arange(len(prob) + 1))
+        return hist.reshape(pat.shape).astype(np.int32)
+
+    def mean_pattern(self) -> np.ndarray:
+        """Time-averaged intensity, (N, N) float32."""
+        frames = self._all_frames()
+        if not frames:
+            return np.zeros((self._N, self._N), dtype=np.float32)
+        return (np.sum(frames, axis=0) / len(frames)).astype(np.float32)
+
+    def line_profile(self, perp_axis: int = 0) -> np.ndarray:
+        """The pattern summed along `perp_axis`."""
+        return self.pattern.sum(axis=perp_axis).astype(np.float32)
+
+    @property
+    def position(self) -> int:
+        return self._pos
+
+    @property
+    def axis(self) -> int:
+        return self._axis
+
+    @property
+    def n_frames(self) -> int:
+        return len(self._all_frames())
+
+    def _get_field(self):
+        sim = self._sim
+        name = self._field_name
+        if name == "energy_density":
+            return sim.energy_density
+        if name == "psi_real":
+            return sim.psi_real
+        if name == "psi_imag":
+            f = sim.psi_imag
+            return f if f is not None else np.zeros((self._N,) * 3, dtype=np.float32)
+        if name == "chi":
+            return sim.chi
+        raise ValueError(f"Unknown field '{name}'. Choose from 'energy_density', 'psi_real', "
+                         "'psi_imag', 'chi'.")
+
+    def __repr__(self) -> str:
+        return (f"DetectorScreen(axis={self._axis}, position={self._pos}, "
+                f"field='{self._field_name}', n_frames={self.n_frames})")

@@ -51,7 +51,7 @@ def _ics(level, N, seed):
 def test_source_flag_equals_plain_step_plus_addition(gpu, level, axis):
     """One step with LFM_F_SOURCE equals the plain step followed by
     psi[plane] += amount * envelope in torch float32 (product and sum each
-    rounded once); amounts come from the table at counter + src_off, and
+    rounded once); amounts come from the table at counter + step_off, and
     indexes outside the table add nothing.  Tiled and untiled kernels."""
     import torch
     from lfm_hip import _abi
@@ -73,8 +73,8 @@ def test_source_flag_equals_plain_step_plus_addition(gpu, level, axis):
         p = _abi.LfmParams()
         p.N, p.dt2, p.kappa, p.lambda_self, p.chi0, p.epsilon_w = N, 4e-4, 1 / 63, 4 / 31, 19.0, 0.1
         p.flags = flags | _abi.LFM_F_CHI_DELTA | (_abi.LFM_F_DIRECT if direct else 0)
-        p.src_env, p.src_amp, p.src_step = env.data_ptr(), table.data_ptr(), counter.data_ptr()
-        p.src_len, p.src_axis, p.src_plane, p.src_off = length, axis, plane, off
+        p.src_env, p.src_amp, p.step_counter = env.data_ptr(), table.data_ptr(), counter.data_ptr()
+        p.src_len, p.src_axis, p.src_plane, p.step_off = length, axis, plane, off
         o = [torch.empty_like(pr), torch.empty_like(pr) if level else None, torch.empty_like(chi)]
         if level == 0:
             rc = lib.lfm_step_real_f32(ptr(pr), ptr(prp), ptr(chi), ptr(chip), None, ptr(o[0]),

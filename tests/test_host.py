@@ -139,21 +139,23 @@ def test_slab_step_validation():
     assert rc == -1 and b"S_a" in lib.lfm_last_error()
 
 
-def test_plane_source_validation():
-    """LFM_F_SOURCE is checked before any device work: its pointers, axis,
-    plane and table length, an output overlapping the envelope, and the entry
-    points that do not take it (COLOR, two-step, slab share)."""
+def test_plane_source_and_detector_validation():
+    """LFM_F_SOURCE and LFM_F_DETECTOR are checked before any device work:
+    their pointers, axes, planes, lengths and interval, an output over the
+    envelope, the amounts or the counter, frames over an input, and the entry
+    points that do not take them (COLOR, two-step, slab share)."""
     from lfm_hip import _abi
     lib = _abi.load()
     N = 16
     n3 = N ** 3
     ptr = [(1 << 24) + q * 4 * 3 * n3 for q in range(16)]
-    env, amp, cnt = ptr[10], ptr[11], ptr[12]
+    env, amp, cnt, frames = ptr[10], ptr[11], ptr[12], ptr[13]
 
-    def params(**kw):
-        p = _abi.LfmParams(N, _abi.LFM_F_SOURCE, 4e-4, 1 / 63, 0, 19, 0, 0.1, 0, 0, 0, 0)
-        p.src_env, p.src_amp, p.src_step, p.src_len = env, amp, cnt, 100
-        p.src_axis, p.src_plane, p.src_off = 2, 4, 1
+    def params(flags=_abi.LFM_F_SOURCE, **kw):
+        p = _abi.LfmParams(N, flags, 4e-4, 1 / 63, 0, 19, 0, 0.1, 0, 0, 0, 0)
+        p.src_env, p.src_amp, p.step_counter, p.src_len = env, amp, cnt, 100
+        p.src_axis, p.src_plane, p.step_off = 2, 4, 1
+        p.det_frames, p.det_len, p.det_axis, p.det_plane, p.det_every = frames, 10, 1, 5, 1
         for k, v in kw.items():
             setattr(p, k, v)
         return p
@@ -166,26 +168,37 @@ def test_plane_source_validation():
         return lib.lfm_step_complex_f32(ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5], None,
                                         ptr[6], None, ptr[7], None, ptr[8], None, ctypes.byref(p),
                                         None)
-    for kw, what in (({"src_env": None}, b"src_env"), ({"src_step": None}, b"src_env"),
-                     ({"src_axis": 3}, b"axis"), ({"src_plane": N}, b"plane"),
-                     ({"src_plane": -1}, b"plane"), ({"src_len": -1}, b"src_len")):
+    S, D = _abi.LFM_F_SOURCE, _abi.LFM_F_DETECTOR
+    for flags, kw, what in ((S, {"src_env": None}, b"src_env"),
+                            (S, {"step_counter": None}, b"step_counter"),
+                            (D, {"step_counter": None}, b"step_counter"),
+                            (S, {"src_axis": 3}, b"axis"), (S, {"src_plane": N}, b"plane"),
+                            (S, {"src_plane": -1}, b"plane"), (S, {"src_len": -1}, b"src_len"),
+                            (D, {"det_frames": None}, b"det_frames"),
+                            (D, {"det_axis": -1}, b"axis"), (D, {"det_plane": N}, b"plane"),
+                            (D, {"det_every": 0}, b"interval"), (D, {"det_field": 2}, b"field"),
+                            (D, {"det_len": -1}, b"frames")):
         for fn in (real, cplx):
-            rc = fn(params(**kw))
-            assert rc == -1 and what in lib.lfm_last_error(), (kw, fn)
-    # an output over the envelope, the amounts or the counter
+            rc = fn(params(flags, **kw))
+            assert rc == -1 and what in lib.lfm_last_error(), (flags, kw, fn)
+    # an output over the envelope, the amounts or the counter; frames over an input
     for out in (env, amp + 64, cnt):
         rc = real(params(), out=out)
         assert rc == -1 and b"overlaps" in lib.lfm_last_error(), out
-    p = params()
-    rc = lib.lfm_step_color_f32(ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5], None, ptr[6], None,
-                                ptr[7], None, ptr[8], None, None, None, None, ctypes.byref(p), None)
-    assert rc == -1 and b"LFM_F_SOURCE" in lib.lfm_last_error()
-    rc = lib.lfm_step2_real_f32(ptr[0], ptr[1], ptr[2], ptr[3], None, ptr[6], ptr[7], ptr[8],
-                                ptr[9], ctypes.byref(p), None)
-    assert rc == -1 and b"LFM_F_SOURCE" in lib.lfm_last_error()
-    rc = lib.lfm_step_slab_f32(1, 8, ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5], None, None,
-                               ptr[6], ptr[7], ptr[8], None, ctypes.byref(p), None)
-    assert rc == -1 and b"LFM_F_SOURCE" in lib.lfm_last_error()
+    rc = real(params(D, det_frames=ptr[1] + 64))
+    assert rc == -1 and b"overlaps" in lib.lfm_last_error()
+    for flags in (S, D):
+        p = params(flags)
+        rc = lib.lfm_step_color_f32(ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5], None, ptr[6],
+                                    None, ptr[7], None, ptr[8], None, None, None, None,
+                                    ctypes.byref(p), None)
+        assert rc == -1 and b"COMPLEX levels" in lib.lfm_last_error()
+        rc = lib.lfm_step2_real_f32(ptr[0], ptr[1], ptr[2], ptr[3], None, ptr[6], ptr[7], ptr[8],
+                                    ptr[9], ctypes.byref(p), None)
+        assert rc == -1 and b"not supported" in lib.lfm_last_error()
+        rc = lib.lfm_step_slab_f32(1, 8, ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5], None,
+                                   None, ptr[6], ptr[7], ptr[8], None, ctypes.byref(p), None)
+        assert rc == -1 and b"not supported" in lib.lfm_last_error()
 
 
 def test_continuous_source_amounts():
