@@ -15,13 +15,13 @@ from .backend import HipBackend
 from .evolver import Evolver, get_backend
 from .simulation import Simulation
 from .batch import LatticeBatch, sweep, sweep_2d
-from .experiment import ContinuousSource, DetectorScreen, gpu_snapshot_loop
+from .experiment import Barrier, ContinuousSource, DetectorScreen, Slit, gpu_snapshot_loop
 from . import fields
 
 __all__ = ["BoundaryType", "ChiMode", "FieldLevel", "PhysicsScale", "SimulationConfig",
            "full_physics", "gravity_em", "gravity_only", "spinor_field", "HipBackend",
            "Evolver", "get_backend", "Simulation", "LatticeBatch", "sweep", "sweep_2d",
-           "gpu_snapshot_loop", "ContinuousSource", "DetectorScreen", "fields",
+           "gpu_snapshot_loop", "Barrier", "Slit", "ContinuousSource", "DetectorScreen", "fields",
            "ALPHA_S", "BETA_0", "CHI0", "D_ST", "DT_DEFAULT", "E_AMPLITUDE_BY_GRID", "EPSILON_CC",
            "EPSILON_W", "KAPPA", "KAPPA_C", "KAPPA_STRING", "KAPPA_TUBE", "LAMBDA_H", "N_COLORS",
            "RANK_SU3", "SA_D", "SA_GAMMA"]

@@ -27,7 +27,9 @@ reference's ``get_backend("gpu")`` raises when its GPU stack is missing
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
+import gc
 
 import numpy as np
 import torch
@@ -108,6 +110,25 @@ def params(N, dt2, kappa, lambda_self, chi0, e0_sq, epsilon_w=0.0, kappa_c=0.0,
                           float(chi0), float(e0_sq), float(epsilon_w), float(kappa_c),
                           float(epsilon_cc), float(kappa_string), float(kappa_tube),
                           int(tile), int(chunk))
+
+
+@contextlib.contextmanager
+def graph_capture(graph, stream):
+    """Capture the enclosed launches on `stream` into `graph` with Python's
+    cyclic garbage collector held off.  A collection during the capture could
+    free an unreachable object that owns another CUDAGraph (an Evolver in a
+    reference cycle, say), and destroying a graph while a stream captures is
+    not permitted: the destructor's error would abort the process.  Dead
+    cycles are left to the collector's next run after the capture."""
+    enabled = gc.isenabled()
+    gc.disable()
+    try:
+        with torch.cuda.stream(stream):
+            with torch.cuda.graph(graph, stream=stream):
+                yield
+    finally:
+        if enabled:
+            gc.enable()
 
 
 def stream_handle() -> int:

@@ -35,7 +35,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from .backend import ScvPlan, params, stream_handle
+from .backend import ScvPlan, graph_capture, params, stream_handle
 from .config import BoundaryType, SimulationConfig
 from .evolver import get_backend
 from .fields import boundary_mask_host, gaussian_soliton
@@ -475,11 +475,10 @@ class LatticeBatch:
             self._advance2(seq[-1])
         if (self._cur, self._prev, self._chi_cur, self._chi_prev) != saved:
             raise RuntimeError("two-step slot rotation did not return to its start")
-        with torch.cuda.stream(s):
-            with torch.cuda.graph(g, stream=s):
-                for grp in range(len(self.groups)):
-                    for slots2 in seq:
-                        self._launch2(grp, slots2)
+        with graph_capture(g, s):
+            for grp in range(len(self.groups)):
+                for slots2 in seq:
+                    self._launch2(grp, slots2)
         torch.cuda.current_stream(self.device).wait_stream(s)
         self._graphs[key] = g
         return g
@@ -499,11 +498,10 @@ class LatticeBatch:
         for _ in range(3 * self.cycles):
             seq.append(self._slots())
             self._advance(seq[-1])
-        with torch.cuda.stream(s):
-            with torch.cuda.graph(g, stream=s):
-                for grp in range(len(self.groups)):
-                    for slots in seq:
-                        self._launch(grp, slots)
+        with graph_capture(g, s):
+            for grp in range(len(self.groups)):
+                for slots in seq:
+                    self._launch(grp, slots)
         torch.cuda.current_stream(self.device).wait_stream(s)
         # a whole number of rotations returns every slot to where it was
         self._cur, self._prev, self._chi_cur, self._chi_prev = saved

@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from .backend import ScvPlan, stream_handle
+from .backend import ScvPlan, graph_capture, stream_handle
 from .evolver import get_backend
 
 CHI0 = 19.0
@@ -147,11 +147,10 @@ class _Relaxer:
             s.wait_stream(torch.cuda.current_stream(self.dev))
             g = torch.cuda.CUDAGraph()
             E0, E1 = self.E, self.E_alt
-            with torch.cuda.stream(s):
-                with torch.cuda.graph(g, stream=s):
-                    for u in range(self.GRAPH_STEPS):
-                        k = (k0 + u) % 3
-                        self._step(k, (k - 1) % 3, amplitude, dt)
+            with graph_capture(g, s):
+                for u in range(self.GRAPH_STEPS):
+                    k = (k0 + u) % 3
+                    self._step(k, (k - 1) % 3, amplitude, dt)
             torch.cuda.current_stream(self.dev).wait_stream(s)
             # capture recorded the launches without running them; six swaps
             # leave the buffers in their starting order

@@ -31,7 +31,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from .backend import HipBackend, ScvPlan, params, stream_handle
+from .backend import HipBackend, ScvPlan, graph_capture, params, stream_handle
 from .config import BoundaryType, FieldLevel, SimulationConfig
 
 
@@ -160,6 +160,9 @@ class Evolver:
         self._keep = None      # set_psi_keep: per-site factor folded into every step
         self._src = None       # set_plane_source: plane source folded into every step
         self._det = None       # set_plane_detector: detector plane recorded by the steps
+        # True: every evolve holds chi frozen (a barrier folded into the steps,
+        # whose per-step chi rewrite makes chi's own update irrelevant)
+        self.hold_chi = False
         self._pending = 0      # steps launched since the device step counter was advanced
         self.tile = 0      # tiled-kernel shape (0 = default, see autotune())
         self.chunk = 0     # i planes per workgroup (0 = automatic)
@@ -624,19 +627,18 @@ class Evolver:
         s.wait_stream(torch.cuda.current_stream(self.device))
         g = torch.cuda.CUDAGraph()
         self._pending = 0
-        with torch.cuda.stream(s):
-            with torch.cuda.graph(g, stream=s):
-                for _ in range(m):
-                    if pair:
-                        self._step2()
-                    elif hook is not None:
-                        self._hooked_step(hook)
-                    else:
-                        # source / detector launches read counter + 1 .. m;
-                        # the counter advances once per replay
-                        self._step(freeze_chi=freeze)
-                if self._counted():
-                    self._advance_counter()
+        with graph_capture(g, s):
+            for _ in range(m):
+                if pair:
+                    self._step2()
+                elif hook is not None:
+                    self._hooked_step(hook)
+                else:
+                    # source / detector launches read counter + 1 .. m;
+                    # the counter advances once per replay
+                    self._step(freeze_chi=freeze)
+            if self._counted():
+                self._advance_counter()
         self._pending = 0
         torch.cuda.current_stream(self.device).wait_stream(s)
         # Capture recorded the launches without running them.  The bookkeeping
@@ -673,6 +675,7 @@ class Evolver:
         report = self.config.report_interval
         left = int(steps)
         hook = device_hook
+        freeze_chi = freeze_chi or self.hold_chi
         pair_ok = (self.two_step and not freeze_chi and self.N <= 1024 and hook is None
                    and self._keep is None and not self._counted())
         if hook is not None and freeze_chi:

@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import math
 import time
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -414,3 +415,223 @@ class DetectorScreen:
     def __repr__(self) -> str:
         return (f"DetectorScreen(axis={self._axis}, position={self._pos}, "
                 f"field='{self._field_name}', n_frames={self.n_frames})")
+
+
+@dataclass
+class Slit:
+    """One slit opening of a :class:`Barrier` (lfm/experiment/barrier.py:71-95):
+    `center` and `width` in cells along the slit axis; a which-path detector
+    damps Psi there by `detector_strength` per slit transit."""
+
+    center: int
+    width: int = 4
+    detector: bool = False
+    detector_strength: float = 1.0
+
+
+class Barrier:
+    """A chi barrier with slit openings (lfm/experiment/barrier.py:98-427):
+    chi = `height` (default chi0 + 50) in a slab of `thickness` cells across
+    `axis` at `position`, chi0 in the slits; with `absorb` Psi is zeroed in
+    the solid cells, and a detector slit multiplies Psi by 1 - gamma per step,
+    gamma = 1 - (1 - strength)^(1 / transit_steps).  The barrier is applied
+    once on construction.
+
+    Two ways to keep it in place, with the same fields bit for bit:
+
+    * ``step_callback(sim, step)``: the reference's form, after every step:
+      all four chi buffers rewritten to the barrier's chi, Psi multiplied by
+      the absorber and detector factors through the native views;
+    * ``fold()``: chi held at the barrier's chi (every evolve freezes chi,
+      ``Evolver.hold_chi``) and the factors applied by the step kernel as it
+      writes Psi (``Evolver.set_psi_keep``); ``unfold()`` stops it.  Slits
+      whose cells overlap, where two factors would meet at one site, are
+      refused by fold().
+
+    A folded barrier applies its factors after a folded source's addition and
+    before a folded detector's record, the order of the reference's callbacks
+    in its examples."""
+
+    def __init__(self, sim, axis: int = 2, position: int | None = None,
+                 height: float | None = None, thickness: int = 2, slits=None,
+                 slit_axis: int | None = None, absorb: bool = True,
+                 transit_steps: int = 10) -> None:
+        from .config import CHI0
+        N = sim.config.grid_size
+        self._sim = sim
+        self._N = N
+        self._axis = axis
+        self._height = height if height is not None else CHI0 + 50.0
+        self._pos = position if position is not None else N // 2
+        self._thick = thickness
+        self._absorb = absorb
+        if slits is None:
+            half_sep = max(4, N // 8)
+            slit_w = max(2, N // 32)
+            slits = [Slit(center=N // 2 - half_sep, width=slit_w),
+                     Slit(center=N // 2 + half_sep, width=slit_w)]
+        self._slits = list(slits)
+        self._slit_axis = slit_axis if slit_axis is not None else (axis + 1) % 3
+        self._barrier_mask, self._slit_masks = self._build_masks()
+        start = max(0, self._pos - self._thick // 2)
+        end = min(N, start + self._thick)
+        slab = [slice(None)] * 3
+        slab[self._axis] = slice(start, end)
+        self._slab = tuple(slab)
+        self._slit_slices = []
+        for slit in self._slits:
+            s0 = max(0, slit.center - slit.width // 2)
+            s1 = min(N, s0 + slit.width)
+            sl = [slice(None)] * 3
+            sl[self._axis] = slice(start, end)
+            sl[self._slit_axis] = slice(s0, s1)
+            self._slit_slices.append(tuple(sl))
+        self._chi0 = float(sim.config.chi0)
+        # per-step Psi factors: 0 in the solid cells (absorb), 1 - gamma in
+        # detector slits, as float32 device arrays
+        td = sim._to_device
+        self._psi_mask = td((~self._barrier_mask).astype(np.float32)).reshape(N, N, N)
+        self._det_scales = []
+        for i, slit in enumerate(self._slits):
+            if slit.detector and slit.detector_strength > 0.0:
+                transit = max(1, transit_steps)
+                strength = float(min(1.0, slit.detector_strength))
+                gamma = 1.0 if strength >= 1.0 else 1.0 - (1.0 - strength) ** (1.0 / transit)
+                scale = np.ones((N, N, N), dtype=np.float32)
+                scale[self._slit_masks[i]] = float(1.0 - gamma)
+                self._det_scales.append(scale)
+            else:
+                self._det_scales.append(None)
+        self._det_scales_dev = [td(sc).reshape(N, N, N) if sc is not None else None
+                                for sc in self._det_scales]
+        self._folded = False
+        self.apply()
+
+    def _build_masks(self):
+        N, axis = self._N, self._axis
+        start = max(0, self._pos - self._thick // 2)
+        end = min(N, start + self._thick)
+        barrier_mask = np.zeros((N, N, N), dtype=bool)
+        sl = [slice(None)] * 3
+        sl[axis] = slice(start, end)
+        barrier_mask[tuple(sl)] = True
+        slit_masks = []
+        for slit in self._slits:
+            s0 = max(0, slit.center - slit.width // 2)
+            s1 = min(N, s0 + slit.width)
+            sm = np.zeros((N, N, N), dtype=bool)
+            ssl = [slice(None)] * 3
+            ssl[axis] = slice(start, end)
+            ssl[self._slit_axis] = slice(s0, s1)
+            sm[tuple(ssl)] = True
+            slit_masks.append(sm)
+            barrier_mask[tuple(ssl)] = False
+        return barrier_mask, slit_masks
+
+    def chi_template(self) -> np.ndarray:
+        """The barrier's chi field, (N, N, N) float32: chi0, the slab at
+        `height`, the slits back at chi0."""
+        chi = np.full((self._N,) * 3, self._chi0, dtype=np.float32)
+        chi[self._slab] = self._height
+        for sl in self._slit_slices:
+            chi[sl] = self._chi0
+        return chi
+
+    def keep_factors(self) -> np.ndarray:
+        """The per-site Psi factor of one enforcement, (N, N, N) float32: the
+        absorber (0 in the solid cells) times the detector scales."""
+        k = (~self._barrier_mask).astype(np.float32) if self._absorb else \
+            np.ones((self._N,) * 3, np.float32)
+        for sc in self._det_scales:
+            if sc is not None:
+                k = k * sc
+        return k
+
+    def apply(self) -> None:
+        """Re-impose the barrier: chi in all four buffers, Psi multiplied by
+        the absorber and detector factors (lfm/experiment/barrier.py:321-367)."""
+        chi0, h = self._chi0, float(self._height)
+        for buf in self._sim._native_chi_pair():
+            buf.fill(chi0)
+            buf[self._slab] = h
+            for sl in self._slit_slices:
+                buf[sl] = chi0
+        pr, pi = self._sim._native_psi_real(), self._sim._native_psi_imag()
+        if self._absorb:
+            pr *= self._psi_mask
+            if pi is not None:
+                pi *= self._psi_mask
+        for sc in self._det_scales_dev:
+            if sc is not None:
+                pr *= sc
+        if pi is not None:
+            for sc in self._det_scales_dev:
+                if sc is not None:
+                    pi *= sc
+
+    def attenuate_slits(self) -> None:
+        """No-op (the detector damping is part of apply())."""
+
+    def measure_slits(self) -> dict:
+        """Mean energy density in each slit opening."""
+        ed = self._sim.energy_density
+        return {f"slit_{i}": float(ed[sm].mean()) for i, sm in enumerate(self._slit_masks)}
+
+    def step_callback(self, sim, step: int) -> None:
+        """Simulation step callback: apply()."""
+        if self._folded:
+            raise RuntimeError("this barrier is folded into the steps (unfold() first)")
+        self.apply()
+
+    def fold(self) -> None:
+        """Keep the barrier inside the steps from now on: chi held at the
+        barrier's chi, Psi factors applied by the step kernel."""
+        n_factors = self._barrier_mask.astype(np.int8) if self._absorb else \
+            np.zeros(self._barrier_mask.shape, np.int8)
+        for i, sc in enumerate(self._det_scales):
+            if sc is not None:
+                n_factors += self._slit_masks[i].astype(np.int8)
+        if int(n_factors.max(initial=0)) > 1:
+            raise ValueError("slits overlap: two Psi factors meet at one site, so a single "
+                             "per-site factor would not round as the callback's multiplies")
+        ev = self._sim.evolver
+        ev.set_chi(self.chi_template())
+        ev.set_psi_keep(self.keep_factors())
+        ev.hold_chi = True
+        self._folded = True
+
+    def unfold(self) -> None:
+        """Stop keeping the barrier inside the steps."""
+        if self._folded:
+            ev = self._sim.evolver
+            ev.set_psi_keep(None)
+            ev.hold_chi = False
+            self._folded = False
+
+    @property
+    def mask(self) -> np.ndarray:
+        return self._barrier_mask
+
+    @property
+    def slit_masks(self) -> list:
+        return self._slit_masks
+
+    @property
+    def slits(self) -> list:
+        return self._slits
+
+    @property
+    def position(self) -> int:
+        return self._pos
+
+    @property
+    def axis(self) -> int:
+        return self._axis
+
+    @property
+    def height(self) -> float:
+        return self._height
+
+    def __repr__(self) -> str:
+        return (f"Barrier(axis={self._axis}, position={self._pos}, height={self._height:.1f}, "
+                f"thickness={self._thick}, n_slits={len(self._slits)})")

@@ -168,6 +168,35 @@ class Simulation:
         N = self.config.grid_size
         return tuple(as_device_array(t.view(N, N, N)) for t in ev.native_chi_refs())
 
+    # Experiment components (lfm/simulation.py:984-1121): the barrier, the
+    # detector screen and the continuous source, each with the reference's
+    # callback form and a fold() that moves its per-step work into the steps.
+    def place_barrier(self, axis: int = 2, position: int | None = None,
+                      height: float | None = None, thickness: int = 2, slits=None,
+                      slit_axis: int | None = None, absorb: bool = True,
+                      transit_steps: int = 10):
+        """A :class:`~lfm_hip.experiment.Barrier` attached to this simulation
+        (applied once now)."""
+        from .experiment import Barrier
+        return Barrier(self, axis=axis, position=position, height=height, thickness=thickness,
+                       slits=slits, slit_axis=slit_axis, absorb=absorb,
+                       transit_steps=transit_steps)
+
+    def add_detector(self, axis: int = 2, position: int | None = None,
+                     field: str = "energy_density"):
+        """A :class:`~lfm_hip.experiment.DetectorScreen` on this simulation."""
+        from .experiment import DetectorScreen
+        return DetectorScreen(self, axis=axis, position=position, field=field)
+
+    def add_source(self, *, axis: int = 2, position: float = 0.25, omega: float = 2.0,
+                   amplitude: float = 2.0, envelope_sigma: float = 0.25, phase: float = 0.0,
+                   boost: float = 10.0):
+        """A :class:`~lfm_hip.experiment.ContinuousSource` on this simulation."""
+        from .experiment import ContinuousSource
+        return ContinuousSource(self, axis=axis, position=position, omega=omega,
+                                amplitude=amplitude, envelope_sigma=envelope_sigma, phase=phase,
+                                boost=boost)
+
     def _to_device(self, arr):
         """A host array as a flat float32 DeviceArray (the backend's format;
         lfm/simulation.py:253-255)."""

@@ -12,7 +12,10 @@ must give the same frames bit for bit:
   buffer bounds; untouched frames keep their contents);
 * ``Simulation.run_with_snapshots`` with a source, a barrier and the screen
   as the reference's callbacks against the three folded into the steps, at
-  REAL and COMPLEX, recording every step and every 5th step.
+  REAL and COMPLEX, recording every step and every 5th step;
+* the same with the experiment classes from the Simulation factories
+  (``place_barrier`` with a which-path detector slit, ``add_source``,
+  ``add_detector``) and their ``fold()``.
 """
 import ctypes
 import sys
@@ -184,3 +187,56 @@ def test_folded_detector_equals_record(gpu, level, every):
     assert b.n_frames == 0
     b_sim.run(2 * every)
     assert b.n_frames == 2
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_folded_experiment_classes_equal_callbacks(gpu, level):
+    """The reference's double-slit set-up through the Simulation factories:
+    place_barrier (two slits, the second with a which-path detector),
+    add_source and add_detector, run with the three callbacks in the
+    reference's order (source, barrier, screen) against the three folded into
+    the steps: frames, pattern and fields equal bit for bit (48^3, 240 steps
+    in two run calls)."""
+    import torch
+    import lfm_hip as L
+    N = 48
+    log(f"Barrier + ContinuousSource + DetectorScreen folded vs callbacks, level {level}")
+    out = {}
+    for fold in (False, True):
+        cfg = (L.gravity_only, L.gravity_em)[level](N)
+        sim = L.Simulation(cfg, backend="hip")
+        rng = np.random.default_rng(3)
+        sim.set_psi_real((0.02 * rng.standard_normal((N, N, N))).astype(np.float32))
+        if level:
+            sim.set_psi_imag((0.02 * rng.standard_normal((N, N, N))).astype(np.float32))
+        barrier = sim.place_barrier(axis=2, slits=[L.Slit(center=18, width=3),
+                                                   L.Slit(center=30, width=3, detector=True,
+                                                          detector_strength=0.9)])
+        source = sim.add_source(axis=2, position=0.2, omega=21.0, phase=0.7)
+        screen = sim.add_detector(axis=2, position=40)
+        t0 = time.perf_counter()
+        if fold:
+            source.fold()
+            barrier.fold()
+            screen.fold()
+            for n in (160, 80):
+                sim.run_with_snapshots(n, snapshot_every=n, record_metrics=False)
+        else:
+            def callbacks(s, step):
+                source.step_callback(s, step)
+                barrier.step_callback(s, step)
+                screen.record()
+            for n in (160, 80):
+                sim.run_with_snapshots(n, snapshot_every=n, step_callback=callbacks,
+                                       record_metrics=False)
+        torch.cuda.synchronize()
+        log(f"  {'folded   ' if fold else 'callbacks'} {(time.perf_counter() - t0) / 240 * 1e6:.0f} "
+            f"us per step")
+        out[fold] = (screen.snapshots, screen.pattern, sim.psi_real, sim.chi,
+                     barrier.measure_slits())
+    a, b = out[False], out[True]
+    assert a[0].shape == (240, N, N)
+    for x, y, name in zip(a[:4], b[:4], ("frames", "pattern", "psi_real", "chi")):
+        np.testing.assert_array_equal(x, y, err_msg=name)
+    assert a[4] == b[4]
+    assert float(a[1].max()) > 0.0

@@ -240,6 +240,63 @@ def test_continuous_source_amounts():
         L.ContinuousSource(FakeSim(1, N), axis=3)
 
 
+def test_barrier_geometry_and_factors():
+    """Barrier (lfm/experiment/barrier.py:98-427) on a host-array simulation:
+    the default two slits, the slab, the chi template, the absorber and the
+    which-path detector factor 1 - gamma, gamma = 1 - (1 - s)^(1/transit),
+    apply() on the live buffers, and fold() refusing overlapping factors."""
+    import types
+    import lfm_hip as L
+    N = 64
+
+    class HostSim:
+        def __init__(self):
+            self.config = types.SimpleNamespace(grid_size=N, chi0=19.0)
+            self.chi = [np.full((N, N, N), 20.0, np.float32) for _ in range(4)]
+            self.pr = np.ones((N, N, N), np.float32)
+            self.evolver = types.SimpleNamespace(level=0)
+
+        def _to_device(self, arr):
+            return np.asarray(arr, np.float32).ravel()
+
+        def _native_chi_pair(self):
+            return tuple(self.chi)
+
+        def _native_psi_real(self):
+            return self.pr
+
+        def _native_psi_imag(self):
+            return None
+    sim = HostSim()
+    b = L.Barrier(sim, slits=[L.Slit(center=24, width=2),
+                              L.Slit(center=40, width=2, detector=True, detector_strength=0.95)])
+    assert (b.position, b.axis, b.height) == (32, 2, 69.0)
+    # slab k in [31, 33); slits along axis 0 at [23, 25) and [39, 41)
+    solid = np.zeros((N, N, N), bool)
+    solid[:, :, 31:33] = True
+    solid[23:25, :, 31:33] = False
+    solid[39:41, :, 31:33] = False
+    np.testing.assert_array_equal(b.mask, solid)
+    chi = b.chi_template()
+    assert set(np.unique(chi)) == {19.0, 69.0} and bool((chi[solid] == 69.0).all())
+    gamma = 1.0 - (1.0 - 0.95) ** (1.0 / 10)
+    k = b.keep_factors()
+    assert bool((k[solid] == 0.0).all()) and bool((k[39:41, :, 31:33] == np.float32(1 - gamma)).all())
+    assert bool((k[23:25, :, 31:33] == 1.0).all()) and float(k[:, :, :31].min()) == 1.0
+    # construction applied the barrier to the live buffers
+    for c in sim.chi:
+        np.testing.assert_array_equal(c, chi)
+    np.testing.assert_array_equal(sim.pr, k)
+    # default geometry: two slits max(4, N/8) either side of the centre
+    d = L.Barrier(HostSim())
+    assert [(s.center, s.width) for s in d.slits] == [(24, 2), (40, 2)]
+    # overlapping detector slits cannot fold into one factor per site
+    o = L.Barrier(HostSim(), slits=[L.Slit(center=20, width=4, detector=True, detector_strength=0.5),
+                                    L.Slit(center=21, width=4, detector=True, detector_strength=0.5)])
+    with pytest.raises(ValueError, match="overlap"):
+        o.fold()
+
+
 def test_host_mask_bit_identical_to_oracle():
     from lfm_hip.fields import boundary_mask_host
     for N, bf in ((8, 0.3), (16, 0.15), (48, 0.3), (64, 0.3)):
