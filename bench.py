@@ -661,7 +661,10 @@ def main():
     else:
         step = ev._step2 if pairs else ev._step
     per = 2 if pairs else 1
-    warm = max(per, args.warmup)
+    # whole launches: with two steps per launch an odd warm-up would end on a
+    # single step, off the two-step rotation's cycle, and the timed call
+    # would then capture its graphs
+    warm = max(per, -(-args.warmup // per) * per)
     if args.direct:
         for _ in range(warm):
             step()

@@ -96,10 +96,12 @@ class Evolver:
         # bit-identical to single steps), which needs a fourth rotation slot:
         # both outputs must differ from both inputs.  Off by default: measured
         # on MI355X it moves fewer HBM bytes (COMPLEX 512^3: 26.3 against 39.1
-        # B per site and step) but is not faster (0.86-0.90 against 0.85-0.92
-        # ms per step; slower below 128^3, where a chunk's four extra planes
-        # and the few workgroups dominate): its ring recomputation leaves it
-        # latency-bound at 15 waves per CU.
+        # B per site and step) but its ring recomputation leaves it
+        # latency-bound at 15 waves per CU, so it wins only on boxes with
+        # slower HBM (0.903 against 1.07 ms per step there; 0.86-0.92 against
+        # 0.85-0.92 elsewhere) and loses below 128^3, where a chunk's four
+        # extra planes and the few workgroups dominate.  autotune() picks
+        # between the two from 256^3 up.
         self.two_step = bool(two_step) and lvl != FieldLevel.COLOR
         ns = 4 if self.two_step else 3
         self._pr = [torch.zeros(n, dtype=f32, device=dev) for _ in range(ns)]
@@ -359,42 +361,79 @@ class Evolver:
             self._keep = t.reshape(-1).contiguous().clone()
         self._graphs.clear()
 
-    def autotune(self, reps: int = 6, chunks=(0, 1, 2, 4, 8, 16, 32, 64, 128)) -> dict:
+    def autotune(self, reps: int = 6, chunks=(0, 1, 2, 4, 8, 16, 32, 64, 128),
+                 choose_two_step: bool = True) -> dict:
         """Time every tile shape and the given plane chunks (0: the library's
         choice) of the stencil kernel on the current state and keep the
-        fastest.  The state is saved and restored, so tuning does not
-        advance the simulation.  Returns {(tile, chunk): ms per kernel}."""
+        fastest.  At REAL and COMPLEX (N <= 1024, no keep factor, source or
+        detector) from 256^3 up with `choose_two_step`, the two-step kernel's
+        chunks are timed as well and `two_step` is set to whichever kernel
+        advances a step faster: the single-step kernel is bound by the box's HBM rate,
+        the two-step kernel by its latency, so the faster one differs from
+        box to box.  With `choose_two_step` False, only the kernel `two_step`
+        selects is tuned.  The state is saved and restored, so tuning does
+        not advance the simulation.  Returns {(tile, chunk): ms per step},
+        tile 0 standing for the two-step kernel (or the fused COLOR kernel)."""
         saved = ([t.clone() for t in self._pr], [t.clone() for t in self._pi] if self._pi else None,
                  [t.clone() for t in self._chi], (self._cur, self._prev, self._damped, self._cc,
                                                   self._cp, self._oc, self._op))
         results = {}
         old = (self.tile, self.chunk, self.kernel_events)
         N = self.N
-        pairs = (self.two_step and N <= 1024 and self._keep is None and self._src is None
-                 and self._det is None)
+        pair_ok = (self.level < 2 and N <= 1024 and self._keep is None and self._src is None
+                   and self._det is None)
+        families = []
         if self._fused:
             # the fused COLOR kernel has one tile shape (a full k-row); tune its chunk
-            combos = [(0, ch) for ch in dict.fromkeys((0, N, N // 2, N // 4, N // 8)) if ch == 0 or ch >= 4]
-        elif pairs:
-            # the two-step kernel has one tile shape; tune its chunk
-            combos = [(0, ch) for ch in dict.fromkeys((0, 2, 4, 8, 16, 32, 64)) if ch <= N]
+            families.append((False, [(0, ch) for ch in dict.fromkeys((0, N, N // 2, N // 4, N // 8))
+                                     if ch == 0 or ch >= 4]))
         else:
-            # one-plane chunks only pay on small cache-resident lattices
-            # (REAL 64^3: 4.47 us per step at tile 2 / chunk 1 against 4.68)
-            combos = [(t, ch) for t in range(1, _abi.LFM_TILE_VARIANTS + 1)
-                      for ch in dict.fromkeys(min(c, N) for c in chunks) if ch != 1 or N <= 128]
-        for tile, ch in combos:
-            self.tile, self.chunk = tile, ch
-            self.kernel_events, self.kernel_event_steps = [], []
-            for _ in range(reps + 1):
-                self._step2() if pairs else self._step()
-            torch.cuda.synchronize(self.device)
-            ts = [a.elapsed_time(b) / n for (a, b), n in
-                  zip(self.kernel_events[1:], self.kernel_event_steps[1:])]
-            results[(tile, ch)] = sorted(ts)[len(ts) // 2]
+            if not (pair_ok and self.two_step and not choose_two_step):
+                # one-plane chunks only pay on small cache-resident lattices
+                # (REAL 64^3: 4.47 us per step at tile 2 / chunk 1 against 4.68)
+                families.append((False, [(t, ch) for t in range(1, _abi.LFM_TILE_VARIANTS + 1)
+                                         for ch in dict.fromkeys(min(c, N) for c in chunks)
+                                         if ch != 1 or N <= 128]))
+            # below 256^3 the kernel events of single launches include launch
+            # gaps that graph replay hides, and replayed single steps beat
+            # pairs there (REAL 64^3: 4.65 against 5.81 us per step), so the
+            # two-step kernel is a candidate only from 256^3 up
+            if pair_ok and (self.two_step or (choose_two_step and N >= 256)):
+                # the two-step kernel has one tile shape; tune its chunk
+                families.append((True, [(0, ch) for ch in dict.fromkeys((0, 2, 4, 8, 16, 32, 64))
+                                        if ch <= N]))
+        grown = False
+        if any(pairs for pairs, _ in families) and len(self._pr) < 4:
+            # the two-step kernel's rotation needs a fourth slot
+            self._pr.append(torch.zeros_like(self._pr[0]))
+            if self._pi is not None:
+                self._pi.append(torch.zeros_like(self._pi[0]))
+            self._chi.append(torch.zeros_like(self._chi[0]))
+            grown = True
+        kind = {}
+        for pairs, combos in families:
+            for tile, ch in combos:
+                self.tile, self.chunk = tile, ch
+                self.kernel_events, self.kernel_event_steps = [], []
+                for _ in range(reps + 1):
+                    self._step2() if pairs else self._step()
+                torch.cuda.synchronize(self.device)
+                ts = [a.elapsed_time(b) / n for (a, b), n in
+                      zip(self.kernel_events[1:], self.kernel_event_steps[1:])]
+                results[(tile, ch)] = sorted(ts)[len(ts) // 2]
+                kind[(tile, ch)] = pairs
         best = min(results, key=results.get)
         self.tile, self.chunk = best
+        if pair_ok and not self._fused and (self.two_step or (choose_two_step and N >= 256)):
+            self.two_step = kind[best]
         self.kernel_events = old[2]
+        if grown and not self.two_step:
+            # single steps rotate through the first three slots (the restored
+            # state below lies in them): drop the fourth again
+            self._pr.pop()
+            if self._pi is not None:
+                self._pi.pop()
+            self._chi.pop()
         for dst, src in zip(self._pr, saved[0]):
             dst.copy_(src)
         if saved[1]:

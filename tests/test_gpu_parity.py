@@ -1027,6 +1027,50 @@ def test_tile_shapes_and_chunks_equal_direct_kernel(gpu, level, N):
                 np.testing.assert_array_equal(a, b, err_msg=f"{key} {name}")
 
 
+@pytest.mark.parametrize("level", [0, 1])
+def test_autotune_chooses_kernel_without_changing_results(gpu, level):
+    """Evolver.autotune at REAL / COMPLEX 256^3 times the single-step tile
+    shapes and the two-step kernel's chunks, keeps the fastest per step
+    (setting two_step), restores the state, and the run that follows gives
+    the same bits as an untuned Evolver (the kernels and shapes agree bit for
+    bit); with choose_two_step=False only the kernel two_step names is tuned,
+    and below 256^3 only single steps are candidates."""
+    import lfm_hip as L
+    N = 256
+    log(f"autotune kernel choice, level {level}")
+    cfg = L.gravity_only(N) if level == 0 else L.gravity_em(N)
+    r, i = make_ics(level, N, seed=23)
+    out = []
+    for tune in (True, False):
+        ev = L.Evolver(cfg, backend="hip")
+        ev.set_psi_real(r)
+        if i is not None:
+            ev.set_psi_imag(i)
+        ev.evolve(3)
+        if tune:
+            res = ev.autotune(reps=3)
+            singles = [k for k in res if k[0] >= 1]
+            pairs = [k for k in res if k[0] == 0]
+            assert singles and pairs, res
+            best = min(res, key=res.get)
+            assert ev.two_step == (best[0] == 0) and (ev.tile, ev.chunk) == best
+            log(f"  best {best}: {res[best]:.4f} ms per step (two_step={ev.two_step}); best single "
+                f"{min(res[k] for k in singles):.4f}, best pair {min(res[k] for k in pairs):.4f}")
+        ev.evolve(37)
+        out.append((ev.get_psi_real(), ev.get_chi_storage()[0],
+                    ev.get_psi_imag() if i is not None else None))
+    for a, b in zip(*out):
+        if a is not None:
+            np.testing.assert_array_equal(a, b)
+    ev = L.Evolver(cfg, backend="hip", two_step=True)
+    ev.set_psi_real(r)
+    res = ev.autotune(reps=2, choose_two_step=False)
+    assert all(k[0] == 0 for k in res) and ev.two_step
+    small = L.Evolver((L.gravity_only if level == 0 else L.gravity_em)(64), backend="hip")
+    res = small.autotune(reps=2, chunks=(0, 8))
+    assert all(k[0] >= 1 for k in res) and not small.two_step
+
+
 @pytest.mark.parametrize("N", [16, 64, 128])
 def test_scv_term_path_equals_three_field_path(gpu, N):
     """COLOR full_physics through the separate transform passes: the default
