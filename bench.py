@@ -278,6 +278,8 @@ def run_batch(args):
 
     reset_batch()
     tuning = batch.autotune() if args.tile == "auto" else None
+    # the tuner may pick the two-step kernel (or single steps) for this box
+    two = batch.two_step
     # untimed runs that start at every slot-rotation phase capture the graphs
     # the timed run replays; then the same initial state again
     for n in WARMUP_CALLS[:3]:
@@ -295,12 +297,12 @@ def run_batch(args):
     for _ in range(12):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        batch._step2() if args.two_step else batch._step()
+        batch._step2() if two else batch._step()
         e1.record()
         evs.append((e0, e1))
     torch.cuda.synchronize()
     # per step (a two-step launch advances every lattice twice)
-    kms = sorted(a.elapsed_time(b) for a, b in evs[2:])[len(evs[2:]) // 2] / (2 if args.two_step else 1)
+    kms = sorted(a.elapsed_time(b) for a, b in evs[2:])[len(evs[2:]) // 2] / (2 if two else 1)
     ok = all(bool(torch.isfinite(t).all()) for t in batch._chi)
     del batch
     torch.cuda.empty_cache()
@@ -333,7 +335,7 @@ def run_batch(args):
         torch.cuda.empty_cache()
     # COMPLEX with mask: 7 reads + 3 writes of float32 per step; two steps per
     # launch read the 7 once and write both steps' 3 (26 B per step)
-    b_alg = 26 if args.two_step else 40
+    b_alg = 26 if two else 40
     achieved = b_alg * sites / (kms * 1e-3) / 1e9
     out = {"metric": METRIC, "value": nlat / t_batch, "unit": "lattices/s", "n_gpus": 1,
            "steps": steps, "warmup": args.warmup, "ms_per_step": t_batch * 1e3 / steps,
@@ -341,7 +343,7 @@ def run_batch(args):
            "data": "synthetic",
            "config": {"workload": "batched sweep: 64 COMPLEX lattices, N in {32,48,64,96,128}, "
                                   f"{steps} steps, mask on", "lattices": nlat, "sites": sites,
-                      "steps_per_launch": 2 if args.two_step else 1},
+                      "steps_per_launch": 2 if two else 1},
            "batched": {"lattices_per_s": nlat / t_batch, "GLUPS": sites * steps / t_batch / 1e9,
                        "wall_s": t_batch, "tile": batch_tile(tuning)},
            "sequential_graphs": {"lattices_per_s": nlat / t_seq[True],
@@ -354,7 +356,7 @@ def run_batch(args):
            "speedup_vs_sequential": min(t_seq.values()) / t_batch,
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                        "kernel": "step2_tiled_batch<COMPLEX>" if args.two_step else "step_tiled_batch<COMPLEX>",
+                        "kernel": "step2_tiled_batch<COMPLEX>" if two else "step_tiled_batch<COMPLEX>",
                         "kernel_ms": kms,
                         "bytes_per_site": b_alg},
            "finite": ok, "batch_equals_sequential_lattice0": same}
@@ -567,10 +569,14 @@ def tune_key(tile, chunk):
 
 
 def batch_tile(tuning):
+    """LatticeBatch.autotune's timings; a negative tile is the two-step
+    kernel at that tile shape."""
     if not tuning:
         return None
     best = min(tuning, key=tuning.get)
-    return {"best": list(best), "ms": {tune_key(t, c): round(v, 4) for (t, c), v in tuning.items()}}
+    label = lambda t, c: f"2step {-t}/{c}" if t < 0 else f"{t}/{c}"
+    return {"best": list(best), "two_step": best[0] < 0,
+            "ms": {label(t, c): round(v, 4) for (t, c), v in tuning.items()}}
 
 
 def main():

@@ -508,46 +508,76 @@ class LatticeBatch:
         self._graphs[key] = g
         return g
 
-    def autotune(self, steps: int = 6, tiles=None, chunks=None) -> dict:
+    def autotune(self, steps: int = 6, tiles=None, chunks=None,
+                 choose_two_step: bool = True) -> dict:
         """Time tile shapes x plane chunks of the batched kernel on the
-        current state (restored afterwards) and keep the fastest (two_step:
-        tile variants 2 and 4, pairs of steps)."""
-        if self.two_step:
-            tiles = tiles or (2, 4)
-            chunks = chunks or (16, 32, 64)
-        else:
-            tiles = tiles or range(1, _abi.LFM_TILE_VARIANTS + 1)
-            chunks = chunks or (8, 16, 32)
+        current state (restored afterwards) and keep the fastest.  REAL and
+        COMPLEX batches with `choose_two_step` also time the two-step kernel
+        (tile variants 2 and 4) and set `two_step` to whichever advances a
+        step faster (the single-step kernel is bound by the box's HBM rate,
+        the two-step kernel by its latency); with it False only the kernel
+        `two_step` names is tuned.  Returns {(tile, chunk): ms per step}, a
+        negative tile standing for the two-step kernel at that tile shape."""
+        fams = []
+        if not self.two_step or choose_two_step:
+            fams.append((False, tuple(tiles or range(1, _abi.LFM_TILE_VARIANTS + 1)),
+                         tuple(chunks or (8, 16, 32))))
+        if self.level < 2 and (self.two_step or choose_two_step):
+            fams.append((True, tuple(t for t in (tiles or (2, 4)) if t in (2, 4)) or (2, 4),
+                         tuple(chunks or (16, 32, 64))))
+        wanted = self.two_step
+        grown = False
+        if any(pair for pair, _, _ in fams) and len(self._pr) < 4:
+            # the two-step rotation needs a fourth slot
+            self._pr.append(torch.zeros_like(self._pr[0]))
+            if self._pi is not None:
+                self._pi.append(torch.zeros_like(self._pi[0]))
+            self._chi.append(torch.zeros_like(self._chi[0]))
+            self._ns = 4
+            self._set_groups(self.group_bytes)
+            grown = True
         snap = ([t.clone() for t in self._pr], [t.clone() for t in self._pi] if self._pi else None,
                 [t.clone() for t in self._chi], self._state_key())
         res = {}
-        for tile, ch in itertools.product(tiles, chunks):
-            self._make_plan(tile, ch)
-            self._step2() if self.two_step else self._step()
-            per = 4 if self.two_step else 3
-            reps = max(1, -(-steps // per))
-            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            ev0.record()
-            for _ in range(reps):
-                if self.two_step:
-                    self._step2()
-                    self._step2()
-                else:
-                    self._cycle()
-            ev1.record()
-            torch.cuda.synchronize(self.device)
-            res[(tile, ch)] = ev0.elapsed_time(ev1) / (per * reps)
-            for dst, src in zip(self._pr, snap[0]):
-                dst.copy_(src)
-            if snap[1]:
-                for dst, src in zip(self._pi, snap[1]):
+        for pair, ftiles, fchunks in fams:
+            self.two_step = pair
+            for tile, ch in itertools.product(ftiles, fchunks):
+                self._make_plan(tile, ch)
+                self._step2() if pair else self._step()
+                per = 4 if pair else 3
+                reps = max(1, -(-steps // per))
+                ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                ev0.record()
+                for _ in range(reps):
+                    if pair:
+                        self._step2()
+                        self._step2()
+                    else:
+                        self._cycle()
+                ev1.record()
+                torch.cuda.synchronize(self.device)
+                res[(-tile if pair else tile, ch)] = ev0.elapsed_time(ev1) / (per * reps)
+                for dst, src in zip(self._pr, snap[0]):
                     dst.copy_(src)
-            for dst, src in zip(self._chi, snap[2]):
-                dst.copy_(src)
-            self._cur, self._prev, self._chi_cur, self._chi_prev, self._damped = snap[3]
-            self._psisq_ok = [False] * self.nlat    # the |Psi'|^2 arena holds tuning steps
-        best = min(res, key=res.get)
-        self._make_plan(*best)
+                if snap[1]:
+                    for dst, src in zip(self._pi, snap[1]):
+                        dst.copy_(src)
+                for dst, src in zip(self._chi, snap[2]):
+                    dst.copy_(src)
+                self._cur, self._prev, self._chi_cur, self._chi_prev, self._damped = snap[3]
+                self._psisq_ok = [False] * self.nlat    # the |Psi'|^2 arena holds tuning steps
+        best = min(res, key=res.get) if res else (self.tile, self.chunk)
+        self.two_step = best[0] < 0 if res else wanted
+        if grown and not self.two_step:
+            # single steps rotate through the first three slots, where the
+            # restored state lies: drop the fourth again
+            self._pr.pop()
+            if self._pi is not None:
+                self._pi.pop()
+            self._chi.pop()
+            self._ns = 3
+            self._set_groups(self.group_bytes)
+        self._make_plan(abs(best[0]), best[1])
         return res
 
     # ----------------------------------------------------------- analysis

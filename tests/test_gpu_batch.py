@@ -264,3 +264,45 @@ def test_two_step_batch_equals_single_step_batch(gpu, level, group_bytes, cycles
     used = [k for k, v in batches[True]._graphs.items() if v is not None]
     log(f"  two-step graphs captured: {len(used)}")
     assert used
+
+
+@pytest.mark.parametrize("group_bytes", [None, 1 << 22])
+def test_batch_autotune_kernel_choice_keeps_results(gpu, group_bytes):
+    """LatticeBatch.autotune times the single-step tile shapes and the
+    two-step kernel (negative tiles), sets two_step to the faster, restores
+    the state, and the run that follows equals an untuned one-step batch bit
+    for bit, whichever kernel it picked; with choose_two_step=False a
+    two-step batch tunes only its own kernel."""
+    import torch
+    import lfm_hip as L
+    sizes = [24, 40, 16, 72, 33, 20]
+    log(f"batch autotune kernel choice, groups {group_bytes}")
+    rng = np.random.default_rng(57)
+    cfgs = _configs(1, sizes, rng)
+    ics = [_ics(1, N, rng) for N in sizes]
+    batches = []
+    for tune in (True, False):
+        bt = L.LatticeBatch(cfgs, backend="hip", group_bytes=group_bytes)
+        for b, (r, i) in enumerate(ics):
+            bt.set_psi_real(b, r)
+            bt.set_psi_imag(b, i)
+        bt.evolve(5)
+        if tune:
+            res = bt.autotune(steps=4)
+            best = min(res, key=res.get)
+            assert any(k[0] < 0 for k in res) and any(k[0] > 0 for k in res)
+            assert bt.two_step == (best[0] < 0) and (bt.tile, bt.chunk) == (abs(best[0]), best[1])
+            assert len(bt._pr) == (4 if bt.two_step else 3)
+            log(f"  best {best}: {res[best]:.4f} ms per step (two_step={bt.two_step})")
+        batches.append(bt)
+    for n in (9, 24):
+        for bt in batches:
+            bt.evolve(n)
+        torch.cuda.synchronize()
+        for b in range(len(sizes)):
+            np.testing.assert_array_equal(batches[0].get_psi_real(b), batches[1].get_psi_real(b))
+            np.testing.assert_array_equal(batches[0].get_psi_imag(b), batches[1].get_psi_imag(b))
+            np.testing.assert_array_equal(batches[0].get_chi(b), batches[1].get_chi(b))
+    two = L.LatticeBatch(cfgs, backend="hip", two_step=True)
+    res = two.autotune(steps=4, choose_two_step=False)
+    assert all(k[0] < 0 for k in res) and two.two_step
