@@ -732,7 +732,8 @@ int choose_chunk(int N, int tiles, int min_ch = 8, int target = 8192) {
     // several tiles in flight, with at least min_ch planes per chunk: 8 when
     // the step streams from HBM (the two halo planes a chunk re-reads cost at
     // most a quarter of its traffic), fewer when the lattice's fields stay in
-    // the last-level cache and a small lattice needs the extra workgroups.
+    // the last-level cache and a small lattice needs the extra workgroups
+    // (the caller picks min_ch).
     const int nch = (target + tiles - 1) / tiles;
     int ch = (N + nch - 1) / nch;
     if (ch < min_ch) ch = min_ch;
@@ -746,7 +747,11 @@ int launch_tiled(const Ptrs& P, const KParams& kp, hipStream_t st) {
     const int nkt = (N + TK - 1) / TK;
     const int njt = (N + TJ - 1) / TJ;
     const int span = kp.ie - kp.ib;       // planes computed
-    const int CH = kp.chunk > 0 ? min(kp.chunk, span) : choose_chunk(span, nkt * njt, NT ? 8 : 2);
+    // cache-resident lattices: 2-plane chunks up to 64^3 (the few tiles need
+    // the workgroups), 4 from 96^3 up (COLOR 128^3: 44.9 against 49.9 us,
+    // COMPLEX 160^3: 30.2 against 32.1 us at tile 1)
+    const int min_ch = NT ? 8 : (N >= 96 ? 4 : 2);
+    const int CH = kp.chunk > 0 ? min(kp.chunk, span) : choose_chunk(span, nkt * njt, min_ch);
     const int nch = (span + CH - 1) / CH;
     const long long nblocks = (long long)nkt * njt * nch;
     hipLaunchKernelGGL((step_tiled<L, BJ, RJ, NT, EXTRA>), dim3((unsigned)nblocks), dim3(64 * BJ), 0,
