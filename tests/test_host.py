@@ -240,6 +240,20 @@ def test_continuous_source_amounts():
         L.ContinuousSource(FakeSim(1, N), axis=3)
 
 
+def test_integration_stub_matches_params_layout():
+    """The ctypes stub that INTEGRATION.md shows a maintainer lists the
+    fields of lfm_params_t in the order the library's own binding does."""
+    from lfm_hip import _abi
+    text = open(os.path.join(os.path.dirname(HEADER), "..", "INTEGRATION.md")).read()
+    block = text[text.index("class LfmParams(ctypes.Structure)"):]
+    block = block[:block.index("lib.lfm_step_real_f32.argtypes")]
+    names = re.findall(r'\("([A-Za-z_0-9]+)", (?:i32|i64|vp)\)', block)
+    floats = re.search(r'for n in \(([^)]*)\)', block).group(1)
+    floats = re.findall(r'"([a-z_0-9]+)"', floats)
+    stub = names[:2] + floats + names[2:]
+    assert stub == [n for n, _ in _abi.LfmParams._fields_], stub
+
+
 def test_barrier_geometry_and_factors():
     """Barrier (lfm/experiment/barrier.py:98-427) on a host-array simulation:
     the default two slits, the slab, the chi template, the absorber and the
